@@ -1,0 +1,133 @@
+"""Host-side calls into libfpt_hip.so, mirroring how the reference's Rust host
+drives its bridge (src/main.rs:912-963 `execute_metal_render_internal`,
+src/main.rs:3254-3270 field sampling).  Results come back as numpy arrays.
+Every call goes through the C ABI; a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import ffi
+from .ffi import FptRenderConfig
+
+
+@dataclass
+class RenderResult:
+    linear: np.ndarray            # (H, W, 4) float32 running-mean accumulation
+    elapsed_ms: float
+    build_ms: float
+    profile: dict = field(default_factory=dict)
+    voxel: dict = field(default_factory=dict)
+
+
+def render_full(cfg: FptRenderConfig, output_png: str | None = None, profile: bool = False) -> RenderResult:
+    """fpt_metal_render with every optional output requested."""
+    lib = ffi.load_library()
+    n = cfg.width * cfg.height * 4
+    out = np.zeros(n, np.float32)
+    build_ms = C.c_double(0)
+    elapsed = C.c_double(0)
+    vbytes = C.c_uint64(0)
+    vbricks = C.c_uint32(0)
+    vcells = C.c_uint64(0)
+    vrej = C.c_uint32(0)
+    prof = ffi.FptSdfProfileStats()
+    err = ffi.error_buffer()
+    rc = lib.fpt_metal_render(None, None, None, output_png.encode() if output_png else None, None, 0,
+                              C.byref(cfg), C.byref(build_ms), C.byref(elapsed), C.byref(vbytes),
+                              C.byref(vbricks), C.byref(vcells), C.byref(vrej), None, None, None, None,
+                              C.byref(prof) if profile else None,
+                              out.ctypes.data_as(C.POINTER(C.c_float)), n, err, len(err))
+    ffi.check(rc, err)
+    pdict = {}
+    if profile:
+        pdict = {k: getattr(prof, k) for k, _ in ffi.FptSdfProfileStats._fields_[:13]}
+    return RenderResult(out.reshape(cfg.height, cfg.width, 4), elapsed.value, build_ms.value, pdict,
+                        dict(memory_bytes=vbytes.value, active_bricks=vbricks.value,
+                             active_cells=vcells.value))
+
+
+def render(cfg: FptRenderConfig) -> np.ndarray:
+    return render_full(cfg).linear
+
+
+def trace_primary(cfg: FptRenderConfig) -> dict:
+    lib = ffi.load_library()
+    n = cfg.width * cfg.height
+    hits = (ffi.FptRayHit * n)()
+    err = ffi.error_buffer()
+    ffi.check(lib.fpt_hip_trace_primary(C.byref(cfg), hits, n, err, len(err)), err)
+    a = np.frombuffer(hits, dtype=np.dtype([("t", "f4"), ("position", "f4", 3), ("normal", "f4", 3),
+                                            ("hit", "u4"), ("steps", "u4"), ("_pad0", "u4")]))
+    a = a.reshape(cfg.height, cfg.width)
+    return {k: np.array(a[k]) for k in ("t", "position", "normal", "hit", "steps")}
+
+
+def sample_field(cfg: FptRenderConfig, points) -> np.ndarray:
+    """fpt_mandelbulber_sample_field: (N, 4) {distance, radius, derivative, iterations}."""
+    lib = ffi.load_library()
+    p = np.zeros((len(points), 4), np.float32)
+    p[:, :3] = np.asarray(points, np.float32).reshape(-1, 3)
+    out = (ffi.FptMandelbulberFieldSample * len(p))()
+    err = ffi.error_buffer()
+    ffi.check(lib.fpt_mandelbulber_sample_field(None, None, 0, C.byref(cfg),
+                                                p.ctypes.data_as(C.POINTER(C.c_float)), len(p), out,
+                                                err, len(err)), err)
+    return np.frombuffer(out, np.float32).reshape(-1, 4).copy()
+
+
+def program_surface(cfg: FptRenderConfig, points) -> np.ndarray:
+    """fpt_hip_program_surface: (N, 4) {distance, gx, gy, gz}."""
+    lib = ffi.load_library()
+    lib.fpt_hip_program_surface.argtypes = [C.POINTER(FptRenderConfig), C.POINTER(C.c_float), C.c_size_t,
+                                            C.POINTER(C.c_float), C.c_char_p, C.c_size_t]
+    p = np.zeros((len(points), 4), np.float32)
+    p[:, :3] = np.asarray(points, np.float32).reshape(-1, 3)
+    out = np.zeros((len(p), 4), np.float32)
+    err = ffi.error_buffer()
+    ffi.check(lib.fpt_hip_program_surface(C.byref(cfg), p.ctypes.data_as(C.POINTER(C.c_float)), len(p),
+                                          out.ctypes.data_as(C.POINTER(C.c_float)), err, len(err)), err)
+    return out
+
+
+def device_name() -> str:
+    lib = ffi.load_library()
+    buf = C.create_string_buffer(256)
+    lib.fpt_metal_device_name(buf, len(buf))
+    return buf.value.decode()
+
+
+class Plan:
+    """Device-resident render plan (fpt_hip_plan_*): upload once, render many."""
+
+    def __init__(self, cfg: FptRenderConfig):
+        self.lib = ffi.load_library()
+        self.cfg = cfg
+        self.handle = C.c_void_p()
+        err = ffi.error_buffer()
+        ffi.check(self.lib.fpt_hip_plan_create(C.byref(cfg), C.byref(self.handle), err, len(err)), err)
+
+    def render_into(self, device_ptr: int, stream_ptr: int = 0) -> None:
+        err = ffi.error_buffer()
+        ffi.check(self.lib.fpt_hip_plan_render(self.handle, C.c_void_p(device_ptr), C.c_void_p(stream_ptr),
+                                               err, len(err)), err)
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.fpt_hip_plan_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

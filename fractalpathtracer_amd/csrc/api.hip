@@ -1,0 +1,477 @@
+// C-ABI layer of libfpt_hip.so: the reference's entry points
+// (src/metal_bridge.h:404-486, behaviour of src/metal_bridge.mm) and the new
+// fpt_hip_* entry points declared in include/fpt_abi.h.
+//
+// Every call is synchronous on its own HIP stream, allocates what it needs
+// and frees it before returning; the caller owns every pointer
+// (metal_bridge.mm:4298-5863 ownership rules).
+#include <hip/hip_runtime.h>
+#include <zlib.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fpt_abi.h"
+#include "fpt_internal.h"
+#include "voxel_api.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+void set_error(char *error, size_t error_len, const char *fmt, ...) {
+    char buf[1024];
+    va_list args;
+    va_start(args, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, args);
+    va_end(args);
+    g_last_error = buf;
+    if (error && error_len) {
+        std::snprintf(error, error_len, "%s", buf);
+    }
+}
+
+#define HIP_TRY(expr, what)                                                                     \
+    do {                                                                                        \
+        hipError_t _e = (expr);                                                                 \
+        if (_e != hipSuccess) {                                                                 \
+            set_error(error, error_len, "%s failed: %s", what, hipGetErrorString(_e));          \
+            return 1;                                                                           \
+        }                                                                                       \
+    } while (0)
+
+// Device buffer owned for the duration of one call.
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(size_t n) { return hipMalloc(&p, n ? n : 1); }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+struct Stream {
+    hipStream_t s = nullptr;
+    ~Stream() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+int check_device(char *error, size_t error_len) {
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) {
+        set_error(error, error_len, "no HIP device is available (%s)", hipGetErrorString(e));
+        return 1;
+    }
+    return 0;
+}
+
+uint32_t clamp_samples(uint32_t s) { return std::min<uint32_t>(std::max<uint32_t>(s, 1u), 512u); }
+
+// Reference accumulation policy (metal_bridge.mm:4531-4564): all samples in
+// one dispatch when spp <= 16, else chunks of sdf_chunk_samples (default 8).
+// The running mean makes every policy produce the same image.
+uint32_t chunk_for(const FptRenderConfig &cfg) {
+    const uint32_t spp = clamp_samples(cfg.samples);
+    if (cfg.sdf_accumulation_mode == FPT_SDF_ACCUMULATION_PER_SAMPLE) return 1u;
+    if (cfg.sdf_accumulation_mode == FPT_SDF_ACCUMULATION_BATCH) return spp;
+    if (cfg.sdf_accumulation_mode == FPT_SDF_ACCUMULATION_AUTO && spp <= 16u) return spp;
+    uint32_t c = cfg.sdf_chunk_samples ? cfg.sdf_chunk_samples : 8u;
+    return std::min<uint32_t>(std::max<uint32_t>(c, 1u), 64u);
+}
+
+int validate_config(const FptRenderConfig *config, char *error, size_t error_len) {
+    if (!config) {
+        set_error(error, error_len, "missing render config");
+        return 1;
+    }
+    if (config->width == 0 || config->height == 0) {
+        set_error(error, error_len, "invalid render size %ux%u", config->width, config->height);
+        return 1;
+    }
+    if (config->width > 16384 || config->height > 16384) {
+        set_error(error, error_len, "render size %ux%u exceeds 16384x16384", config->width, config->height);
+        return 1;
+    }
+    if (config->renderer_backend != FPT_RENDERER_SDF && config->renderer_backend != FPT_RENDERER_VOXEL) {
+        set_error(error, error_len, "renderer backend %u is not supported by this library", config->renderer_backend);
+        return 1;
+    }
+    if (config->sdf_id == FPT_SDF_MANDELBULBER) {
+        set_error(error, error_len,
+                  "sdf_id MANDELBULBER needs generated Mandelbulber2 formula code; use the typed program "
+                  "fractal opcodes instead");
+        return 1;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// PNG writer (RGBA8, zlib).
+// ---------------------------------------------------------------------------
+void put_be32(std::vector<uint8_t> &v, uint32_t x) {
+    v.push_back(uint8_t(x >> 24)); v.push_back(uint8_t(x >> 16)); v.push_back(uint8_t(x >> 8)); v.push_back(uint8_t(x));
+}
+void put_chunk(std::vector<uint8_t> &out, const char *type, const std::vector<uint8_t> &data) {
+    put_be32(out, uint32_t(data.size()));
+    size_t start = out.size();
+    out.insert(out.end(), type, type + 4);
+    out.insert(out.end(), data.begin(), data.end());
+    uLong crc = crc32(0L, out.data() + start, uInt(out.size() - start));
+    put_be32(out, uint32_t(crc));
+}
+
+bool write_png(const char *path, uint32_t w, uint32_t h, const std::vector<uint8_t> &rgba, char *error,
+               size_t error_len) {
+    std::vector<uint8_t> raw;
+    raw.reserve(size_t(h) * (size_t(w) * 4 + 1));
+    for (uint32_t y = 0; y < h; ++y) {
+        raw.push_back(0);
+        raw.insert(raw.end(), rgba.begin() + size_t(y) * w * 4, rgba.begin() + size_t(y + 1) * w * 4);
+    }
+    uLongf zlen = compressBound(uLong(raw.size()));
+    std::vector<uint8_t> z(zlen);
+    if (compress2(z.data(), &zlen, raw.data(), uLong(raw.size()), 6) != Z_OK) {
+        set_error(error, error_len, "PNG compression failed");
+        return false;
+    }
+    z.resize(zlen);
+    std::vector<uint8_t> png = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1a, '\n'};
+    std::vector<uint8_t> ihdr;
+    put_be32(ihdr, w);
+    put_be32(ihdr, h);
+    ihdr.insert(ihdr.end(), {8, 6, 0, 0, 0});
+    put_chunk(png, "IHDR", ihdr);
+    put_chunk(png, "IDAT", z);
+    put_chunk(png, "IEND", {});
+    FILE *f = std::fopen(path, "wb");
+    if (!f) {
+        set_error(error, error_len, "cannot open %s for writing", path);
+        return false;
+    }
+    size_t n = std::fwrite(png.data(), 1, png.size(), f);
+    std::fclose(f);
+    if (n != png.size()) {
+        set_error(error, error_len, "short write to %s", path);
+        return false;
+    }
+    return true;
+}
+
+// metal_bridge.mm:2302-2313: distinct colours among the first 2048 found.
+size_t unique_colour_sample(const std::vector<uint8_t> &rgba) {
+    std::vector<uint32_t> colours;
+    for (size_t i = 0; i + 3 < rgba.size(); i += 4) {
+        uint32_t c;
+        std::memcpy(&c, &rgba[i], 4);
+        if (std::find(colours.begin(), colours.end(), c) == colours.end()) {
+            colours.push_back(c);
+            if (colours.size() >= 2048) break;
+        }
+    }
+    return colours.size();
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Render plan.
+// ---------------------------------------------------------------------------
+struct FptHipPlan {
+    FptRenderConfig host;
+    FptRenderConfig *dcfg = nullptr;
+    fpt::VoxelStore voxels;  // populated for the voxel backend
+    double build_ms = 0.0;
+};
+
+extern "C" {
+
+const char *fpt_hip_last_error(void) { return g_last_error.c_str(); }
+
+int fpt_hip_plan_create(const FptRenderConfig *config, FptHipPlan **plan, char *error, size_t error_len) {
+    if (!plan) {
+        set_error(error, error_len, "missing plan out-pointer");
+        return 1;
+    }
+    *plan = nullptr;
+    if (validate_config(config, error, error_len)) return 1;
+    if (check_device(error, error_len)) return 1;
+    FptHipPlan *p = new FptHipPlan();
+    p->host = *config;
+    hipError_t e = hipMalloc(&p->dcfg, sizeof(FptRenderConfig));
+    if (e == hipSuccess) e = hipMemcpy(p->dcfg, config, sizeof(FptRenderConfig), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        set_error(error, error_len, "config upload failed: %s", hipGetErrorString(e));
+        fpt_hip_plan_destroy(p);
+        return 1;
+    }
+    if (config->renderer_backend == FPT_RENDERER_VOXEL) {
+        auto t0 = std::chrono::steady_clock::now();
+        if (fpt::voxel_build(*config, p->dcfg, p->voxels, nullptr, error, error_len)) {
+            g_last_error = error && error_len ? error : "voxel build failed";
+            fpt_hip_plan_destroy(p);
+            return 1;
+        }
+        p->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    *plan = p;
+    return 0;
+}
+
+int fpt_hip_plan_render(FptHipPlan *plan, float *device_accum, void *stream, char *error, size_t error_len) {
+    if (!plan || !device_accum) {
+        set_error(error, error_len, "missing plan or accumulation buffer");
+        return 1;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const FptRenderConfig &c = plan->host;
+    const uint32_t spp = clamp_samples(c.samples);
+    const uint32_t chunk = chunk_for(c);
+    float4 *acc = reinterpret_cast<float4 *>(device_accum);
+    for (uint32_t start = 0; start < spp; start += chunk) {
+        hipError_t e = c.renderer_backend == FPT_RENDERER_VOXEL
+                           ? fpt::launch_voxel_accumulate(plan->dcfg, c, plan->voxels, acc, start, chunk, s)
+                           : fpt::launch_sdf_accumulate(plan->dcfg, c, acc, start, chunk, nullptr, s);
+        HIP_TRY(e, "render kernel launch");
+    }
+    return 0;
+}
+
+void fpt_hip_plan_destroy(FptHipPlan *plan) {
+    if (!plan) return;
+    if (plan->dcfg) (void)hipFree(plan->dcfg);
+    fpt::voxel_free(plan->voxels);
+    delete plan;
+}
+
+int fpt_metal_device_name(char *name, size_t name_len) {
+    if (!name || name_len == 0) return 1;
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        std::snprintf(name, name_len, "%s", "no HIP device");
+        return 1;
+    }
+    std::snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    return 0;
+}
+
+int fpt_metal_render(const char *metallib_path, const char *stitch_metallib_path, const char *stitch_archive_path,
+                     const char *output_path, const char *shader_source, size_t shader_source_len,
+                     const FptRenderConfig *config, double *build_ms, double *elapsed_ms,
+                     uint64_t *voxel_memory_bytes, uint32_t *voxel_active_bricks, uint64_t *voxel_active_cells,
+                     uint32_t *voxel_rejected_bricks, FptBoundGridStats *bound_grid_stats,
+                     uint32_t *stitch_cache_status, FptStitchValidationStats *stitch_validation_stats,
+                     FptStitchPipelineStats *stitch_pipeline_stats, FptSdfProfileStats *sdf_profile_stats,
+                     float *linear_output, size_t linear_output_len, char *error, size_t error_len) {
+    (void)metallib_path; (void)stitch_metallib_path; (void)stitch_archive_path;
+    (void)shader_source; (void)shader_source_len;
+    if (validate_config(config, error, error_len)) return 1;
+    if (build_ms) *build_ms = 0.0;
+    if (elapsed_ms) *elapsed_ms = 0.0;
+    if (voxel_memory_bytes) *voxel_memory_bytes = 0;
+    if (voxel_active_bricks) *voxel_active_bricks = 0;
+    if (voxel_active_cells) *voxel_active_cells = 0;
+    if (voxel_rejected_bricks) *voxel_rejected_bricks = 0;
+    if (bound_grid_stats) std::memset(bound_grid_stats, 0, sizeof(*bound_grid_stats));
+    if (stitch_cache_status) *stitch_cache_status = 0;
+    if (stitch_validation_stats) std::memset(stitch_validation_stats, 0, sizeof(*stitch_validation_stats));
+    if (stitch_pipeline_stats) std::memset(stitch_pipeline_stats, 0, sizeof(*stitch_pipeline_stats));
+    if (sdf_profile_stats) std::memset(sdf_profile_stats, 0, sizeof(*sdf_profile_stats));
+    const size_t pixels = size_t(config->width) * config->height;
+    if (linear_output && linear_output_len < pixels * 4) {  // metal_bridge.mm:4901-4906
+        set_error(error, error_len, "linear output buffer is too small (%zu < %zu)", linear_output_len, pixels * 4);
+        return 1;
+    }
+    FptHipPlan *plan = nullptr;
+    if (fpt_hip_plan_create(config, &plan, error, error_len)) return 1;
+    struct PlanGuard {
+        FptHipPlan *p;
+        ~PlanGuard() { fpt_hip_plan_destroy(p); }
+    } guard{plan};
+    if (build_ms) *build_ms = plan->build_ms;
+    if (config->renderer_backend == FPT_RENDERER_VOXEL) {
+        if (voxel_memory_bytes) *voxel_memory_bytes = plan->voxels.memory_bytes;
+        if (voxel_active_bricks) *voxel_active_bricks = plan->voxels.active_bricks;
+        if (voxel_active_cells) *voxel_active_cells = plan->voxels.active_cells;
+    }
+    Stream st;
+    HIP_TRY(hipStreamCreate(&st.s), "hipStreamCreate");
+    DevBuf accum;
+    HIP_TRY(accum.alloc(pixels * sizeof(float4)), "accumulation buffer allocation");
+    hipEvent_t ev0, ev1;
+    HIP_TRY(hipEventCreate(&ev0), "hipEventCreate");
+    HIP_TRY(hipEventCreate(&ev1), "hipEventCreate");
+    HIP_TRY(hipEventRecord(ev0, st.s), "hipEventRecord");
+    if (fpt_hip_plan_render(plan, accum.as<float>(), st.s, error, error_len)) return 1;
+    HIP_TRY(hipEventRecord(ev1, st.s), "hipEventRecord");
+    HIP_TRY(hipStreamSynchronize(st.s), "render");
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1), "hipEventElapsedTime");
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    if (elapsed_ms) *elapsed_ms = ms;
+    if (linear_output) {
+        HIP_TRY(hipMemcpy(linear_output, accum.p, pixels * sizeof(float4), hipMemcpyDeviceToHost), "linear readback");
+    }
+    if (sdf_profile_stats && config->renderer_backend == FPT_RENDERER_SDF) {
+        // Counting pass (untimed), every pixel at stride 1.
+        DevBuf prof, acc2;
+        HIP_TRY(prof.alloc(sizeof(fpt::DeviceProfile)), "profile buffer");
+        HIP_TRY(acc2.alloc(pixels * sizeof(float4)), "profile accumulation buffer");
+        HIP_TRY(hipMemsetAsync(prof.p, 0, sizeof(fpt::DeviceProfile), st.s), "profile clear");
+        const uint32_t spp = clamp_samples(config->samples), chunk = chunk_for(*config);
+        for (uint32_t s0 = 0; s0 < spp; s0 += chunk)
+            HIP_TRY(fpt::launch_sdf_accumulate(plan->dcfg, *config, acc2.as<float4>(), s0, chunk,
+                                               prof.as<fpt::DeviceProfile>(), st.s),
+                    "profile kernel launch");
+        fpt::DeviceProfile hp;
+        HIP_TRY(hipMemcpyAsync(&hp, prof.p, sizeof(hp), hipMemcpyDeviceToHost, st.s), "profile readback");
+        HIP_TRY(hipStreamSynchronize(st.s), "profile pass");
+        sdf_profile_stats->primary_steps = hp.primary_steps;
+        sdf_profile_stats->secondary_steps = hp.secondary_steps;
+        sdf_profile_stats->shadow_steps = hp.shadow_steps;
+        sdf_profile_stats->normal_evals = hp.normal_evals;
+        sdf_profile_stats->bounces = hp.bounces;
+        sdf_profile_stats->pixels = hp.pixels;
+        sdf_profile_stats->distance_evals = hp.distance_evals;
+        sdf_profile_stats->march_orbit_iterations = hp.orbit_iterations;
+    }
+    if (output_path && output_path[0]) {
+        DevBuf rgba_d;
+        HIP_TRY(rgba_d.alloc(pixels * 4), "present buffer");
+        HIP_TRY(fpt::launch_present(plan->dcfg, *config, accum.as<float4>(), rgba_d.as<uchar4>(), st.s),
+                "present kernel launch");
+        std::vector<uint8_t> rgba(pixels * 4);
+        HIP_TRY(hipMemcpyAsync(rgba.data(), rgba_d.p, rgba.size(), hipMemcpyDeviceToHost, st.s), "present readback");
+        HIP_TRY(hipStreamSynchronize(st.s), "present");
+        if (unique_colour_sample(rgba) < 2) {  // metal_bridge.mm:5852-5857
+            set_error(error, error_len, "render output is blank or single-colour");
+            return 1;
+        }
+        if (!write_png(output_path, config->width, config->height, rgba, error, error_len)) return 1;
+    }
+    return 0;
+}
+
+int fpt_hip_trace_primary(const FptRenderConfig *config, FptRayHit *hits, size_t hit_count, char *error,
+                          size_t error_len) {
+    if (validate_config(config, error, error_len)) return 1;
+    if (config->renderer_backend != FPT_RENDERER_SDF) {
+        set_error(error, error_len, "fpt_hip_trace_primary needs the SDF backend");
+        return 1;
+    }
+    const size_t pixels = size_t(config->width) * config->height;
+    if (!hits || hit_count < pixels) {
+        set_error(error, error_len, "hit buffer is too small (%zu < %zu)", hit_count, pixels);
+        return 1;
+    }
+    if (check_device(error, error_len)) return 1;
+    Stream st;
+    HIP_TRY(hipStreamCreate(&st.s), "hipStreamCreate");
+    DevBuf dcfg, dh;
+    HIP_TRY(dcfg.alloc(sizeof(FptRenderConfig)), "config allocation");
+    HIP_TRY(dh.alloc(pixels * sizeof(FptRayHit)), "hit buffer allocation");
+    HIP_TRY(hipMemcpyAsync(dcfg.p, config, sizeof(FptRenderConfig), hipMemcpyHostToDevice, st.s), "config upload");
+    HIP_TRY(fpt::launch_trace_primary(dcfg.as<FptRenderConfig>(), *config, dh.as<FptRayHit>(), st.s),
+            "trace kernel launch");
+    HIP_TRY(hipMemcpyAsync(hits, dh.p, pixels * sizeof(FptRayHit), hipMemcpyDeviceToHost, st.s), "hit readback");
+    HIP_TRY(hipStreamSynchronize(st.s), "trace");
+    return 0;
+}
+
+int fpt_mandelbulber_sample_field(const char *metallib_path, const char *shader_source, size_t shader_source_len,
+                                  const FptRenderConfig *config, const float *points_xyzw, size_t point_count,
+                                  FptMandelbulberFieldSample *samples, char *error, size_t error_len) {
+    (void)metallib_path; (void)shader_source; (void)shader_source_len;
+    if (!config || !points_xyzw || !samples) {
+        set_error(error, error_len, "missing config, points or sample buffer");
+        return 1;
+    }
+    if (point_count == 0) return 0;
+    if (check_device(error, error_len)) return 1;
+    Stream st;
+    HIP_TRY(hipStreamCreate(&st.s), "hipStreamCreate");
+    DevBuf dcfg, dp, ds;
+    HIP_TRY(dcfg.alloc(sizeof(FptRenderConfig)), "config allocation");
+    HIP_TRY(dp.alloc(point_count * 16), "point buffer allocation");
+    HIP_TRY(ds.alloc(point_count * sizeof(FptMandelbulberFieldSample)), "sample buffer allocation");
+    HIP_TRY(hipMemcpyAsync(dcfg.p, config, sizeof(FptRenderConfig), hipMemcpyHostToDevice, st.s), "config upload");
+    HIP_TRY(hipMemcpyAsync(dp.p, points_xyzw, point_count * 16, hipMemcpyHostToDevice, st.s), "point upload");
+    HIP_TRY(fpt::launch_field_sample(dcfg.as<FptRenderConfig>(), dp.as<float4>(), point_count,
+                                     ds.as<FptMandelbulberFieldSample>(), st.s),
+            "field kernel launch");
+    HIP_TRY(hipMemcpyAsync(samples, ds.p, point_count * sizeof(FptMandelbulberFieldSample), hipMemcpyDeviceToHost,
+                           st.s),
+            "sample readback");
+    HIP_TRY(hipStreamSynchronize(st.s), "field sampling");
+    return 0;
+}
+
+int fpt_hip_program_surface(const FptRenderConfig *config, const float *points_xyzw, size_t point_count,
+                            float *out_dgrad, char *error, size_t error_len) {
+    if (!config || !points_xyzw || !out_dgrad) {
+        set_error(error, error_len, "missing config, points or output buffer");
+        return 1;
+    }
+    if (point_count == 0) return 0;
+    if (check_device(error, error_len)) return 1;
+    Stream st;
+    HIP_TRY(hipStreamCreate(&st.s), "hipStreamCreate");
+    DevBuf dcfg, dp, dout;
+    HIP_TRY(dcfg.alloc(sizeof(FptRenderConfig)), "config allocation");
+    HIP_TRY(dp.alloc(point_count * 16), "point buffer allocation");
+    HIP_TRY(dout.alloc(point_count * 16), "output buffer allocation");
+    HIP_TRY(hipMemcpyAsync(dcfg.p, config, sizeof(FptRenderConfig), hipMemcpyHostToDevice, st.s), "config upload");
+    HIP_TRY(hipMemcpyAsync(dp.p, points_xyzw, point_count * 16, hipMemcpyHostToDevice, st.s), "point upload");
+    HIP_TRY(fpt::launch_surface_sample(dcfg.as<FptRenderConfig>(), dp.as<float4>(), point_count, dout.as<float4>(),
+                                       st.s),
+            "surface kernel launch");
+    HIP_TRY(hipMemcpyAsync(out_dgrad, dout.p, point_count * 16, hipMemcpyDeviceToHost, st.s), "readback");
+    HIP_TRY(hipStreamSynchronize(st.s), "surface sampling");
+    return 0;
+}
+
+int fpt_metal_diagnostic_render(const char *metallib_path, const char *output_path, const char *shader_source,
+                                size_t shader_source_len, const FptRenderConfig *config,
+                                const FptDiagnosticConfig *diagnostic, double *elapsed_ms, char *error,
+                                size_t error_len) {
+    (void)metallib_path; (void)output_path; (void)shader_source; (void)shader_source_len; (void)config;
+    (void)diagnostic; (void)elapsed_ms;
+    set_error(error, error_len, "diagnostic images are not implemented yet; use fpt_hip_trace_primary");
+    return 1;
+}
+
+int fpt_compare_images(const char *baseline_path, const char *candidate_path, const char *report_path, char *error,
+                       size_t error_len) {
+    (void)baseline_path; (void)candidate_path; (void)report_path;
+    set_error(error, error_len, "image comparison is not implemented yet");
+    return 1;
+}
+
+int fpt_test_async_stitch_context(const char *, const char *, const char *, const FptRenderConfig *,
+                                  FptAsyncJitStats *, char *error, size_t error_len) {
+    set_error(error, error_len, "Metal function stitching has no counterpart in this library");
+    return 1;
+}
+
+int fpt_test_typed_soa(const char *, const FptRenderConfig *, FptStitchValidationStats *, char *error,
+                       size_t error_len) {
+    set_error(error, error_len, "typed SoA lowering has no counterpart in this library");
+    return 1;
+}
+
+int fpt_metal_preview(const char *, const char *, const char *const *, const char *, size_t,
+                      const FptRenderConfig *, const FptRenderConfig *, uint32_t, uint32_t, char *error,
+                      size_t error_len) {
+    set_error(error, error_len, "the interactive preview has no counterpart in this library");
+    return 1;
+}
+
+}  // extern "C"

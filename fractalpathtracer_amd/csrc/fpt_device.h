@@ -1,0 +1,926 @@
+// Device-side scene evaluation for gfx950: vector helpers, the reference's
+// RNG and camera, the typed SDF program interpreter with fractal leaves, the
+// analytic program gradient, march, normals, lighting and the path tracer.
+//
+// Reference semantics: shaders/Shaders.metal (SM) and
+// src/mandelbulber/orbit.rs (ORB); line numbers are cited per function.
+//
+// Design notes (MI355X):
+//  * All geometry is FP32.  RNG keys and the pixel uv use explicitly rounded
+//    intrinsics (__fmul_rn/__fadd_rn/__fdiv_rn) so they are bit-identical to
+//    an IEEE CPU computation: the hash keys on float bit patterns.
+//  * The integer-power Mandelbulb step avoids the seven transcendentals of
+//    the triplex formula (asin, atan2, pow, 2 sin, 2 cos): (cos t + i sin t)^p
+//    and (cos f + i sin f)^p are raised by repeated complex squaring from
+//    cos/sin that are ratios of the orbit coordinates.  It is the same
+//    function in exact arithmetic; only VALU adds/muls, one sqrt and one
+//    reciprocal remain per iteration.
+//  * Leaves are templated on the scalar type so the analytic normal is the
+//    forward-mode derivative (Dual) of exactly the function the march uses.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/fpt_abi.h"
+
+namespace fpt {
+
+constexpr float kPi = 3.14159265359f;  // SM:4
+constexpr float kInf = 1.0e20f;        // SM:5
+
+struct f3 {
+    float x, y, z;
+};
+__device__ __forceinline__ f3 mk(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, f3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, float k) { return {a.x * k, a.y * k, a.z * k}; }
+__device__ __forceinline__ f3 operator/(f3 a, float k) { return {a.x / k, a.y / k, a.z / k}; }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float length(f3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ f3 normalize(f3 a) { return a * (1.0f / sqrtf(dot(a, a))); }
+__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ f3 vabs(f3 a) { return {fabsf(a.x), fabsf(a.y), fabsf(a.z)}; }
+__device__ __forceinline__ f3 vmix(f3 a, f3 b, float t) { return a + (b - a) * t; }
+__device__ __forceinline__ f3 vmax0(f3 a) { return {fmaxf(a.x, 0.f), fmaxf(a.y, 0.f), fmaxf(a.z, 0.f)}; }
+__device__ __forceinline__ f3 vmin(f3 a, float k) { return {fminf(a.x, k), fminf(a.y, k), fminf(a.z, k)}; }
+__device__ __forceinline__ f3 reflect(f3 i, f3 n) { return i - n * (2.0f * dot(n, i)); }
+__device__ __forceinline__ f3 refract(f3 i, f3 n, float eta) {
+    float ni = dot(n, i);
+    float k = 1.0f - eta * eta * (1.0f - ni * ni);
+    if (k < 0.0f) return {0.f, 0.f, 0.f};
+    return i * eta - n * (eta * ni + sqrtf(k));
+}
+__device__ __forceinline__ float signf(float x) { return x > 0.f ? 1.f : (x < 0.f ? -1.f : 0.f); }
+__device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+__device__ __forceinline__ float mixf(float a, float b, float t) { return a + (b - a) * t; }
+__device__ __forceinline__ float fractf(float x) { return x - floorf(x); }
+__device__ __forceinline__ f3 ld3(const float *v) { return {v[0], v[1], v[2]}; }
+__device__ __forceinline__ bool finite3(f3 a) { return isfinite(a.x) && isfinite(a.y) && isfinite(a.z); }
+
+// ---------------------------------------------------------------------------
+// Forward-mode dual number: value + gradient wrt the 3 point coordinates.
+// ---------------------------------------------------------------------------
+struct Df {
+    float v, a, b, c;
+};
+__device__ __forceinline__ Df dconst(float x) { return {x, 0.f, 0.f, 0.f}; }
+__device__ __forceinline__ Df operator+(Df x, Df y) { return {x.v + y.v, x.a + y.a, x.b + y.b, x.c + y.c}; }
+__device__ __forceinline__ Df operator-(Df x, Df y) { return {x.v - y.v, x.a - y.a, x.b - y.b, x.c - y.c}; }
+__device__ __forceinline__ Df operator-(Df x) { return {-x.v, -x.a, -x.b, -x.c}; }
+__device__ __forceinline__ Df operator*(Df x, Df y) {
+    return {x.v * y.v, x.a * y.v + x.v * y.a, x.b * y.v + x.v * y.b, x.c * y.v + x.v * y.c};
+}
+__device__ __forceinline__ Df operator*(Df x, float k) { return {x.v * k, x.a * k, x.b * k, x.c * k}; }
+__device__ __forceinline__ Df operator/(Df x, Df y) {
+    float inv = 1.0f / y.v;
+    float q = x.v * inv;
+    return {q, (x.a - q * y.a) * inv, (x.b - q * y.b) * inv, (x.c - q * y.c) * inv};
+}
+
+// Scalar traits: the same leaf code runs on float and Df.
+__device__ __forceinline__ float val(float x) { return x; }
+__device__ __forceinline__ float val(Df x) { return x.v; }
+__device__ __forceinline__ float s_sqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ Df s_sqrt(Df x) {
+    float r = sqrtf(x.v);
+    float k = 0.5f / r;
+    return {r, x.a * k, x.b * k, x.c * k};
+}
+__device__ __forceinline__ float s_rcp(float x) { return 1.0f / x; }
+__device__ __forceinline__ Df s_rcp(Df x) {
+    float r = 1.0f / x.v;
+    float k = -r * r;
+    return {r, x.a * k, x.b * k, x.c * k};
+}
+__device__ __forceinline__ float s_log(float x) { return logf(x); }
+__device__ __forceinline__ Df s_log(Df x) {
+    float k = 1.0f / x.v;
+    return {logf(x.v), x.a * k, x.b * k, x.c * k};
+}
+__device__ __forceinline__ float s_sin(float x) { return sinf(x); }
+__device__ __forceinline__ Df s_sin(Df x) {
+    float s, c;
+    sincosf(x.v, &s, &c);
+    return {s, x.a * c, x.b * c, x.c * c};
+}
+__device__ __forceinline__ float s_cos(float x) { return cosf(x); }
+__device__ __forceinline__ Df s_cos(Df x) {
+    float s, c;
+    sincosf(x.v, &s, &c);
+    return {c, -x.a * s, -x.b * s, -x.c * s};
+}
+__device__ __forceinline__ float s_asin(float x) { return asinf(x); }
+__device__ __forceinline__ Df s_asin(Df x) {
+    float k = 1.0f / sqrtf(1.0f - x.v * x.v);
+    return {asinf(x.v), x.a * k, x.b * k, x.c * k};
+}
+__device__ __forceinline__ float s_atan2(float y, float x) { return atan2f(y, x); }
+__device__ __forceinline__ Df s_atan2(Df y, Df x) {
+    float den = 1.0f / (x.v * x.v + y.v * y.v);
+    float ky = x.v * den, kx = -y.v * den;
+    return {atan2f(y.v, x.v), ky * y.a + kx * x.a, ky * y.b + kx * x.b, ky * y.c + kx * x.c};
+}
+__device__ __forceinline__ float s_pow(float x, float e) { return powf(x, e); }
+__device__ __forceinline__ Df s_pow(Df x, float e) {
+    float p = powf(x.v, e);
+    float k = e * powf(x.v, e - 1.0f);
+    return {p, x.a * k, x.b * k, x.c * k};
+}
+__device__ __forceinline__ float s_abs(float x) { return fabsf(x); }
+__device__ __forceinline__ Df s_abs(Df x) { return x.v < 0.f ? -x : x; }
+__device__ __forceinline__ float s_max(float x, float y) { return fmaxf(x, y); }
+__device__ __forceinline__ Df s_max(Df x, Df y) { return x.v < y.v ? y : x; }
+__device__ __forceinline__ float s_min(float x, float y) { return fminf(x, y); }
+__device__ __forceinline__ Df s_min(Df x, Df y) { return y.v < x.v ? y : x; }
+__device__ __forceinline__ float s_scale(float x, float k) { return x * k; }
+__device__ __forceinline__ Df s_scale(Df x, float k) { return x * k; }
+
+template <class S>
+struct v3 {
+    S x, y, z;
+};
+template <class S> __device__ __forceinline__ v3<S> add(v3<S> a, v3<S> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class S> __device__ __forceinline__ S dot3(v3<S> a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+
+// ---------------------------------------------------------------------------
+// RNG (SM:541-576): MurmurHash2-style mix over float32 bit patterns.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t hash13bits(float x, float y, float z) {
+    uint32_t h = 1190494759u;
+    h = (h ^ __float_as_uint(x)) * 0x5bd1e995u;
+    h = (h ^ __float_as_uint(y)) * 0x5bd1e995u;
+    h = (h ^ __float_as_uint(z)) * 0x5bd1e995u;
+    h ^= h >> 13u;
+    h *= 0x5bd1e995u;
+    h ^= h >> 15u;
+    return h;
+}
+__device__ __forceinline__ float hash13(float x, float y, float z) {
+    // (h & 0xffffff) / 2^24 is exact in float32.
+    return float(hash13bits(x, y, z) & 0x00ffffffu) * (1.0f / 16777216.0f);
+}
+// float32 a*b + c rounded after the product (the reference source order).
+__device__ __forceinline__ float madd_rn(float a, float b, float c) { return __fadd_rn(__fmul_rn(a, b), c); }
+
+struct UV {
+    float x, y;
+};
+
+// randomVector (SM:557-568).
+__device__ __forceinline__ f3 random_vector(f3 normal, UV xy, float seed) {
+    float k = __fmul_rn(seed, 2.0f);
+    float h1 = hash13(xy.x, xy.y, __fadd_rn(k, 0.0f));
+    float h2 = hash13(xy.x, xy.y, __fadd_rn(k, 1.0f));
+    f3 n = normalize(normal);
+    f3 uu = normalize(cross(n, mk(0.f, 1.f, 1.f)));
+    f3 vv = cross(uu, n);
+    float ra = sqrtf(h2);
+    float s, c;
+    sincosf(kPi * 2.0f * h1, &s, &c);
+    float rz = sqrtf(fmaxf(1.0f - h2, 0.0f));
+    return normalize(uu * (ra * c) + vv * (ra * s) + n * rz);
+}
+
+// randomPoint (SM:570-576).
+__device__ __forceinline__ void random_point(float power, UV xy, float seed, float &ox, float &oy) {
+    float k = __fmul_rn(seed, 2.0f);
+    float h1 = hash13(xy.x, xy.y, __fadd_rn(k, 0.0f));
+    float h2 = hash13(xy.x, xy.y, __fadd_rn(k, 1.0f));
+    float r = 2.0f * h1 * kPi;
+    float r2 = 2.0f * h2 * kPi;
+    float s, c;
+    sincosf(r, &s, &c);
+    float m = sqrtf(r2) * power;
+    ox = c * m;
+    oy = s * m;
+}
+
+// ---------------------------------------------------------------------------
+// Camera (SM:460-471, 6847-6855).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ f3 rotate_camera(f3 v, float yaw, float pitch) {
+    float sp, cp, sy, cy;
+    sincosf(pitch, &sp, &cp);
+    sincosf(yaw, &sy, &cy);
+    v = mk(v.x, v.z * sp + v.y * cp, v.z * cp - v.y * sp);
+    return mk(v.x * cy + v.z * sy, v.y, -v.x * sy + v.z * cy);
+}
+__device__ __forceinline__ f3 rotate_camera_roll(f3 v, float yaw, float pitch, float roll) {
+    float s, c;
+    sincosf(roll, &s, &c);
+    return rotate_camera(mk(c * v.x - s * v.y, s * v.x + c * v.y, v.z), yaw, pitch);
+}
+
+__device__ __forceinline__ UV screen_uv(uint32_t x, uint32_t y, uint32_t w, uint32_t h) {
+    float fw = float(w), fh = float(h);
+    float u = __fsub_rn(__fdiv_rn(__fadd_rn(float(x), 0.5f), fw), 0.5f);
+    float v = __fsub_rn(__fdiv_rn(__fadd_rn(float(y), 0.5f), fh), 0.5f);
+    return {__fmul_rn(u, __fdiv_rn(fw, fh)), v};
+}
+
+// ---------------------------------------------------------------------------
+// Fractal leaves.
+// ---------------------------------------------------------------------------
+template <class S>
+struct Leaf {
+    S d;
+    float radius, derivative, trap;
+    int iterations;
+};
+
+// |z| robust to float32 overflow of |z|^2 (escaping Mandelbulb orbits pass
+// |z| ~ 1e20 one iteration after |p| ~ 235).
+template <class S>
+__device__ __forceinline__ S orbit_radius(v3<S> z) {
+    S r2 = dot3(z);
+    if (__builtin_expect(isfinite(val(r2)), 1)) return s_sqrt(r2);
+    float m = fmaxf(fabsf(val(z.x)), fmaxf(fabsf(val(z.y)), fabsf(val(z.z))));
+    if (!isfinite(m)) return r2;
+    float im = 1.0f / m;
+    v3<S> q = {s_scale(z.x, im), s_scale(z.y, im), s_scale(z.z, im)};
+    return s_scale(s_sqrt(dot3(q)), m);
+}
+
+// (c + i s)^n for integer n >= 1 by binary exponentiation.
+template <class S>
+__device__ __forceinline__ void cpow_int(S c, S s, int n, S &oc, S &os) {
+    S rc = c, rs = s;
+    bool have = false;
+    S ac = c, as = s;
+    while (true) {
+        if (n & 1) {
+            if (!have) {
+                rc = ac;
+                rs = as;
+                have = true;
+            } else {
+                S t = rc * ac - rs * as;
+                rs = rc * as + rs * ac;
+                rc = t;
+            }
+        }
+        n >>= 1;
+        if (!n) break;
+        S t = ac * ac - as * as;
+        as = (ac * as) * 2.0f;
+        ac = t;
+    }
+    oc = rc;
+    os = rs;
+}
+__device__ __forceinline__ Df operator*(Df x, int k) { return x * float(k); }
+
+template <class S>
+__device__ __forceinline__ S ipow_int(S x, int n) {  // x^n, n >= 0
+    S r = x;
+    bool have = false;
+    S b = x;
+    while (n) {
+        if (n & 1) {
+            r = have ? r * b : b;
+            have = true;
+        }
+        n >>= 1;
+        if (n) b = b * b;
+    }
+    if (!have) {
+        if constexpr (sizeof(S) == sizeof(float)) return S(1.0f);
+        else return dconst(1.0f);
+    }
+    return r;
+}
+
+template <class S> __device__ __forceinline__ S sconst(float x);
+template <> __device__ __forceinline__ float sconst<float>(float x) { return x; }
+template <> __device__ __forceinline__ Df sconst<Df>(float x) { return dconst(x); }
+
+// Mandelbulb leaf: orbit loop (ORB:530-585) with the triplex kernel
+// (ORB:766-778), +c (mandelbulber.rs:1421-1440) and the Logarithmic
+// finalizer (ORB:654-660).
+template <class S>
+__device__ Leaf<S> leaf_mandelbulb(v3<S> p, const float *data, int *orbit_counter) {
+    const float power = data[0];
+    const int iters = int(data[1]);
+    const float bailout = data[2];
+    const int ip = int(power);
+    const bool integral = float(ip) == power && ip >= 1 && ip <= 32;
+    v3<S> z = p;
+    S r = orbit_radius(z);
+    S dr = sconst<S>(1.0f);
+    float trap = val(r);
+    int done = 0;
+    for (int i = 0; i < iters; ++i) {
+        v3<S> old = z;
+        if (integral) {
+            S rho2 = z.x * z.x + z.y * z.y;
+            S rho = s_sqrt(rho2);
+            S ir = s_rcp(r);
+            S st = z.z * ir;
+            S ct = rho * ir;
+            S cp, sp;
+            if (val(rho) > 0.0f) {
+                S irho = s_rcp(rho);
+                cp = z.x * irho;
+                sp = z.y * irho;
+            } else {  // atan2(0, +-0) = 0 or pi: cos = sign(x) (x = +0 -> 1)
+                cp = sconst<S>(signbit(val(z.x)) ? -1.0f : 1.0f);
+                sp = sconst<S>(0.0f);
+            }
+            S cpt, spt, cpp, spp;
+            cpow_int(ct, st, ip, cpt, spt);
+            cpow_int(cp, sp, ip, cpp, spp);
+            S rp = ipow_int(r, ip - 1);
+            dr = rp * dr * power + sconst<S>(1.0f);
+            rp = rp * r;
+            S k = cpt * rp;
+            z = {k * cpp, k * spp, spt * rp};
+        } else {
+            S u = z.z / r;
+            S theta = s_asin(u) * power;
+            S phi = s_atan2(z.y, z.x) * power;
+            S rp = s_pow(r, power - 1.0f);
+            dr = rp * dr * power + sconst<S>(1.0f);
+            rp = rp * r;
+            S ctt = s_cos(theta);
+            z = {ctt * s_cos(phi) * rp, ctt * s_sin(phi) * rp, s_sin(theta) * rp};
+        }
+        z = add(z, p);
+        r = orbit_radius(z);
+        done = i + 1;
+        if (!isfinite(val(r))) {
+            z = old;
+            r = orbit_radius(z);
+            break;
+        }
+        trap = fminf(trap, val(r));
+        if (val(r) > bailout) break;
+    }
+    if (orbit_counter) *orbit_counter += done;
+    Leaf<S> out;
+    if (!(val(dr) > 0.0f)) out.d = r;
+    else if (val(r) > 1.0f) out.d = (r * s_log(r) / dr) * 0.5f;
+    else out.d = sconst<S>(0.0f);
+    out.radius = val(r);
+    out.derivative = val(dr);
+    out.trap = trap;
+    out.iterations = done;
+    return out;
+}
+
+// Mandelbox leaf: ORB:1636-1687 (fold limit 1, value 2, no rotations,
+// no offset), orbit loop ORB:530-585, bailout 100, Linear finalizer.
+template <class S>
+__device__ Leaf<S> leaf_mandelbox(v3<S> p, const float *data, int *orbit_counter) {
+    const float scale = data[0];
+    const int iters = int(data[1]);
+    const float min_r2 = data[2] * data[2];
+    const float fixed_r2 = data[3] * data[3];
+    const float min_factor = fixed_r2 / min_r2;
+    const float abs_scale = fabsf(scale);
+    v3<S> z = p;
+    S r = orbit_radius(z);
+    S dr = sconst<S>(1.0f);
+    float trap = val(r);
+    int done = 0;
+    for (int i = 0; i < iters; ++i) {
+        v3<S> old = z;
+        if (fabsf(val(z.x)) > 1.0f) z.x = sconst<S>(val(z.x) > 0.f ? 2.0f : -2.0f) - z.x;
+        if (fabsf(val(z.y)) > 1.0f) z.y = sconst<S>(val(z.y) > 0.f ? 2.0f : -2.0f) - z.y;
+        if (fabsf(val(z.z)) > 1.0f) z.z = sconst<S>(val(z.z) > 0.f ? 2.0f : -2.0f) - z.z;
+        S r2 = dot3(z);
+        if (val(r2) < min_r2) {
+            z = {s_scale(z.x, min_factor), s_scale(z.y, min_factor), s_scale(z.z, min_factor)};
+            dr = s_scale(dr, min_factor);
+        } else if (val(r2) < fixed_r2) {
+            S f = sconst<S>(fixed_r2) / r2;
+            z = {z.x * f, z.y * f, z.z * f};
+            dr = dr * f;
+        }
+        z = {s_scale(z.x, scale), s_scale(z.y, scale), s_scale(z.z, scale)};
+        dr = s_scale(dr, abs_scale) + sconst<S>(1.0f);
+        z = add(z, p);
+        r = orbit_radius(z);
+        done = i + 1;
+        if (!isfinite(val(r))) {
+            z = old;
+            r = orbit_radius(z);
+            break;
+        }
+        trap = fminf(trap, val(r));
+        if (val(r) > 100.0f) break;
+    }
+    if (orbit_counter) *orbit_counter += done;
+    Leaf<S> out;
+    out.d = !(val(dr) > 0.0f) ? r : r / dr;
+    out.radius = val(r);
+    out.derivative = val(dr);
+    out.trap = trap;
+    out.iterations = done;
+    return out;
+}
+
+// Menger leaf (SM:756-784, parametric count/size/divisor, no rotation).
+template <class S>
+__device__ __forceinline__ S menger_axis_mod(S x, float s) {
+    S y = x - sconst<S>(0.5f * s);
+    return y - sconst<S>(s * floorf(val(y) / s)) - sconst<S>(0.5f * s);
+}
+template <class S>
+__device__ Leaf<S> leaf_menger(v3<S> p, const float *data, int *orbit_counter) {
+    const int iters = int(data[0]);
+    const float size = data[1] != 0.0f ? data[1] : 1.0f;
+    const float divisor = data[2] != 0.0f ? data[2] : 3.0f;
+    const float inv = 1.0f / size;
+    v3<S> q = {s_scale(p.x, inv), s_scale(p.y, inv), s_scale(p.z, inv)};
+    S sdf = s_max(s_max(s_abs(q.x) - sconst<S>(1.f), s_abs(q.y) - sconst<S>(1.f)), s_abs(q.z) - sconst<S>(1.f));
+    float scale = 1.0f;
+    for (int i = 0; i < iters; ++i) {
+        float s = scale * 2.0f;
+        S mx = menger_axis_mod(q.x, s), my = menger_axis_mod(q.y, s), mz = menger_axis_mod(q.z, s);
+        S ax = s_abs(mx), ay = s_abs(my), az = s_abs(mz);
+        S da = s_max(ax, ay), db = s_max(ay, az), dc = s_max(az, ax);
+        S cut = s_min(da, s_min(db, dc)) - sconst<S>(s / 6.0f);
+        sdf = s_max(sdf, -cut);
+        scale /= divisor;
+    }
+    if (orbit_counter) *orbit_counter += iters;
+    Leaf<S> out;
+    out.d = s_scale(sdf, size);
+    out.radius = 0.f;
+    out.derivative = 0.f;
+    out.trap = 0.f;
+    out.iterations = iters;
+    return out;
+}
+
+__device__ __forceinline__ bool is_leaf(uint32_t op) {
+    return op == FPT_SDF_OP_MANDELBULB || op == FPT_SDF_OP_MANDELBOX || op == FPT_SDF_OP_MENGER;
+}
+
+template <class S>
+__device__ __forceinline__ Leaf<S> eval_leaf(uint32_t op, v3<S> p, const float *data, int *oc) {
+    if (op == FPT_SDF_OP_MANDELBULB) return leaf_mandelbulb<S>(p, data, oc);
+    if (op == FPT_SDF_OP_MANDELBOX) return leaf_mandelbox<S>(p, data, oc);
+    return leaf_menger<S>(p, data, oc);
+}
+
+// ---------------------------------------------------------------------------
+// Typed program interpreter (SM:1065-1069, 1882-2036) with fractal leaves.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float combine(float cur, float cand, uint32_t mode) {
+    if (mode == 1u) return fmaxf(cur, cand);
+    if (mode == 2u) return fmaxf(cur, -cand);
+    return fminf(cur, cand);
+}
+
+__device__ __forceinline__ void rot2(float &a, float &b, float angle) {
+    float s, c;
+    sincosf(angle, &s, &c);
+    float na = c * a - s * b;
+    b = s * a + c * b;
+    a = na;
+}
+
+// Apply a transform op to the running point; false when `op` is not one.
+__device__ __forceinline__ bool apply_transform(const FptSdfInstruction &op, f3 &p, float &ds) {
+    const float *d = op.data;
+    switch (op.opcode) {
+        case FPT_SDF_OP_ABS: p = vabs(p); return true;
+        case FPT_SDF_OP_TRANSLATE: p = p - ld3(d); return true;
+        case FPT_SDF_OP_SCALE: {
+            float s = fabsf(d[0]) > 1.0e-6f ? d[0] : 1.0f;
+            p = p * s;
+            ds *= fabsf(s);
+            return true;
+        }
+        case FPT_SDF_OP_ROTATE_X: rot2(p.y, p.z, d[0]); return true;
+        case FPT_SDF_OP_ROTATE_Y: rot2(p.x, p.z, d[0]); return true;
+        case FPT_SDF_OP_ROTATE_Z: rot2(p.x, p.y, d[0]); return true;
+        case FPT_SDF_OP_REPEAT: {
+            f3 per = mk(fmaxf(fabsf(d[0]), 1.0e-5f), fmaxf(fabsf(d[1]), 1.0e-5f), fmaxf(fabsf(d[2]), 1.0e-5f));
+            p.x -= per.x * floorf(p.x / per.x + 0.5f);
+            p.y -= per.y * floorf(p.y / per.y + 0.5f);
+            p.z -= per.z * floorf(p.z / per.z + 0.5f);
+            return true;
+        }
+        case FPT_SDF_OP_SORT_DESC: {
+            float t;
+            if (p.x < p.z) { t = p.x; p.x = p.z; p.z = t; }
+            if (p.y < p.z) { t = p.y; p.y = p.z; p.z = t; }
+            if (p.x < p.y) { t = p.x; p.x = p.y; p.y = t; }
+            return true;
+        }
+        case FPT_SDF_OP_BOX_FOLD: {
+            float l = d[0] != 0.0f ? d[0] : 1.0f;
+            p.x = clampf(p.x, -l, l) * 2.0f - p.x;
+            p.y = clampf(p.y, -l, l) * 2.0f - p.y;
+            p.z = clampf(p.z, -l, l) * 2.0f - p.z;
+            return true;
+        }
+        default: return false;
+    }
+}
+
+struct Work {  // per-thread work counters (SM:7650-7704 categories)
+    int distance_evals;
+    int orbit_iterations;
+};
+
+// deProgramDistance / deShadingProgram: distance (and orbit when asked).
+template <bool ORBIT>
+__device__ float program_eval(const FptSdfInstruction *ops, uint32_t count, f3 p, float *orbit_out,
+                              int *orbit_counter) {
+    float ds = 1.0f;
+    float dist = kInf;
+    float orbit = 0.0f;
+    bool has = false;
+    for (uint32_t i = 0; i < count; ++i) {
+        const FptSdfInstruction &op = ops[i];
+        if (apply_transform(op, p, ds)) continue;
+        const float *d = op.data;
+        const float div = fmaxf(ds, 1.0e-6f);
+        float cand;
+        bool leaf = true;
+        switch (op.opcode) {
+            case FPT_SDF_OP_SPHERE: cand = length(p) - d[0]; break;
+            case FPT_SDF_OP_BOX: {
+                f3 q = vabs(p) - vabs(ld3(d));
+                cand = fminf(fmaxf(q.x, fmaxf(q.y, q.z)), 0.0f) + length(vmax0(q));
+                break;
+            }
+            case FPT_SDF_OP_PLANE: cand = dot(p, normalize(ld3(d))) + d[3]; break;
+            case FPT_SDF_OP_MANDELBULB:
+            case FPT_SDF_OP_MANDELBOX:
+            case FPT_SDF_OP_MENGER: {
+                Leaf<float> lf = eval_leaf<float>(op.opcode, v3<float>{p.x, p.y, p.z}, d, orbit_counter);
+                cand = lf.d;
+                if (ORBIT) orbit += lf.trap;
+                break;
+            }
+            case FPT_SDF_OP_ORBIT_ADD:
+                if (ORBIT) orbit += length(p * ld3(d)) * d[3];
+                leaf = false;
+                break;
+            default: leaf = false; break;
+        }
+        if (!leaf) continue;
+        cand = cand / div;
+        dist = has ? combine(dist, cand, op.flags) : cand;
+        has = true;
+        if (ORBIT && (op.opcode == FPT_SDF_OP_SPHERE || op.opcode == FPT_SDF_OP_BOX))
+            orbit += fabsf(cand) * d[3];
+    }
+    if (ORBIT) *orbit_out = orbit;
+    return dist;
+}
+
+// programSurfaceInterpreted (SM:3266-3369) through fractal leaves.
+__device__ float program_surface(const FptSdfInstruction *ops, uint32_t count, f3 p, f3 &grad) {
+    f3 jx = mk(1.f, 0.f, 0.f), jy = mk(0.f, 1.f, 0.f), jz = mk(0.f, 0.f, 1.f);
+    float ds = 1.0f;
+    float dist = kInf;
+    grad = mk(0.f, 0.f, 0.f);
+    bool has = false;
+    for (uint32_t i = 0; i < count; ++i) {
+        const FptSdfInstruction &op = ops[i];
+        const float *d = op.data;
+        switch (op.opcode) {
+            case FPT_SDF_OP_ABS:
+                jx = jx * signf(p.x); jy = jy * signf(p.y); jz = jz * signf(p.z);
+                p = vabs(p);
+                continue;
+            case FPT_SDF_OP_TRANSLATE: p = p - ld3(d); continue;
+            case FPT_SDF_OP_SCALE: {
+                float s = fabsf(d[0]) > 1.0e-6f ? d[0] : 1.0f;
+                p = p * s; jx = jx * s; jy = jy * s; jz = jz * s;
+                ds *= fabsf(s);
+                continue;
+            }
+            case FPT_SDF_OP_ROTATE_X: {
+                float s, c; sincosf(d[0], &s, &c);
+                float py = p.y; f3 oj = jy;
+                p.y = c * py - s * p.z; p.z = s * py + c * p.z;
+                jy = oj * c - jz * s; jz = oj * s + jz * c;
+                continue;
+            }
+            case FPT_SDF_OP_ROTATE_Y: {
+                float s, c; sincosf(d[0], &s, &c);
+                float px = p.x; f3 oj = jx;
+                p.x = c * px - s * p.z; p.z = s * px + c * p.z;
+                jx = oj * c - jz * s; jz = oj * s + jz * c;
+                continue;
+            }
+            case FPT_SDF_OP_ROTATE_Z: {
+                float s, c; sincosf(d[0], &s, &c);
+                float px = p.x; f3 oj = jx;
+                p.x = c * px - s * p.y; p.y = s * px + c * p.y;
+                jx = oj * c - jy * s; jy = oj * s + jy * c;
+                continue;
+            }
+            case FPT_SDF_OP_REPEAT: { float u = 1.f; apply_transform(op, p, u); continue; }
+            case FPT_SDF_OP_SORT_DESC: {
+                float t; f3 tj;
+                if (p.x < p.z) { t = p.x; p.x = p.z; p.z = t; tj = jx; jx = jz; jz = tj; }
+                if (p.y < p.z) { t = p.y; p.y = p.z; p.z = t; tj = jy; jy = jz; jz = tj; }
+                if (p.x < p.y) { t = p.x; p.x = p.y; p.y = t; tj = jx; jx = jy; jy = tj; }
+                continue;
+            }
+            case FPT_SDF_OP_BOX_FOLD: {
+                float l = d[0] != 0.0f ? d[0] : 1.0f;
+                float fx = fabsf(p.x) > l ? -1.f : 1.f, fy = fabsf(p.y) > l ? -1.f : 1.f,
+                      fz = fabsf(p.z) > l ? -1.f : 1.f;
+                float u = 1.f;
+                apply_transform(op, p, u);
+                jx = jx * fx; jy = jy * fy; jz = jz * fz;
+                continue;
+            }
+            default: break;
+        }
+        const float scale = fmaxf(ds, 1.0e-6f);
+        float cand;
+        f3 lg;
+        if (op.opcode == FPT_SDF_OP_SPHERE) {
+            float radius = length(p);
+            cand = (radius - d[0]) / scale;
+            lg = radius > 1.0e-8f ? p / radius : mk(0.f, 1.f, 0.f);
+        } else if (op.opcode == FPT_SDF_OP_BOX) {
+            f3 q = vabs(p) - vabs(ld3(d));
+            f3 o = vmax0(q);
+            float ol = length(o);
+            cand = (fminf(fmaxf(q.x, fmaxf(q.y, q.z)), 0.0f) + ol) / scale;
+            if (ol > 1.0e-8f) lg = mk(signf(p.x), signf(p.y), signf(p.z)) * o / ol;
+            else if (q.x >= q.y && q.x >= q.z) lg = mk(signf(p.x), 0.f, 0.f);
+            else if (q.y >= q.z) lg = mk(0.f, signf(p.y), 0.f);
+            else lg = mk(0.f, 0.f, signf(p.z));
+        } else if (op.opcode == FPT_SDF_OP_PLANE) {
+            lg = normalize(ld3(d));
+            cand = (dot(p, lg) + d[3]) / scale;
+        } else if (is_leaf(op.opcode)) {
+            v3<Df> dp = {{p.x, 1.f, 0.f, 0.f}, {p.y, 0.f, 1.f, 0.f}, {p.z, 0.f, 0.f, 1.f}};
+            Leaf<Df> lf = eval_leaf<Df>(op.opcode, dp, d, nullptr);
+            cand = lf.d.v / scale;
+            lg = mk(lf.d.a, lf.d.b, lf.d.c);
+        } else {
+            continue;
+        }
+        f3 cg = (jx * lg.x + jy * lg.y + jz * lg.z) / scale;
+        bool choose = !has || (op.flags == 1u ? cand > dist : (op.flags == 2u ? -cand > dist : cand < dist));
+        if (choose) {
+            bool neg = op.flags == 2u && has;
+            dist = neg ? -cand : cand;
+            grad = neg ? -cg : cg;
+        }
+        has = true;
+    }
+    return dist;
+}
+
+// ---------------------------------------------------------------------------
+// Built-in fractal scenes (SM:730-784).
+// ---------------------------------------------------------------------------
+__device__ float de_mandelbox_builtin(f3 pos, const FptRenderConfig &cfg, float *orbit, int *oc) {
+    const float SCALE = 2.8f + cfg.set_values[2];
+    const float MINRAD2 = 0.25f + cfg.set_values[3];
+    const float sxyz = SCALE / MINRAD2, sw = fabsf(SCALE) / MINRAD2;
+    const float min_rad2 = clampf(MINRAD2, 1.0e-9f, 1.0f);
+    const float abs_scale_m1 = fabsf(SCALE - 1.0f);
+    const float abs_scale = powf(fabsf(SCALE), -9.0f);
+    const f3 off = mk(cfg.set_values[4], cfg.set_values[5], cfg.set_values[6]);
+    f3 p = pos;
+    float w = 1.0f;
+    float sdf = 0.f, col = 0.f;
+    for (int i = 0; i < 12; ++i) {
+        p.x = clampf(p.x, -1.f, 1.f) * 2.0f - p.x;
+        p.y = clampf(p.y, -1.f, 1.f) * 2.0f - p.y;
+        p.z = clampf(p.z, -1.f, 1.f) * 2.0f - p.z;
+        float r2 = dot(p, p);
+        float k = clampf(fmaxf(min_rad2 / fmaxf(r2, 0.00001f), min_rad2), 0.f, 1.f);
+        p = p * k;
+        w *= k;
+        p = p * sxyz + pos + off;
+        w = w * sw + 1.0f;
+        sdf = (length(p) - abs_scale_m1) / w - abs_scale;
+        col += sdf;
+    }
+    if (oc) *oc += 12;
+    if (orbit) *orbit = col;
+    return sdf;
+}
+
+__device__ float de_menger_builtin(f3 p, const FptRenderConfig &cfg, int *oc) {
+    f3 dd = vabs(p) - mk(1.f, 1.f, 1.f);
+    float sdf = fmaxf(fmaxf(dd.x, dd.y), dd.z);
+    float scale = fmaxf(cfg.set_values[0], 0.0001f);
+    const float divisor = fmaxf(cfg.set_values[1], 0.0001f);
+    const float ry = cfg.set_values[2], rp = cfg.set_values[3];
+    for (int i = 0; i < 12; ++i) {
+        float s = scale * 2.0f;
+        float h = s / 2.0f;
+        f3 y = mk(p.x - h, p.y - h, p.z - h);
+        f3 q = mk(y.x - s * floorf(y.x / s) - h, y.y - s * floorf(y.y / s) - h, y.z - s * floorf(y.z / s) - h);
+        float da = fmaxf(fabsf(q.x), fabsf(q.y)), db = fmaxf(fabsf(q.y), fabsf(q.z)), dc = fmaxf(fabsf(q.z), fabsf(q.x));
+        sdf = fmaxf(sdf, -(fminf(da, fminf(db, dc)) - s / 6.0f));
+        scale /= divisor;
+        p = rotate_camera(p, ry, rp);
+    }
+    if (oc) *oc += 12;
+    return sdf;
+}
+
+// ---------------------------------------------------------------------------
+// Scene dispatch and materials (SM:3418-3615).
+// ---------------------------------------------------------------------------
+struct Material {
+    f3 rgb;
+    float roughness, specular, translucency, ior, emission;
+};
+
+__device__ __forceinline__ bool is_program(const FptRenderConfig &cfg) {
+    return cfg.sdf_id == FPT_SDF_PROGRAM && cfg.sdf_program_count > 0u;
+}
+__device__ __forceinline__ uint32_t prog_count(const FptRenderConfig &cfg) {
+    return cfg.sdf_program_count < 64u ? cfg.sdf_program_count : 64u;
+}
+
+__device__ __forceinline__ float distance_sdf(f3 p, const FptRenderConfig &cfg, Work &w) {
+    w.distance_evals++;
+    if (is_program(cfg)) return program_eval<false>(cfg.sdf_program, prog_count(cfg), p, nullptr, &w.orbit_iterations);
+    if (cfg.sdf_id == FPT_SDF_MANDELBOX_FRACTAL) return de_mandelbox_builtin(p, cfg, nullptr, &w.orbit_iterations);
+    if (cfg.sdf_id == FPT_SDF_MENGER_SPONGE) return de_menger_builtin(p, cfg, &w.orbit_iterations);
+    return 1000.0f;
+}
+
+__device__ __forceinline__ float map_sdf(f3 p, const FptRenderConfig &cfg, Work &w) {  // SM:3612-3615
+    float d = distance_sdf(p, cfg, w);
+    return mixf(d, d / 2.0f, cfg.render[6]);
+}
+
+__device__ __forceinline__ f3 hsv2rgb(f3 c) {  // SM:534-539
+    f3 p = mk(fabsf(fractf(c.x + 1.0f) * 6.0f - 3.0f), fabsf(fractf(c.x + 2.0f / 3.0f) * 6.0f - 3.0f),
+              fabsf(fractf(c.x + 1.0f / 3.0f) * 6.0f - 3.0f));
+    f3 q = mk(clampf(p.x - 1.f, 0.f, 1.f), clampf(p.y - 1.f, 0.f, 1.f), clampf(p.z - 1.f, 0.f, 1.f));
+    return vmix(mk(1.f, 1.f, 1.f), q, c.y) * c.z;
+}
+
+__device__ f3 program_gradient(float value, const FptRenderConfig &cfg) {  // SM:3418-3432
+    uint32_t count = cfg.gradient_count < 16u ? cfg.gradient_count : 16u;
+    if (count == 0u) return ld3(cfg.program_material);
+    if (count == 1u) return ld3(&cfg.gradient_stops[0][1]);
+    float t = fractf(value);
+    const float *prev = cfg.gradient_stops[0];
+    for (uint32_t i = 1; i < count; ++i) {
+        const float *cur = cfg.gradient_stops[i];
+        if (t <= cur[0]) {
+            float f = clampf((t - prev[0]) / fmaxf(cur[0] - prev[0], 1.0e-6f), 0.f, 1.f);
+            return vmix(ld3(prev + 1), ld3(cur + 1), f);
+        }
+        prev = cur;
+    }
+    return ld3(prev + 1);
+}
+
+__device__ Material user_material(f3 p, const FptRenderConfig &cfg, Work &w) {  // SM:3434-3483, 3485-3562
+    Material m;
+    m.rgb = mk(1.f, 1.f, 1.f);
+    m.roughness = 1.f; m.specular = 0.f; m.translucency = 0.f; m.ior = 1.5f; m.emission = 0.f;
+    if (is_program(cfg)) {
+        float orbit = 0.f;
+        // The colour depends on the orbit only in modes 1 and 2; mode 0 is
+        // constant, so the shading program is not evaluated for it.
+        if (cfg.material_mode == 1u || cfg.material_mode == 2u) {
+            uint32_t n = cfg.sdf_shading_program_count < 64u ? cfg.sdf_shading_program_count : 64u;
+            program_eval<true>(cfg.sdf_shading_program, n, p, &orbit, &w.orbit_iterations);
+        }
+        float o = orbit * (1.0f + cfg.set_values[1]) + cfg.set_values[0];
+        if (cfg.material_mode == 1u) m.rgb = hsv2rgb(mk(o, 0.5f, 1.0f));
+        else if (cfg.material_mode == 2u) m.rgb = program_gradient(o, cfg);
+        else m.rgb = ld3(cfg.program_material);
+        m.roughness = cfg.program_material[3];
+        m.specular = cfg.program_material[4];
+        m.translucency = cfg.program_material[5];
+        m.ior = cfg.program_material[6];
+        m.emission = cfg.program_material[7];
+        return m;
+    }
+    float orbit = 0.f;
+    bool menger = cfg.sdf_id == FPT_SDF_MENGER_SPONGE;
+    if (cfg.sdf_id == FPT_SDF_MANDELBOX_FRACTAL) de_mandelbox_builtin(p, cfg, &orbit, &w.orbit_iterations);
+    if (menger) orbit = 1.0f;
+    if (cfg.fractal_style_mode != 0u) {  // SM:3451-3468
+        float o = orbit * cfg.fractal_style[1] + cfg.fractal_style[0];
+        if (cfg.fractal_style_mode == 1u) m.rgb = hsv2rgb(mk(o, cfg.fractal_style[2], cfg.fractal_style[3]));
+        else if (cfg.fractal_style_mode == 2u) m.rgb = ld3(&cfg.fractal_style[8]) * cfg.fractal_style[3];
+        else m.rgb = program_gradient(o, cfg) * cfg.fractal_style[3];
+        m.rgb = vmix(m.rgb, ld3(&cfg.fractal_style[8]), clampf(cfg.fractal_style[7], 0.f, 1.f));
+        m.roughness = cfg.fractal_style[4];
+        m.specular = cfg.fractal_style[5];
+        m.emission = cfg.fractal_style[6];
+        return m;
+    }
+    if (menger) return m;
+    float o = orbit * (1.0f + cfg.set_values[1]) + cfg.set_values[0];
+    m.rgb = hsv2rgb(mk(o, 0.5f, 1.0f));
+    return m;
+}
+
+// march (SM:3753-3783), non-Mandelbulber branch.
+__device__ f3 march(f3 dr, f3 rp, int ni, float min_dist, float lod_falloff, const FptRenderConfig &cfg,
+                    Work &w, int &steps) {
+    const f3 cam = rp;
+    const int max_iter = ni < 360 ? ni : 360;
+    const bool check_t = cfg.sdf_id == FPT_SDF_PROGRAM && cfg.program_material[5] > 0.0f;
+    const float adapt = cfg.render[5];
+    const float maxd = cfg.render[4];
+    steps = 0;
+    for (int i = 0; i < max_iter; ++i) {
+        float d = map_sdf(rp, cfg, w);
+        ++steps;
+        if (!isfinite(d)) break;
+        float o = fabsf(d) * 0.99f;
+        rp = rp + dr * o;
+        float lod = min_dist;
+        if (lod_falloff > 0.00001f) {
+            f3 df = cam - rp;
+            float l2 = mixf(lod, 0.1f, dot(df, df) * lod_falloff);
+            lod = mixf(min_dist, l2, adapt);
+        }
+        if (check_t && user_material(rp, cfg, w).translucency > 0.0f) lod = 0.0001f;
+        if (o < lod) break;
+        if (maxd < o) break;
+    }
+    return rp;
+}
+
+__device__ __forceinline__ f3 normalize_fd(f3 g) {  // SM:3863-3869
+    float m = fmaxf(fabsf(g.x), fmaxf(fabsf(g.y), fabsf(g.z)));
+    if (!(m > 0.0f) || !isfinite(m)) return mk(0.f, 1.f, 0.f);
+    return normalize(g / m);
+}
+
+// normalAt (SM:3871-3971), non-Mandelbulber branches.
+__device__ f3 normal_at(f3 p, const FptRenderConfig &cfg, Work &w) {
+    const float e = fmaxf(cfg.render[2], 0.0002f);
+    if ((cfg.sdf_normal_mode == 0u || cfg.sdf_normal_mode == 2u) && is_program(cfg)) {
+        f3 g;
+        program_surface(cfg.sdf_program, prog_count(cfg), p, g);
+        if (dot(g, g) >= 1.0e-12f && finite3(g)) return normalize(g);
+    }
+    if (cfg.sdf_normal_mode == 1u) {
+        float te = e * 0.57735026919f;
+        f3 k0 = mk(1.f, -1.f, -1.f), k1 = mk(-1.f, -1.f, 1.f), k2 = mk(-1.f, 1.f, -1.f), k3 = mk(1.f, 1.f, 1.f);
+        float d0 = map_sdf(p + k0 * te, cfg, w), d1 = map_sdf(p + k1 * te, cfg, w);
+        float d2 = map_sdf(p + k2 * te, cfg, w), d3 = map_sdf(p + k3 * te, cfg, w);
+        return normalize_fd(k0 * d0 + k1 * d1 + k2 * d2 + k3 * d3);
+    }
+    f3 ex = mk(e, 0.f, 0.f), ey = mk(0.f, e, 0.f), ez = mk(0.f, 0.f, e);
+    f3 n = mk(map_sdf(p + ex, cfg, w) - map_sdf(p - ex, cfg, w), map_sdf(p + ey, cfg, w) - map_sdf(p - ey, cfg, w),
+              map_sdf(p + ez, cfg, w) - map_sdf(p - ez, cfg, w));
+    return normalize_fd(n);
+}
+
+// ---------------------------------------------------------------------------
+// Lighting (SM:5764-5861).
+// ---------------------------------------------------------------------------
+__device__ f3 sky(f3 dr, f3 sun) {  // SM:5764-5787
+    const float T = 2.0f;
+    float A = 0.1787f * T - 1.4630f, B = -0.3554f * T + 0.4275f, C = -0.0227f * T + 5.3251f;
+    float D = 0.1206f * T - 2.5771f, E = -0.0670f * T + 0.3703f;
+    float theta = acosf(clampf(dr.y, -1.f, 1.f));
+    float gamma = acosf(clampf(dot(dr, sun), -1.f, 1.f));
+    float cg = cosf(gamma);
+    float sk = (1.0f + A * expf(B / fmaxf(0.1f, cosf(theta)))) * (1.0f + C * expf(D * gamma) + E * cg * cg);
+    f3 col = vmix(mk(0.004f, 0.048f, 0.253f), mk(0.8f, 0.9f, 1.0f), sk);
+    float hd = (dr.y + 1.0f) / 2.0f, hs = (sun.y + 1.0f) / 2.0f;
+    col = vmix(mk(1.0f, 0.85f, 0.53f) / 4.0f, col * 1.2f, hd) * hs;
+    col = (col - mk(0.5f, 0.5f, 0.5f)) * 1.2f + mk(0.5f, 0.5f, 0.5f);
+    float disk = fmaxf((dot(dr, sun) - 1.0f) / (1.0f - cosf(0.05f)) + 1.0f, 0.0f);
+    disk *= powf(hd, 10.0f);
+    return col * 0.8f + mk(disk, disk, disk) * 100.0f;
+}
+
+__device__ f3 environment(f3 view, const FptRenderConfig &cfg) {  // SM:5823-5844 (no HDRI)
+    if (cfg.world[4] < 0.00001f) return mk(0.f, 0.f, 0.f);
+    f3 ld = rotate_camera(mk(0.f, 0.f, 1.f), cfg.world[2] * kPi / 180.0f, cfg.world[3] * kPi / 180.0f);
+    f3 env;
+    if (cfg.world[0] == 0.0f) {
+        float ls = fmaxf(cfg.world[1], 0.0001f);
+        float l = fmaxf((dot(view, ld) - 1.0f) / (1.0f - cosf(ls)) + 1.0f, 0.0f) / ls * 3.0f;
+        env = mk(l, l, l);
+    } else if (cfg.world[0] == 1.0f) {
+        env = sky(view, ld);
+    } else if (cfg.world[0] == 3.0f) {
+        env = ld3(cfg.world_one_color);
+    } else {
+        env = mk(1.f, 1.f, 1.f);
+    }
+    float contrast = cfg.world[0] == 3.0f ? 1.0f : cfg.world[5];
+    return vmax0(env * cfg.world[4] - mk(0.5f, 0.5f, 0.5f) * contrast + mk(0.5f, 0.5f, 0.5f));
+}
+
+}  // namespace fpt

@@ -1,0 +1,293 @@
+"""ctypes mirror of the C ABI in include/fpt_abi.h.
+
+This is the Python counterpart of the reference's src/ffi.rs:1-521: the same
+structs (field for field, so sizes match src/ffi.rs:494-507) and the same
+entry points, bound to libfpt_hip.so.  Loading fails loudly when the HIP
+library has not been built; there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+MAX_OPS = 64
+MAX_PRIMS = 64
+MAX_STOPS = 16
+
+# FptSdfId (metal_bridge.h:10-24)
+SDF_CORNELL_BOX = 0
+SDF_GLASS_BALL = 1
+SDF_BALL_FRACTAL = 2
+SDF_CAGE_FRACTAL = 3
+SDF_IFS_FRACTAL = 4
+SDF_MANDELBOX_FRACTAL = 5
+SDF_MENGER_SPONGE = 6
+SDF_TOWER_FRACTAL = 7
+SDF_TREE_FRACTAL = 8
+SDF_PROGRAM = 9
+SDF_README_CORNELL = 10
+SDF_README_GLASS = 11
+SDF_MANDELBULBER = 12
+
+# FptSdfOpcode (metal_bridge.h:40-56 plus the fractal opcodes of fpt_abi.h)
+OP_ABS = 1
+OP_TRANSLATE = 2
+OP_SCALE = 3
+OP_ROTATE_X = 4
+OP_ROTATE_Y = 5
+OP_ROTATE_Z = 6
+OP_REPEAT = 7
+OP_SORT_DESC = 8
+OP_BOX_FOLD = 9
+OP_SPHERE = 16
+OP_BOX = 17
+OP_PLANE = 18
+OP_MANDELBULB = 19
+OP_MANDELBOX = 20
+OP_MENGER = 21
+OP_UNION = 24
+OP_INTERSECTION = 25
+OP_SUBTRACT = 26
+OP_ORBIT_ADD = 32
+
+RENDERER_SDF = 0
+RENDERER_VOXEL = 1
+
+VOXEL_STORAGE_DENSE = 0
+VOXEL_STORAGE_SPARSE_BRICKS = 1
+VOXEL_STORAGE_SPARSE_BRICKS8 = 8
+
+u32 = C.c_uint32
+f32 = C.c_float
+
+
+class FptSdfInstruction(C.Structure):
+    _fields_ = [("opcode", u32), ("flags", u32), ("material_index", u32), ("_pad0", u32),
+                ("data", f32 * 4)]
+
+
+class FptPrimitiveInstance(C.Structure):
+    _fields_ = [("transform", f32 * 12), ("data", f32 * 4), ("opcode", u32),
+                ("distance_scale", f32), ("source_instruction", u32), ("_pad0", u32)]
+
+
+class FptAffineTransform(C.Structure):
+    _fields_ = [("transform", f32 * 12), ("distance_scale", f32), ("_pad0", u32 * 3)]
+
+
+class FptIndexedPrimitive(C.Structure):
+    _fields_ = [("data", f32 * 4), ("opcode", u32), ("transform_index", u32),
+                ("source_instruction", u32), ("combine_mode", u32)]
+
+
+class FptTypedSoAProgram(C.Structure):
+    _fields_ = [("sphere_count", u32), ("box_count", u32), ("plane_count", u32), ("_pad0", u32)] + [
+        (name, (u32 if name.endswith("source") else f32) * MAX_PRIMS)
+        for name in ("sphere_x", "sphere_y", "sphere_z", "sphere_radius", "sphere_source",
+                     "box_x", "box_y", "box_z", "box_half_x", "box_half_y", "box_half_z", "box_source",
+                     "plane_x", "plane_y", "plane_z", "plane_center_x", "plane_center_y",
+                     "plane_center_z", "plane_offset", "plane_source")]
+
+
+class FptRenderConfig(C.Structure):
+    _fields_ = [
+        ("width", u32), ("height", u32), ("samples", u32), ("preview", u32), ("sdf_id", u32),
+        ("glass_mode", u32), ("sdf_accumulation_mode", u32), ("sdf_profile", u32),
+        ("sdf_bounce_cap", u32), ("sdf_russian_roulette", u32), ("sdf_normal_mode", u32),
+        ("sdf_chunk_samples", u32), ("sdf_rr_start", f32), ("sdf_rr_min_prob", f32),
+        ("mandel_iteration_scale", f32),
+        ("camera_position", f32 * 3), ("camera_yaw_pitch", f32 * 2), ("camera_roll", f32),
+        ("camera_fov", f32), ("camera_dof", f32), ("focus_distance", f32),
+        ("render", f32 * 8), ("world", f32 * 7), ("world_one_color", f32 * 3), ("sun", f32 * 5),
+        ("sun_color", f32 * 3), ("background_gradient", f32 * 6), ("post", f32 * 7),
+        ("set_values", f32 * 40), ("vset_values", f32 * 133),
+        ("sdf_program_count", u32), ("gradient_count", u32), ("material_mode", u32),
+        ("hdri_enabled", u32), ("hdri_width", u32), ("hdri_height", u32), ("_pad_hdri", u32 * 2),
+        ("fractal_style_mode", u32), ("_pad_style", u32 * 3), ("fractal_style", f32 * 12),
+        ("program_material", f32 * 8), ("sdf_program", FptSdfInstruction * MAX_OPS),
+        ("gradient_stops", (f32 * 4) * MAX_STOPS), ("hdri_path", C.c_uint8 * 256),
+        ("hdri_lut", C.c_uint16 * (32 * 16 * 3)),
+        ("renderer_backend", u32), ("voxel_resolution", u32), ("voxel_normal_mode", u32),
+        ("voxel_storage", u32), ("voxel_bounds_min", f32 * 3), ("voxel_surface_band", f32),
+        ("voxel_bounds_max", f32 * 3), ("voxel_fill_interior", u32), ("voxel_coverage_mode", u32),
+        ("voxel_build_mode", u32), ("voxel_brick_rejection", u32), ("voxel_leaf_refinement", u32),
+        ("voxel_material_mode", u32), ("voxel_offset_mode", u32), ("bound_grid_resolution", u32),
+        ("bound_grid_profile", u32), ("bound_grid_profile_stride", u32),
+        ("bound_grid_cage_bounds", u32), ("bound_grid_directional", u32),
+        ("sdf_program_source_count", u32), ("sdf_program_optimization", u32),
+        ("regional_program_resolution", u32), ("bound_grid_fp16", u32),
+        ("sdf_flat_union_count", u32),
+        ("sdf_flat_union_instances", FptPrimitiveInstance * MAX_PRIMS),
+        ("sdf_shading_program_count", u32), ("sdf_geometry_split", u32),
+        ("sdf_shading_program", FptSdfInstruction * MAX_OPS),
+        ("sdf_topology_specialization", u32), ("sdf_runtime_source_bytecode", u32),
+        ("sdf_function_stitching", u32), ("sdf_stitched_surface", u32),
+        ("sdf_stitch_validation", u32), ("sdf_stitch_distance_only", u32),
+        ("sdf_stitch_split_graph", u32), ("sdf_stitch_fusion", u32), ("sdf_canonical_count", u32),
+        ("sdf_canonical_source_count", u32),
+        ("sdf_canonical_primitives", FptPrimitiveInstance * MAX_PRIMS),
+        ("sdf_canonical_transform_count", u32), ("_pad_canonical_indexed", u32 * 3),
+        ("sdf_canonical_transforms", FptAffineTransform * MAX_PRIMS),
+        ("sdf_indexed_primitives", FptIndexedPrimitive * MAX_PRIMS),
+        ("sdf_typed_soa", FptTypedSoAProgram),
+    ]
+
+
+class FptBoundGridStats(C.Structure):
+    _fields_ = [(n, C.c_uint64 * 3) for n in (
+        "macro_cells", "certified_skips", "candidate_intervals", "candidate_misses",
+        "candidate_hits", "unknown_intervals", "field_evaluations", "directional_steps",
+        "cell_exit_clamps", "unknown_derivative_intervals")] + [(n, C.c_uint64) for n in (
+            "profiled_paths", "certified_cells", "unknown_cells", "sampled_bound_failures",
+            "sampled_false_skips", "certified_derivative_cells", "unknown_derivative_cells",
+            "sampled_derivative_failures", "regional_cells", "regional_fallback_cells",
+            "regional_unique_programs", "regional_retained_instructions",
+            "regional_sampled_distance_failures", "regional_profiled_paths")] + [
+        (n, C.c_uint64 * 3) for n in (
+            "regional_distance_evaluations", "regional_atlas_evaluations",
+            "regional_full_program_evaluations", "regional_cell_entries",
+            "regional_same_cell_reuses", "regional_same_program_reuses",
+            "regional_program_id_loads", "regional_header_loads",
+            "regional_dynamic_instructions")] + [(n, C.c_uint64) for n in (
+                "regional_pruned_cells", "regional_pruned_primitives",
+                "regional_repeat_seam_fallback_cells", "regional_unsupported_fallback_cells",
+                "regional_no_dominance_cells")]
+
+
+class FptStitchValidationStats(C.Structure):
+    _fields_ = [("sample_count", C.c_uint64), ("distance_failures", C.c_uint64),
+                ("gradient_failures", C.c_uint64), ("max_distance_error", f32),
+                ("max_gradient_error", f32)]
+
+
+class FptStitchPipelineStats(C.Structure):
+    _fields_ = [(n, u32) for n in (
+        "instruction_count", "transform_instruction_count", "primitive_instruction_count",
+        "primitive_type_runs", "union_count", "intersection_count", "subtraction_count",
+        "thread_execution_width", "max_total_threads_per_threadgroup", "graph_node_count")] + [
+        ("static_threadgroup_memory_bytes", C.c_uint64), ("runtime_source_bytes", C.c_uint64),
+        ("runtime_library_compile_ms", C.c_double), ("runtime_pipeline_link_ms", C.c_double)]
+
+
+class FptSdfProfileStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in (
+        "primary_steps", "secondary_steps", "shadow_steps", "normal_evals", "bounces", "pixels",
+        "distance_evals", "march_orbit_iterations", "refinement_steps", "normal_field_evals",
+        "material_evals", "max_ray_steps", "max_pixel_steps")] + [
+        ("distance_evals_by_phase", C.c_uint64 * 4), ("orbit_iterations_by_phase", C.c_uint64 * 4),
+        ("formula_slot_iterations", C.c_uint64 * 9), ("refinement_distance_evals", C.c_uint64)] + [
+        (n, C.c_double) for n in ("primary_ms_estimate", "secondary_ms_estimate",
+                                  "shadow_ms_estimate", "normal_ms_estimate",
+                                  "bounce_ms_estimate")]
+
+
+class FptAsyncJitStats(C.Structure):
+    _fields_ = [("fallback_render_ms", C.c_double), ("jit_build_ms", C.c_double),
+                ("stitched_render_ms", C.c_double), ("cache_status", u32),
+                ("fallback_completed_before_jit", u32), ("max_absolute_error", f32),
+                ("_pad0", u32)]
+
+
+class FptDiagnosticConfig(C.Structure):
+    _fields_ = [("mode", u32), ("_pad0", u32), ("max_distance", f32), ("normal_mix", f32),
+                ("dispatch_origin", u32 * 2)]
+
+
+class FptMandelbulberFieldSample(C.Structure):
+    _fields_ = [("distance", f32), ("radius", f32), ("derivative", f32), ("iterations", f32)]
+
+
+class FptRayHit(C.Structure):
+    _fields_ = [("t", f32), ("position", f32 * 3), ("normal", f32 * 3), ("hit", u32),
+                ("steps", u32), ("_pad0", u32)]
+
+
+# Expected sizes (src/ffi.rs:494-507 plus the new FptRayHit).
+EXPECTED_SIZES = {
+    FptSdfInstruction: 32, FptPrimitiveInstance: 80, FptAffineTransform: 64,
+    FptIndexedPrimitive: 32, FptTypedSoAProgram: 5136, FptStitchPipelineStats: 72,
+    FptRenderConfig: 30448, FptSdfProfileStats: 288, FptDiagnosticConfig: 24,
+    FptMandelbulberFieldSample: 16, FptAsyncJitStats: 40, FptRayHit: 40,
+}
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "libfpt_hip.so"
+
+# Symbols declared in include/fpt_abi.h.
+EXPORTED_SYMBOLS = (
+    "fpt_metal_render", "fpt_metal_diagnostic_render", "fpt_mandelbulber_sample_field",
+    "fpt_compare_images", "fpt_metal_device_name", "fpt_test_voxel_dda",
+    "fpt_test_async_stitch_context", "fpt_test_typed_soa", "fpt_metal_preview",
+    "fpt_hip_trace_primary", "fpt_hip_plan_create", "fpt_hip_plan_render",
+    "fpt_hip_plan_destroy", "fpt_hip_last_error",
+)
+
+_lib = None
+
+
+def _declare(lib):
+    P = C.POINTER
+    cp = C.c_char_p
+    lib.fpt_metal_render.argtypes = [cp, cp, cp, cp, cp, C.c_size_t, P(FptRenderConfig),
+                                     P(C.c_double), P(C.c_double), P(C.c_uint64), P(u32),
+                                     P(C.c_uint64), P(u32), P(FptBoundGridStats), P(u32),
+                                     P(FptStitchValidationStats), P(FptStitchPipelineStats),
+                                     P(FptSdfProfileStats), P(f32), C.c_size_t, C.c_char_p,
+                                     C.c_size_t]
+    lib.fpt_metal_render.restype = C.c_int
+    lib.fpt_metal_diagnostic_render.argtypes = [cp, cp, cp, C.c_size_t, P(FptRenderConfig),
+                                                P(FptDiagnosticConfig), P(C.c_double), cp,
+                                                C.c_size_t]
+    lib.fpt_metal_diagnostic_render.restype = C.c_int
+    lib.fpt_mandelbulber_sample_field.argtypes = [cp, cp, C.c_size_t, P(FptRenderConfig),
+                                                  P(f32), C.c_size_t,
+                                                  P(FptMandelbulberFieldSample), cp, C.c_size_t]
+    lib.fpt_mandelbulber_sample_field.restype = C.c_int
+    lib.fpt_compare_images.argtypes = [cp, cp, cp, cp, C.c_size_t]
+    lib.fpt_compare_images.restype = C.c_int
+    lib.fpt_metal_device_name.argtypes = [cp, C.c_size_t]
+    lib.fpt_metal_device_name.restype = C.c_int
+    lib.fpt_test_voxel_dda.argtypes = [cp, cp, C.c_size_t]
+    lib.fpt_test_voxel_dda.restype = C.c_int
+    lib.fpt_hip_trace_primary.argtypes = [P(FptRenderConfig), P(FptRayHit), C.c_size_t, cp,
+                                          C.c_size_t]
+    lib.fpt_hip_trace_primary.restype = C.c_int
+    lib.fpt_hip_plan_create.argtypes = [P(FptRenderConfig), P(C.c_void_p), cp, C.c_size_t]
+    lib.fpt_hip_plan_create.restype = C.c_int
+    lib.fpt_hip_plan_render.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, cp, C.c_size_t]
+    lib.fpt_hip_plan_render.restype = C.c_int
+    lib.fpt_hip_plan_destroy.argtypes = [C.c_void_p]
+    lib.fpt_hip_plan_destroy.restype = None
+    lib.fpt_hip_last_error.argtypes = []
+    lib.fpt_hip_last_error.restype = C.c_char_p
+
+
+def load_library(path: os.PathLike | None = None):
+    """Load libfpt_hip.so (built in-tree by fractalpathtracer_amd.build)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path is not None else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(
+            f"{p} is missing: build the HIP library first "
+            "(python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(str(p))
+    _declare(lib)
+    if path is None:
+        _lib = lib
+    return lib
+
+
+class FptError(RuntimeError):
+    pass
+
+
+def error_buffer(n: int = 4096):
+    return C.create_string_buffer(n)
+
+
+def check(status: int, err) -> None:
+    if status != 0:
+        raise FptError(err.value.decode(errors="replace"))

@@ -1,0 +1,141 @@
+"""ctypes binding of the CPU oracle (oracle/liborc.so).
+
+ORACLE — test infrastructure only.  Imported by tests/, by
+__graft_entry__.smoke() and by bench.py's CPU-baseline leg, always as the
+checker, never as the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liborc.so"
+_lib = None
+
+
+def build(force: bool = False) -> Path:
+    """Compile the oracle with its Makefile (g++, OpenMP)."""
+    args = ["make", "-s", "-C", str(HERE)]
+    if force:
+        args.insert(1, "-B")
+    subprocess.run(args, check=True)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = C.CDLL(str(LIB))
+        P = C.POINTER
+        dp = P(C.c_double)
+        L.orc_render.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_uint32, C.c_uint32, dp]
+        L.orc_trace_primary.argtypes = [C.c_void_p, C.c_int, C.c_int, dp]
+        L.orc_map_distance.argtypes = [C.c_void_p, C.c_int, dp, C.c_size_t, dp]
+        L.orc_normal.argtypes = [C.c_void_p, C.c_int, dp, C.c_size_t, dp]
+        L.orc_program_surface.argtypes = [C.c_void_p, C.c_int, dp, C.c_size_t, dp]
+        L.orc_leaf_orbit.argtypes = [C.c_uint32, P(C.c_float), C.c_int, dp, dp]
+        L.orc_mandelbulb_step.argtypes = [dp, C.c_double]
+        L.orc_mandelbox_step.argtypes = [dp, C.c_double, C.c_double, C.c_double]
+        L.orc_hash13_bits.argtypes = [C.c_float] * 3
+        L.orc_hash13_bits.restype = C.c_uint32
+        L.orc_hash13.argtypes = [C.c_float] * 3
+        L.orc_hash13.restype = C.c_float
+        L.orc_last_counters.argtypes = [P(C.c_uint64)]
+        _voxel_decls(L)
+        _lib = L
+    return _lib
+
+
+def _voxel_decls(L):
+    P = C.POINTER
+    if hasattr(L, "orc_voxel_build"):
+        L.orc_voxel_build.argtypes = [C.c_void_p, C.c_int, P(C.c_uint32), C.c_size_t,
+                                      P(C.c_uint16), P(C.c_uint8), C.c_size_t, P(C.c_uint64)]
+        L.orc_voxel_build.restype = C.c_int
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def render(cfg, precision: int = 64, threads: int = 0, rows=None) -> np.ndarray:
+    """Linear float4 accumulation (H, W, 4) in float64; rows=(y0, y1) renders a band."""
+    y0, y1 = (0, cfg.height) if rows is None else rows
+    out = np.zeros((cfg.height, cfg.width, 4), np.float64)
+    rc = lib().orc_render(C.addressof(cfg), precision, threads, y0, y1, _dp(out))
+    if rc != 0:
+        raise RuntimeError("orc_render failed")
+    return out
+
+
+def counters() -> dict:
+    a = (C.c_uint64 * 7)()
+    lib().orc_last_counters(a)
+    keys = ("distance_evals", "orbit_iterations", "primary_steps", "secondary_steps",
+            "shadow_steps", "normal_evals", "paths")
+    return dict(zip(keys, list(a)))
+
+
+def trace_primary(cfg, precision: int = 64, threads: int = 0) -> np.ndarray:
+    """(H, W, 9): t, px, py, pz, nx, ny, nz, hit, steps."""
+    out = np.zeros((cfg.height, cfg.width, 9), np.float64)
+    if lib().orc_trace_primary(C.addressof(cfg), precision, threads, _dp(out)) != 0:
+        raise RuntimeError("orc_trace_primary failed")
+    return out
+
+
+def map_distance(cfg, points, precision: int = 64) -> np.ndarray:
+    p = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+    out = np.zeros(len(p), np.float64)
+    lib().orc_map_distance(C.addressof(cfg), precision, _dp(p), len(p), _dp(out))
+    return out
+
+
+def normal(cfg, points, precision: int = 64) -> np.ndarray:
+    p = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+    out = np.zeros((len(p), 3), np.float64)
+    lib().orc_normal(C.addressof(cfg), precision, _dp(p), len(p), _dp(out))
+    return out
+
+
+def program_surface(cfg, points, precision: int = 64) -> np.ndarray:
+    p = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+    out = np.zeros((len(p), 4), np.float64)
+    lib().orc_program_surface(C.addressof(cfg), precision, _dp(p), len(p), _dp(out))
+    return out
+
+
+def leaf_orbit(opcode: int, data, point, precision: int = 64) -> dict:
+    d = (C.c_float * 4)(*[float(x) for x in (list(data) + [0.0] * 4)[:4]])
+    p = np.ascontiguousarray(point, np.float64)
+    out = np.zeros(9, np.float64)
+    if lib().orc_leaf_orbit(opcode, d, precision, _dp(p), _dp(out)) != 0:
+        raise ValueError("not a fractal leaf opcode")
+    return dict(distance=out[0], radius=out[1], derivative=out[2], iterations=int(out[3]),
+                escaped=bool(out[4]), z=out[5:8].copy(), trap=out[8])
+
+
+def mandelbulb_step(z, power: float, derivative: float = 1.0):
+    s = np.array([z[0], z[1], z[2], np.sqrt(np.dot(z, z)), derivative], np.float64)
+    lib().orc_mandelbulb_step(_dp(s), power)
+    return s[:3].copy(), s[4]
+
+
+def mandelbox_step(z, scale: float, min_r2: float, fixed_r2: float, derivative: float = 1.0):
+    s = np.array([z[0], z[1], z[2], np.sqrt(np.dot(z, z)), derivative], np.float64)
+    lib().orc_mandelbox_step(_dp(s), scale, min_r2, fixed_r2)
+    return s[:3].copy(), s[4]
+
+
+def hash13_bits(x: float, y: float, z: float) -> int:
+    return int(lib().orc_hash13_bits(x, y, z))
+
+
+def hash13(x: float, y: float, z: float) -> float:
+    return float(lib().orc_hash13(x, y, z))
