@@ -82,8 +82,6 @@ def sample_field(cfg: FptRenderConfig, points) -> np.ndarray:
 def program_surface(cfg: FptRenderConfig, points) -> np.ndarray:
     """fpt_hip_program_surface: (N, 4) {distance, gx, gy, gz}."""
     lib = ffi.load_library()
-    lib.fpt_hip_program_surface.argtypes = [C.POINTER(FptRenderConfig), C.POINTER(C.c_float), C.c_size_t,
-                                            C.POINTER(C.c_float), C.c_char_p, C.c_size_t]
     p = np.zeros((len(points), 4), np.float32)
     p[:, :3] = np.asarray(points, np.float32).reshape(-1, 3)
     out = np.zeros((len(p), 4), np.float32)
@@ -91,6 +89,63 @@ def program_surface(cfg: FptRenderConfig, points) -> np.ndarray:
     ffi.check(lib.fpt_hip_program_surface(C.byref(cfg), p.ctypes.data_as(C.POINTER(C.c_float)), len(p),
                                           out.ctypes.data_as(C.POINTER(C.c_float)), err, len(err)), err)
     return out
+
+
+RECORD_DTYPE = np.dtype([("packed_color", "<u4"), ("packed_properties", "<u4"), ("emission", "<f4")])
+
+
+def voxel_build(cfg: FptRenderConfig) -> dict:
+    """fpt_hip_voxel_build: the GPU's voxel tables as numpy arrays."""
+    lib = ffi.load_library()
+    info = ffi.FptVoxelBuildInfo()
+    err = ffi.error_buffer()
+    ffi.check(lib.fpt_hip_voxel_build(C.byref(cfg), None, 0, None, 0, None, 0, None, 0, None, 0,
+                                      C.byref(info), err, len(err)), err)
+    B, G, pages = info.brick_size, info.bricks_per_axis, info.active_bricks
+    res = cfg.voxel_resolution
+    cpb = B ** 3
+    pt = np.zeros(max(G ** 3, 1), np.uint32)
+    nrec = res ** 3 if info.storage == ffi.VOXEL_STORAGE_DENSE else (
+        pages * cpb if info.storage == ffi.VOXEL_STORAGE_SPARSE_BRICKS else 0)
+    rec = np.zeros(max(nrec, 1), RECORD_DTYPE)
+    dist = np.zeros(max(pages * cpb, 1), np.uint16)
+    mat = np.zeros(max(pages * cpb, 1), np.uint8)
+    pal = np.zeros(256 * 8, np.float32)
+    P = C.POINTER
+    ffi.check(lib.fpt_hip_voxel_build(C.byref(cfg), pt.ctypes.data_as(P(C.c_uint32)), len(pt),
+                                      rec.ctypes.data_as(C.c_void_p), len(rec),
+                                      dist.ctypes.data_as(P(C.c_uint16)), len(dist),
+                                      mat.ctypes.data_as(P(C.c_uint8)), len(mat),
+                                      pal.ctypes.data_as(P(C.c_float)), len(pal), C.byref(info), err,
+                                      len(err)), err)
+    return dict(storage=info.storage, brick_size=B, bricks_per_axis=G, active_bricks=pages,
+                active_cells=info.active_cells, memory_bytes=info.memory_bytes, build_ms=info.build_ms,
+                page_table=pt[:G ** 3], records=rec[:nrec], distance=dist[:pages * cpb],
+                material=mat[:pages * cpb], palette=pal.reshape(256, 8))
+
+
+def voxel_trace_cells(cfg: FptRenderConfig, rays, cap: int = 4096):
+    """fpt_hip_voxel_trace_cells: (cells (n, cap, 3), counts (n,), hits (n, 8))."""
+    lib = ffi.load_library()
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+    n = len(rays)
+    cells = np.zeros((n, cap, 3), np.int32)
+    counts = np.zeros(n, np.uint32)
+    hits = np.zeros((n, 8), np.float32)
+    err = ffi.error_buffer()
+    P = C.POINTER
+    ffi.check(lib.fpt_hip_voxel_trace_cells(C.byref(cfg), rays.ctypes.data_as(P(C.c_float)), n, cap,
+                                            cells.ctypes.data_as(P(C.c_int32)),
+                                            counts.ctypes.data_as(P(C.c_uint32)),
+                                            hits.ctypes.data_as(P(C.c_float)), err, len(err)), err)
+    return cells, counts, hits
+
+
+def test_voxel_dda() -> None:
+    """fpt_test_voxel_dda (the reference's GPU contract test, ffi.rs:509-520)."""
+    lib = ffi.load_library()
+    err = ffi.error_buffer(512)
+    ffi.check(lib.fpt_test_voxel_dda(None, err, len(err)), err)
 
 
 def device_name() -> str:

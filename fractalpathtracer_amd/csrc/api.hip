@@ -236,7 +236,7 @@ int fpt_hip_plan_render(FptHipPlan *plan, float *device_accum, void *stream, cha
     float4 *acc = reinterpret_cast<float4 *>(device_accum);
     for (uint32_t start = 0; start < spp; start += chunk) {
         hipError_t e = c.renderer_backend == FPT_RENDERER_VOXEL
-                           ? fpt::launch_voxel_accumulate(plan->dcfg, c, plan->voxels, acc, start, chunk, s)
+                           ? fpt::launch_voxel_accumulate(plan->dcfg, c, plan->voxels, acc, start, chunk, s, nullptr)
                            : fpt::launch_sdf_accumulate(plan->dcfg, c, acc, start, chunk, nullptr, s);
         HIP_TRY(e, "render kernel launch");
     }
@@ -435,6 +435,108 @@ int fpt_hip_program_surface(const FptRenderConfig *config, const float *points_x
             "surface kernel launch");
     HIP_TRY(hipMemcpyAsync(out_dgrad, dout.p, point_count * 16, hipMemcpyDeviceToHost, st.s), "readback");
     HIP_TRY(hipStreamSynchronize(st.s), "surface sampling");
+    return 0;
+}
+
+int fpt_hip_voxel_build(const FptRenderConfig *config, uint32_t *page_table, size_t page_table_len, void *records,
+                        size_t records_len, uint16_t *distance, size_t distance_len, uint8_t *material,
+                        size_t material_len, float *palette, size_t palette_len, FptVoxelBuildInfo *info,
+                        char *error, size_t error_len) {
+    if (!config) {
+        set_error(error, error_len, "missing render config");
+        return 1;
+    }
+    if (check_device(error, error_len)) return 1;
+    Stream st;
+    HIP_TRY(hipStreamCreate(&st.s), "hipStreamCreate");
+    DevBuf dcfg;
+    HIP_TRY(dcfg.alloc(sizeof(FptRenderConfig)), "config allocation");
+    HIP_TRY(hipMemcpyAsync(dcfg.p, config, sizeof(FptRenderConfig), hipMemcpyHostToDevice, st.s), "config upload");
+    fpt::VoxelStore vs;
+    struct Guard {
+        fpt::VoxelStore *v;
+        ~Guard() { fpt::voxel_free(*v); }
+    } guard{&vs};
+    auto t0 = std::chrono::steady_clock::now();
+    if (fpt::voxel_build(*config, dcfg.as<FptRenderConfig>(), vs, st.s, error, error_len)) {
+        g_last_error = error && error_len ? error : "voxel build failed";
+        return 1;
+    }
+    double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    FptVoxelBuildInfo inf = {vs.storage, vs.brick_size, vs.bricks_per_axis, vs.active_bricks, vs.active_cells,
+                             vs.memory_bytes, ms};
+    if (info) *info = inf;
+    const size_t G = vs.bricks_per_axis;
+    const size_t res = config->voxel_resolution;
+    const size_t cpb = size_t(vs.brick_size) * vs.brick_size * vs.brick_size;
+    const size_t pages = vs.active_bricks;
+    auto fetch = [&](void *dst, size_t have, size_t need, const void *src, size_t elem, const char *what) -> int {
+        if (!dst) return 0;
+        if (have < need) {
+            set_error(error, error_len, "%s buffer is too small (%zu < %zu)", what, have, need);
+            return 1;
+        }
+        if (need && src) {
+            hipError_t e = hipMemcpyAsync(dst, src, need * elem, hipMemcpyDeviceToHost, st.s);
+            if (e != hipSuccess) {
+                set_error(error, error_len, "%s readback failed: %s", what, hipGetErrorString(e));
+                return 1;
+            }
+        }
+        return 0;
+    };
+    if (vs.storage == FPT_VOXEL_STORAGE_DENSE) {
+        if (fetch(records, records_len, res * res * res, vs.pool, 12, "records")) return 1;
+    } else {
+        if (fetch(page_table, page_table_len, G * G * G, vs.page_table, 4, "page table")) return 1;
+        if (vs.storage == FPT_VOXEL_STORAGE_SPARSE_BRICKS) {
+            if (fetch(records, records_len, pages * cpb, vs.pool, 12, "records")) return 1;
+        } else {
+            if (fetch(distance, distance_len, pages * cpb, vs.distance, 2, "distance")) return 1;
+            if (fetch(material, material_len, pages * cpb, vs.material, 1, "material")) return 1;
+            if (fetch(palette, palette_len, 256 * 8, vs.palette, 4, "palette")) return 1;
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(st.s), "voxel readback");
+    return 0;
+}
+
+int fpt_hip_voxel_trace_cells(const FptRenderConfig *config, const float *rays, size_t ray_count, uint32_t cap,
+                              int32_t *cells, uint32_t *counts, float *hits, char *error, size_t error_len) {
+    if (!config || !rays || !counts || !hits || (cap && !cells)) {
+        set_error(error, error_len, "missing config, rays or output buffers");
+        return 1;
+    }
+    if (ray_count == 0) return 0;
+    if (check_device(error, error_len)) return 1;
+    Stream st;
+    HIP_TRY(hipStreamCreate(&st.s), "hipStreamCreate");
+    DevBuf dcfg, dr, dc, dn, dh;
+    HIP_TRY(dcfg.alloc(sizeof(FptRenderConfig)), "config allocation");
+    HIP_TRY(hipMemcpyAsync(dcfg.p, config, sizeof(FptRenderConfig), hipMemcpyHostToDevice, st.s), "config upload");
+    fpt::VoxelStore vs;
+    struct Guard {
+        fpt::VoxelStore *v;
+        ~Guard() { fpt::voxel_free(*v); }
+    } guard{&vs};
+    if (fpt::voxel_build(*config, dcfg.as<FptRenderConfig>(), vs, st.s, error, error_len)) {
+        g_last_error = error && error_len ? error : "voxel build failed";
+        return 1;
+    }
+    HIP_TRY(dr.alloc(ray_count * 6 * sizeof(float)), "ray buffer allocation");
+    HIP_TRY(dc.alloc(ray_count * size_t(cap) * 3 * sizeof(int32_t)), "cell buffer allocation");
+    HIP_TRY(dn.alloc(ray_count * sizeof(uint32_t)), "count buffer allocation");
+    HIP_TRY(dh.alloc(ray_count * 8 * sizeof(float)), "hit buffer allocation");
+    HIP_TRY(hipMemcpyAsync(dr.p, rays, ray_count * 6 * sizeof(float), hipMemcpyHostToDevice, st.s), "ray upload");
+    HIP_TRY(fpt::launch_voxel_trace_cells(dcfg.as<FptRenderConfig>(), vs, dr.as<float>(), ray_count, cap,
+                                          dc.as<int>(), dn.as<uint32_t>(), dh.as<float>(), st.s),
+            "trace kernel launch");
+    if (cap)
+        HIP_TRY(hipMemcpyAsync(cells, dc.p, ray_count * size_t(cap) * 3 * sizeof(int32_t), hipMemcpyDeviceToHost, st.s),
+                "cell readback");
+    HIP_TRY(hipMemcpyAsync(counts, dn.p, ray_count * sizeof(uint32_t), hipMemcpyDeviceToHost, st.s), "count readback");
+    HIP_TRY(hipMemcpyAsync(hits, dh.p, ray_count * 8 * sizeof(float), hipMemcpyDeviceToHost, st.s), "hit readback");
+    HIP_TRY(hipStreamSynchronize(st.s), "voxel trace");
     return 0;
 }
 

@@ -690,7 +690,10 @@ __device__ float de_mandelbox_builtin(f3 pos, const FptRenderConfig &cfg, float 
     const float sxyz = SCALE / MINRAD2, sw = fabsf(SCALE) / MINRAD2;
     const float min_rad2 = clampf(MINRAD2, 1.0e-9f, 1.0f);
     const float abs_scale_m1 = fabsf(SCALE - 1.0f);
-    const float abs_scale = powf(fabsf(SCALE), -9.0f);
+    // |SCALE|^-9 as an IEEE chain (s^2)^2)^2 * s, so every platform agrees
+    // bit for bit (voxel occupancy tests sit on this value).
+    const float as1 = fabsf(SCALE), as2 = as1 * as1, as4 = as2 * as2, as8 = as4 * as4;
+    const float abs_scale = 1.0f / (as8 * as1);
     const f3 off = mk(cfg.set_values[4], cfg.set_values[5], cfg.set_values[6]);
     f3 p = pos;
     float w = 1.0f;
@@ -785,19 +788,42 @@ __device__ f3 program_gradient(float value, const FptRenderConfig &cfg) {  // SM
     return ld3(prev + 1);
 }
 
-__device__ Material user_material(f3 p, const FptRenderConfig &cfg, Work &w) {  // SM:3434-3483, 3485-3562
-    Material m;
-    m.rgb = mk(1.f, 1.f, 1.f);
-    m.roughness = 1.f; m.specular = 0.f; m.translucency = 0.f; m.ior = 1.5f; m.emission = 0.f;
+// The reference's material is a function of one scalar orbit value
+// (SM:3434-3483): split into orbit -> shading parameter -> material so the
+// voxel builder can index materials by a quantised parameter.
+__device__ float material_orbit(f3 p, const FptRenderConfig &cfg, Work &w) {
     if (is_program(cfg)) {
         float orbit = 0.f;
-        // The colour depends on the orbit only in modes 1 and 2; mode 0 is
-        // constant, so the shading program is not evaluated for it.
+        // Only modes 1 and 2 colour by orbit; mode 0 skips the shading program.
         if (cfg.material_mode == 1u || cfg.material_mode == 2u) {
             uint32_t n = cfg.sdf_shading_program_count < 64u ? cfg.sdf_shading_program_count : 64u;
             program_eval<true>(cfg.sdf_shading_program, n, p, &orbit, &w.orbit_iterations);
         }
-        float o = orbit * (1.0f + cfg.set_values[1]) + cfg.set_values[0];
+        return orbit;
+    }
+    float orbit = 0.f;
+    if (cfg.sdf_id == FPT_SDF_MANDELBOX_FRACTAL) de_mandelbox_builtin(p, cfg, &orbit, &w.orbit_iterations);
+    if (cfg.sdf_id == FPT_SDF_MENGER_SPONGE) orbit = 1.0f;
+    return orbit;
+}
+
+__device__ __forceinline__ float material_param(float orbit, const FptRenderConfig &cfg) {
+    if (!is_program(cfg) && cfg.fractal_style_mode != 0u) return orbit * cfg.fractal_style[1] + cfg.fractal_style[0];
+    return orbit * (1.0f + cfg.set_values[1]) + cfg.set_values[0];
+}
+
+// True when the material does not depend on the parameter.
+__device__ __forceinline__ bool material_is_constant(const FptRenderConfig &cfg) {
+    if (is_program(cfg)) return cfg.material_mode != 1u && cfg.material_mode != 2u;
+    if (cfg.fractal_style_mode == 2u) return true;
+    return cfg.fractal_style_mode == 0u && cfg.sdf_id == FPT_SDF_MENGER_SPONGE;
+}
+
+__device__ Material material_at_param(float o, const FptRenderConfig &cfg) {
+    Material m;
+    m.rgb = mk(1.f, 1.f, 1.f);
+    m.roughness = 1.f; m.specular = 0.f; m.translucency = 0.f; m.ior = 1.5f; m.emission = 0.f;
+    if (is_program(cfg)) {  // SM:3434-3449
         if (cfg.material_mode == 1u) m.rgb = hsv2rgb(mk(o, 0.5f, 1.0f));
         else if (cfg.material_mode == 2u) m.rgb = program_gradient(o, cfg);
         else m.rgb = ld3(cfg.program_material);
@@ -808,12 +834,7 @@ __device__ Material user_material(f3 p, const FptRenderConfig &cfg, Work &w) {  
         m.emission = cfg.program_material[7];
         return m;
     }
-    float orbit = 0.f;
-    bool menger = cfg.sdf_id == FPT_SDF_MENGER_SPONGE;
-    if (cfg.sdf_id == FPT_SDF_MANDELBOX_FRACTAL) de_mandelbox_builtin(p, cfg, &orbit, &w.orbit_iterations);
-    if (menger) orbit = 1.0f;
     if (cfg.fractal_style_mode != 0u) {  // SM:3451-3468
-        float o = orbit * cfg.fractal_style[1] + cfg.fractal_style[0];
         if (cfg.fractal_style_mode == 1u) m.rgb = hsv2rgb(mk(o, cfg.fractal_style[2], cfg.fractal_style[3]));
         else if (cfg.fractal_style_mode == 2u) m.rgb = ld3(&cfg.fractal_style[8]) * cfg.fractal_style[3];
         else m.rgb = program_gradient(o, cfg) * cfg.fractal_style[3];
@@ -823,10 +844,13 @@ __device__ Material user_material(f3 p, const FptRenderConfig &cfg, Work &w) {  
         m.emission = cfg.fractal_style[6];
         return m;
     }
-    if (menger) return m;
-    float o = orbit * (1.0f + cfg.set_values[1]) + cfg.set_values[0];
+    if (cfg.sdf_id == FPT_SDF_MENGER_SPONGE) return m;  // SM:3469-3472
     m.rgb = hsv2rgb(mk(o, 0.5f, 1.0f));
     return m;
+}
+
+__device__ Material user_material(f3 p, const FptRenderConfig &cfg, Work &w) {  // SM:3485-3562
+    return material_at_param(material_param(material_orbit(p, cfg, w), cfg), cfg);
 }
 
 // march (SM:3753-3783), non-Mandelbulber branch.

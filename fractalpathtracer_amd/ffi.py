@@ -58,6 +58,9 @@ VOXEL_STORAGE_DENSE = 0
 VOXEL_STORAGE_SPARSE_BRICKS = 1
 VOXEL_STORAGE_SPARSE_BRICKS8 = 8
 
+VOXEL_COVERAGE_LEGACY = 0
+VOXEL_COVERAGE_SHELL = 3
+
 u32 = C.c_uint32
 f32 = C.c_float
 
@@ -203,12 +206,17 @@ class FptRayHit(C.Structure):
                 ("steps", u32), ("_pad0", u32)]
 
 
+class FptVoxelBuildInfo(C.Structure):
+    _fields_ = [("storage", u32), ("brick_size", u32), ("bricks_per_axis", u32), ("active_bricks", u32),
+                ("active_cells", C.c_uint64), ("memory_bytes", C.c_uint64), ("build_ms", C.c_double)]
+
+
 # Expected sizes (src/ffi.rs:494-507 plus the new FptRayHit).
 EXPECTED_SIZES = {
     FptSdfInstruction: 32, FptPrimitiveInstance: 80, FptAffineTransform: 64,
     FptIndexedPrimitive: 32, FptTypedSoAProgram: 5136, FptStitchPipelineStats: 72,
     FptRenderConfig: 30448, FptSdfProfileStats: 288, FptDiagnosticConfig: 24,
-    FptMandelbulberFieldSample: 16, FptAsyncJitStats: 40, FptRayHit: 40,
+    FptMandelbulberFieldSample: 16, FptAsyncJitStats: 40, FptRayHit: 40, FptVoxelBuildInfo: 40,
 }
 
 PKG_DIR = Path(__file__).resolve().parent
@@ -220,7 +228,8 @@ EXPORTED_SYMBOLS = (
     "fpt_compare_images", "fpt_metal_device_name", "fpt_test_voxel_dda",
     "fpt_test_async_stitch_context", "fpt_test_typed_soa", "fpt_metal_preview",
     "fpt_hip_trace_primary", "fpt_hip_plan_create", "fpt_hip_plan_render",
-    "fpt_hip_plan_destroy", "fpt_hip_last_error",
+    "fpt_hip_plan_destroy", "fpt_hip_last_error", "fpt_hip_program_surface", "fpt_hip_voxel_build",
+    "fpt_hip_voxel_trace_cells",
 )
 
 _lib = None
@@ -259,6 +268,15 @@ def _declare(lib):
     lib.fpt_hip_plan_render.restype = C.c_int
     lib.fpt_hip_plan_destroy.argtypes = [C.c_void_p]
     lib.fpt_hip_plan_destroy.restype = None
+    lib.fpt_hip_voxel_build.argtypes = [P(FptRenderConfig), P(u32), C.c_size_t, C.c_void_p, C.c_size_t,
+                                         P(C.c_uint16), C.c_size_t, P(C.c_uint8), C.c_size_t, P(f32),
+                                         C.c_size_t, P(FptVoxelBuildInfo), cp, C.c_size_t]
+    lib.fpt_hip_voxel_build.restype = C.c_int
+    lib.fpt_hip_voxel_trace_cells.argtypes = [P(FptRenderConfig), P(f32), C.c_size_t, u32, P(C.c_int32),
+                                              P(u32), P(f32), cp, C.c_size_t]
+    lib.fpt_hip_voxel_trace_cells.restype = C.c_int
+    lib.fpt_hip_program_surface.argtypes = [P(FptRenderConfig), P(f32), C.c_size_t, P(f32), cp, C.c_size_t]
+    lib.fpt_hip_program_surface.restype = C.c_int
     lib.fpt_hip_last_error.argtypes = []
     lib.fpt_hip_last_error.restype = C.c_char_p
 

@@ -237,14 +237,16 @@ def learned_step_scene(width=1920, height=1080, spp=64) -> FptRenderConfig:
 
 def voxel_mandelbox_scene(width=1920, height=1080, spp=4, resolution=1024) -> FptRenderConfig:
     """Voxel workload: built-in Mandelbox with scale 2.0 (set_values[2] =
-    -0.8 under Shaders.metal:731), 12 iterations, on a resolution^3 grid of
-    8^3 bricks holding FP16 distance + u8 material; 3 bounces."""
+    -0.8 under Shaders.metal:731), 12 iterations, voxelised over [-7, 7]^3
+    (the scale-2 box spans about [-6, 6]^3) on a resolution^3 grid of 8^3
+    bricks holding FP16 distance + u8 material id, shell coverage (about 4%
+    of cells); 3 bounces."""
     c = default_config()
     c.width, c.height, c.samples = width, height, spp
     c.sdf_id = ffi.SDF_MANDELBOX_FRACTAL
     c.set_values[2] = -0.8
-    _set(c.camera_position, [0.0, 0.0, -9.0])
-    _set(c.camera_yaw_pitch, [0.35, 0.25])
+    _set(c.camera_position, [0.0, 0.0, -18.0])
+    _set(c.camera_yaw_pitch, [0.4, 0.3])
     c.camera_fov, c.camera_dof = 60.0, 0.0
     _set(c.render, [4.0, 256.0, 0.0005, 1e-4, 100.0, 0.0, 0.0, 0.0])
     _c1_lighting(c)
@@ -252,8 +254,9 @@ def voxel_mandelbox_scene(width=1920, height=1080, spp=4, resolution=1024) -> Fp
     c.renderer_backend = ffi.RENDERER_VOXEL
     c.voxel_resolution = resolution
     c.voxel_storage = ffi.VOXEL_STORAGE_SPARSE_BRICKS8
-    _set(c.voxel_bounds_min, [-4.0] * 3)
-    _set(c.voxel_bounds_max, [4.0] * 3)
+    c.voxel_coverage_mode = ffi.VOXEL_COVERAGE_SHELL
+    _set(c.voxel_bounds_min, [-7.0] * 3)
+    _set(c.voxel_bounds_max, [7.0] * 3)
     c.voxel_surface_band = 1.0
     return c
 

@@ -226,10 +226,19 @@ typedef enum FptVoxelStorage {
     FPT_VOXEL_STORAGE_SPARSE_BRICKS8 = 8
 } FptVoxelStorage;
 
+/*
+ * FPT_VOXEL_COVERAGE_SHELL is new: legacy occupancy (|d| <= r * band), then
+ * every occupied cell whose 26 neighbours are all occupied is dropped.  The
+ * first occupied cell a DDA meets from an unoccupied cell is always kept (its
+ * predecessor is one of its 26 neighbours), so hits of rays that start
+ * outside the solid are unchanged while a solid fractal interior costs no
+ * storage.
+ */
 typedef enum FptVoxelCoverageMode {
     FPT_VOXEL_COVERAGE_LEGACY = 0,
     FPT_VOXEL_COVERAGE_LIPSCHITZ = 1,
-    FPT_VOXEL_COVERAGE_INTERVAL = 2
+    FPT_VOXEL_COVERAGE_INTERVAL = 2,
+    FPT_VOXEL_COVERAGE_SHELL = 3
 } FptVoxelCoverageMode;
 
 typedef enum FptVoxelBuildMode {
@@ -681,6 +690,59 @@ int fpt_hip_plan_render(FptHipPlan *plan,
                         char *error,
                         size_t error_len);
 void fpt_hip_plan_destroy(FptHipPlan *plan);
+
+/*
+ * Voxel tables of `config` (renderer_backend ignored; voxel_* fields used),
+ * built on the GPU exactly as fpt_metal_render builds them
+ * (Shaders.metal:5388-5424, metal_bridge.mm:1718-1816; pages are numbered in
+ * linear brick order bx + G*(by + G*bz), the reference's staging order).
+ * info is always filled.  Each output buffer is optional (NULL skips it); a
+ * buffer that is too small fails the call and the message names the size.
+ *   page_table  u32[G^3]                      (sparse layouts)
+ *   records     12-byte records (4^3 bricks: pages*64; dense: res^3)
+ *   distance    FP16 bits u16[pages*512]       (8^3 layout)
+ *   material    u8[pages*512], 0 = empty       (8^3 layout)
+ *   palette     float[256*8] {r,g,b,roughness,specular,translucency,ior,
+ *               emission} per material id     (8^3 layout)
+ */
+struct FptVoxelBuildInfo {
+    uint32_t storage;
+    uint32_t brick_size;        /* 4 or 8; 0 for dense */
+    uint32_t bricks_per_axis;   /* G */
+    uint32_t active_bricks;     /* pages */
+    uint64_t active_cells;
+    uint64_t memory_bytes;
+    double build_ms;
+};
+FPT_ABI_ASSERT(sizeof(struct FptVoxelBuildInfo) == 40, "FptVoxelBuildInfo");
+
+int fpt_hip_voxel_build(const struct FptRenderConfig *config,
+                        uint32_t *page_table, size_t page_table_len,
+                        void *records, size_t records_len,
+                        uint16_t *distance, size_t distance_len,
+                        uint8_t *material, size_t material_len,
+                        float *palette, size_t palette_len,
+                        struct FptVoxelBuildInfo *info,
+                        char *error,
+                        size_t error_len);
+
+/*
+ * Exact grid DDA over the voxel tables of `config` for n rays
+ * (rays[6*i..6*i+5] = origin xyz, direction xyz): the cells visited in order
+ * (traceVoxel, Shaders.metal:4492-4612), up to `cap` per ray, into
+ * cells[(i*cap + k)*3 + {0,1,2}]; counts[i] = cells visited (may exceed
+ * cap); hits[8*i..] = {hit, entry t, exit t, nx, ny, nz, payload bits,
+ * steps} with payload the packed colour (4^3/dense) or material id (8^3).
+ */
+int fpt_hip_voxel_trace_cells(const struct FptRenderConfig *config,
+                              const float *rays,
+                              size_t ray_count,
+                              uint32_t cap,
+                              int32_t *cells,
+                              uint32_t *counts,
+                              float *hits,
+                              char *error,
+                              size_t error_len);
 
 /* Last error string of the calling thread (empty when none). */
 const char *fpt_hip_last_error(void);

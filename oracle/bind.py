@@ -54,10 +54,73 @@ def lib():
 
 def _voxel_decls(L):
     P = C.POINTER
-    if hasattr(L, "orc_voxel_build"):
-        L.orc_voxel_build.argtypes = [C.c_void_p, C.c_int, P(C.c_uint32), C.c_size_t,
-                                      P(C.c_uint16), P(C.c_uint8), C.c_size_t, P(C.c_uint64)]
-        L.orc_voxel_build.restype = C.c_int
+    L.orc_voxel_build.argtypes = [C.c_void_p, C.c_int, P(C.c_uint32), C.c_size_t, C.c_void_p, C.c_size_t,
+                                  P(C.c_uint16), C.c_size_t, P(C.c_uint8), C.c_size_t, P(C.c_float),
+                                  C.c_size_t, P(C.c_uint64)]
+    L.orc_voxel_trace_cells.argtypes = [C.c_void_p, P(C.c_float), C.c_size_t, C.c_uint32, P(C.c_int32),
+                                        P(C.c_uint32), P(C.c_float)]
+    L.orc_voxel_render.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, P(C.c_double)]
+    L.orc_voxel_dda_contract.restype = C.c_uint32
+    L.orc_f32_to_f16.argtypes = [C.c_float]
+    L.orc_f32_to_f16.restype = C.c_uint16
+
+
+RECORD_DTYPE = np.dtype([("packed_color", "<u4"), ("packed_properties", "<u4"), ("emission", "<f4")])
+
+
+def voxel_build(cfg, threads: int = 0) -> dict:
+    """FP32 voxel tables: page_table, records or (distance, material, palette)."""
+    L = lib()
+    info = (C.c_uint64 * 6)()
+    if L.orc_voxel_build(C.addressof(cfg), threads, None, 0, None, 0, None, 0, None, 0, None, 0, info) != 0:
+        raise RuntimeError("orc_voxel_build failed")
+    storage, B, G, pages, cells = (int(info[i]) for i in range(5))
+    res = cfg.voxel_resolution
+    out = dict(storage=storage, brick_size=B, bricks_per_axis=G, active_bricks=pages, active_cells=cells)
+    cpb = B ** 3
+    pt = np.zeros(G ** 3, np.uint32)
+    nrec = res ** 3 if storage == 0 else (pages * cpb if storage == 1 else 0)
+    rec = np.zeros(max(nrec, 1), RECORD_DTYPE)
+    dist = np.zeros(max(pages * cpb, 1), np.uint16)
+    mat = np.zeros(max(pages * cpb, 1), np.uint8)
+    pal = np.zeros(256 * 8, np.float32)
+    L.orc_voxel_build(C.addressof(cfg), threads, pt.ctypes.data_as(C.POINTER(C.c_uint32)), len(pt),
+                      rec.ctypes.data_as(C.c_void_p), len(rec), dist.ctypes.data_as(C.POINTER(C.c_uint16)),
+                      len(dist), mat.ctypes.data_as(C.POINTER(C.c_uint8)), len(mat),
+                      pal.ctypes.data_as(C.POINTER(C.c_float)), len(pal), info)
+    out.update(page_table=pt, records=rec[:nrec], distance=dist[:pages * cpb], material=mat[:pages * cpb],
+               palette=pal.reshape(256, 8))
+    return out
+
+
+def voxel_trace_cells(cfg, rays, cap: int = 4096):
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+    n = len(rays)
+    cells = np.zeros((n, cap, 3), np.int32)
+    counts = np.zeros(n, np.uint32)
+    hits = np.zeros((n, 8), np.float32)
+    if lib().orc_voxel_trace_cells(C.addressof(cfg), rays.ctypes.data_as(C.POINTER(C.c_float)), n, cap,
+                                   cells.ctypes.data_as(C.POINTER(C.c_int32)),
+                                   counts.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                   hits.ctypes.data_as(C.POINTER(C.c_float))) != 0:
+        raise RuntimeError("orc_voxel_trace_cells failed")
+    return cells, counts, hits
+
+
+def voxel_render(cfg, threads: int = 0, rows=None) -> np.ndarray:
+    y0, y1 = (0, cfg.height) if rows is None else rows
+    out = np.zeros((cfg.height, cfg.width, 4), np.float64)
+    if lib().orc_voxel_render(C.addressof(cfg), threads, y0, y1, _dp(out)) != 0:
+        raise RuntimeError("orc_voxel_render failed")
+    return out
+
+
+def voxel_dda_contract() -> int:
+    return int(lib().orc_voxel_dda_contract())
+
+
+def f32_to_f16_bits(x: float) -> int:
+    return int(lib().orc_f32_to_f16(x))
 
 
 def _dp(a: np.ndarray):

@@ -710,7 +710,9 @@ T de_mandelbox_builtin(V3<T> pos, const FptRenderConfig &cfg, T *orbit) {  // SM
     T sc[4] = {SCALE / MINRAD2, SCALE / MINRAD2, SCALE / MINRAD2, std::fabs(SCALE) / MINRAD2};
     T min_rad2 = m_clamp(MINRAD2, T(1.0e-9), T(1));
     T abs_scale_m1 = std::fabs(SCALE - T(1));
-    T abs_scale = std::pow(std::fabs(SCALE), T(-9));
+    // |SCALE|^-9 as the IEEE chain ((s^2)^2)^2 * s (the GPU uses the same).
+    T as1 = std::fabs(SCALE), as2 = as1 * as1, as4 = as2 * as2, as8 = as4 * as4;
+    T abs_scale = T(1) / (as8 * as1);
     T off[4] = {T(cfg.set_values[4]), T(cfg.set_values[5]), T(cfg.set_values[6]), T(0)};
     T p[4] = {pos.x, pos.y, pos.z, T(1)};
     T p0[4] = {p[0], p[1], p[2], p[3]};
@@ -790,44 +792,6 @@ V3<T> program_gradient(T value, const FptRenderConfig &cfg) {  // SM:3418-3432
 }
 
 template <class T>
-Material<T> program_material(T orbit, const FptRenderConfig &cfg) {  // SM:3434-3449
-    Material<T> m;
-    T o = orbit * (T(1) + T(cfg.set_values[1])) + T(cfg.set_values[0]);
-    if (cfg.material_mode == 1u) m.rgb = hsv2rgb(V3<T>(o, T(0.5), T(1)));
-    else if (cfg.material_mode == 2u) m.rgb = program_gradient(o, cfg);
-    else m.rgb = load3<T>(cfg.program_material);
-    m.roughness = T(cfg.program_material[3]);
-    m.specular = T(cfg.program_material[4]);
-    m.translucency = T(cfg.program_material[5]);
-    m.ior = T(cfg.program_material[6]);
-    m.emission = T(cfg.program_material[7]);
-    return m;
-}
-
-template <class T>
-Material<T> fractal_material(T orbit, const FptRenderConfig &cfg, bool menger) {  // SM:3451-3483
-    Material<T> m;
-    if (cfg.fractal_style_mode != 0u) {
-        T o = orbit * T(cfg.fractal_style[1]) + T(cfg.fractal_style[0]);
-        if (cfg.fractal_style_mode == 1u)
-            m.rgb = hsv2rgb(V3<T>(o, T(cfg.fractal_style[2]), T(cfg.fractal_style[3])));
-        else if (cfg.fractal_style_mode == 2u)
-            m.rgb = load3<T>(&cfg.fractal_style[8]) * T(cfg.fractal_style[3]);
-        else
-            m.rgb = program_gradient(o, cfg) * T(cfg.fractal_style[3]);
-        m.rgb = vmix(m.rgb, load3<T>(&cfg.fractal_style[8]), m_clamp(T(cfg.fractal_style[7]), T(0), T(1)));
-        m.roughness = T(cfg.fractal_style[4]);
-        m.specular = T(cfg.fractal_style[5]);
-        m.emission = T(cfg.fractal_style[6]);
-        return m;
-    }
-    if (menger) return m;
-    T o = orbit * (T(1) + T(cfg.set_values[1])) + T(cfg.set_values[0]);
-    m.rgb = hsv2rgb(V3<T>(o, T(0.5), T(1)));
-    return m;
-}
-
-template <class T>
 bool is_program(const FptRenderConfig &cfg) {
     return cfg.sdf_id == FPT_SDF_PROGRAM && cfg.sdf_program_count > 0;
 }
@@ -849,18 +813,69 @@ T map_sdf(V3<T> p, const FptRenderConfig &cfg) {  // SM:3612-3615
     return m_mix(d, d / T(2), T(cfg.render[6]));
 }
 
+// Material as orbit -> shading parameter -> material (SM:3434-3483), the
+// split the voxel palette needs; user_material composes the three.
+template <class T>
+T material_orbit(V3<T> p, const FptRenderConfig &cfg) {  // SM:3485-3562 (orbit part)
+    T orbit = T(0);
+    if (is_program<T>(cfg)) {
+        if (cfg.material_mode == 1u || cfg.material_mode == 2u) program_eval<T>(shading_program(cfg), p, &orbit);
+        return orbit;
+    }
+    if (cfg.sdf_id == FPT_SDF_MANDELBOX_FRACTAL) de_mandelbox_builtin<T>(p, cfg, &orbit);
+    else if (cfg.sdf_id == FPT_SDF_MENGER_SPONGE) orbit = T(1);
+    return orbit;
+}
+
+template <class T>
+T material_param(T orbit, const FptRenderConfig &cfg) {
+    if (!is_program<T>(cfg) && cfg.fractal_style_mode != 0u)
+        return orbit * T(cfg.fractal_style[1]) + T(cfg.fractal_style[0]);
+    return orbit * (T(1) + T(cfg.set_values[1])) + T(cfg.set_values[0]);
+}
+
+template <class T>
+bool material_is_constant(const FptRenderConfig &cfg) {
+    if (is_program<T>(cfg)) return cfg.material_mode != 1u && cfg.material_mode != 2u;
+    if (cfg.fractal_style_mode == 2u) return true;
+    return cfg.fractal_style_mode == 0u && cfg.sdf_id == FPT_SDF_MENGER_SPONGE;
+}
+
+template <class T>
+Material<T> material_at_param(T o, const FptRenderConfig &cfg) {
+    Material<T> m;
+    if (is_program<T>(cfg)) {  // SM:3434-3449
+        if (cfg.material_mode == 1u) m.rgb = hsv2rgb(V3<T>(o, T(0.5), T(1)));
+        else if (cfg.material_mode == 2u) m.rgb = program_gradient(o, cfg);
+        else m.rgb = load3<T>(cfg.program_material);
+        m.roughness = T(cfg.program_material[3]);
+        m.specular = T(cfg.program_material[4]);
+        m.translucency = T(cfg.program_material[5]);
+        m.ior = T(cfg.program_material[6]);
+        m.emission = T(cfg.program_material[7]);
+        return m;
+    }
+    if (cfg.fractal_style_mode != 0u) {  // SM:3451-3468
+        if (cfg.fractal_style_mode == 1u)
+            m.rgb = hsv2rgb(V3<T>(o, T(cfg.fractal_style[2]), T(cfg.fractal_style[3])));
+        else if (cfg.fractal_style_mode == 2u)
+            m.rgb = load3<T>(&cfg.fractal_style[8]) * T(cfg.fractal_style[3]);
+        else
+            m.rgb = program_gradient(o, cfg) * T(cfg.fractal_style[3]);
+        m.rgb = vmix(m.rgb, load3<T>(&cfg.fractal_style[8]), m_clamp(T(cfg.fractal_style[7]), T(0), T(1)));
+        m.roughness = T(cfg.fractal_style[4]);
+        m.specular = T(cfg.fractal_style[5]);
+        m.emission = T(cfg.fractal_style[6]);
+        return m;
+    }
+    if (cfg.sdf_id == FPT_SDF_MENGER_SPONGE) return m;
+    m.rgb = hsv2rgb(V3<T>(o, T(0.5), T(1)));
+    return m;
+}
+
 template <class T>
 Material<T> user_material(V3<T> p, const FptRenderConfig &cfg) {  // SM:3485-3562 (material part)
-    if (is_program<T>(cfg)) {
-        T orbit = T(0);
-        program_eval<T>(shading_program(cfg), p, &orbit);
-        return program_material(orbit, cfg);
-    }
-    T orbit = T(0);
-    bool menger = false;
-    if (cfg.sdf_id == FPT_SDF_MANDELBOX_FRACTAL) de_mandelbox_builtin<T>(p, cfg, &orbit);
-    else if (cfg.sdf_id == FPT_SDF_MENGER_SPONGE) { menger = true; orbit = T(1); }
-    return fractal_material(orbit, cfg, menger);
+    return material_at_param(material_param(material_orbit(p, cfg), cfg), cfg);
 }
 
 template <class T>
@@ -1166,6 +1181,8 @@ void trace_primary(const FptRenderConfig &cfg, int threads, double *out) {
         o[8] = double(steps);
     }
 }
+
+#include "fpt_oracle_voxel.inc"
 
 }  // namespace
 
