@@ -528,6 +528,7 @@ int fpt_hip_voxel_trace_cells(const FptRenderConfig *config, const float *rays, 
     HIP_TRY(dn.alloc(ray_count * sizeof(uint32_t)), "count buffer allocation");
     HIP_TRY(dh.alloc(ray_count * 8 * sizeof(float)), "hit buffer allocation");
     HIP_TRY(hipMemcpyAsync(dr.p, rays, ray_count * 6 * sizeof(float), hipMemcpyHostToDevice, st.s), "ray upload");
+    HIP_TRY(hipMemsetAsync(dc.p, 0, ray_count * size_t(cap) * 3 * sizeof(int32_t), st.s), "cell buffer clear");
     HIP_TRY(fpt::launch_voxel_trace_cells(dcfg.as<FptRenderConfig>(), vs, dr.as<float>(), ray_count, cap,
                                           dc.as<int>(), dn.as<uint32_t>(), dh.as<float>(), st.s),
             "trace kernel launch");

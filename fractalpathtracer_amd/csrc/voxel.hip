@@ -174,6 +174,7 @@ __device__ VHit trace_voxel(f3 origin, f3 dir, const FptRenderConfig &cfg, const
     r.steps = 0;
     r.payload = 0;
     r.index = 0;
+    r.position = r.cell_center = r.normal = mk(0.f, 0.f, 0.f);
     const int res = int(cfg.voxel_resolution > 1u ? cfg.voxel_resolution : 1u);
     const f3 bmin = ld3(cfg.voxel_bounds_min), bmax = ld3(cfg.voxel_bounds_max);
     const f3 cs = cell_size_of(cfg);

@@ -75,7 +75,9 @@ def test_cell_visit_sequences_are_bit_exact(storage, coverage):
     gc, gn, gh = api.voxel_trace_cells(cfg, rays, cap=400)
     cc, cn, ch = bind.voxel_trace_cells(cfg, rays, cap=400)
     assert np.array_equal(gn, cn)
-    assert np.array_equal(gc, cc)
+    for i in range(len(rays)):
+        k = min(int(cn[i]), 400)
+        assert np.array_equal(gc[i, :k], cc[i, :k]), i
     assert gh.tobytes() == ch.tobytes()
     assert gh[:, 0].sum() > 100
 
