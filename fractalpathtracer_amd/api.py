@@ -165,6 +165,10 @@ class Plan:
         err = ffi.error_buffer()
         ffi.check(self.lib.fpt_hip_plan_create(C.byref(cfg), C.byref(self.handle), err, len(err)), err)
 
+    def tune(self, kernel: int = 1, workgroups: int = 0, event_batch: int = 0) -> None:
+        err = ffi.error_buffer()
+        ffi.check(self.lib.fpt_hip_plan_tune(self.handle, kernel, workgroups, event_batch, err, len(err)), err)
+
     def render_into(self, device_ptr: int, stream_ptr: int = 0) -> None:
         err = ffi.error_buffer()
         ffi.check(self.lib.fpt_hip_plan_render(self.handle, C.c_void_p(device_ptr), C.c_void_p(stream_ptr),

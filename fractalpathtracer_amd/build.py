@@ -3,7 +3,7 @@
 Every HIP source under csrc/ is compiled separately (in parallel) and linked
 into fractalpathtracer_amd/libfpt_hip.so.  Packed FP32 instructions are
 disabled (`-target-feature -packed-fp32-ops`): the compiler would otherwise
-emit v_pk_* FP32 forms on its own.
+emit v_pk_* FP32 forms on its own.  Contraction is `-ffp-contract=on`.
 """
 from __future__ import annotations
 
@@ -22,6 +22,10 @@ ARCH = os.environ.get("FPT_OFFLOAD_ARCH", "gfx950")
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    # FMA fusion only within one source expression: results then do not
+    # depend on where the optimiser inlines a function (files that must be
+    # IEEE-exact switch contraction off with a pragma).
+    "-ffp-contract=on",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]

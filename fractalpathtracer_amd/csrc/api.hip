@@ -188,7 +188,33 @@ struct FptHipPlan {
     FptRenderConfig *dcfg = nullptr;
     fpt::VoxelStore voxels;  // populated for the voxel backend
     double build_ms = 0.0;
+    // SDF launch shape (fpt_hip_plan_tune).
+    uint32_t kernel = 1;         // 0 one path per lane, 1 persistent
+    uint32_t workgroups = 0;     // persistent grid; 0 = 2 per compute unit
+    uint32_t event_batch = 24;   // waiting lanes before per-hit work runs
+    uint32_t *counter = nullptr; // persistent pixel counter
 };
+
+namespace {
+uint32_t default_workgroups() {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    return uint32_t(cus > 0 ? cus : 256) * 2u;
+}
+
+hipError_t launch_sdf(FptHipPlan *plan, float4 *acc, uint32_t start, uint32_t chunk, fpt::DeviceProfile *prof,
+                      hipStream_t s) {
+    const FptRenderConfig &c = plan->host;
+    if (plan->kernel == 0u) return fpt::launch_sdf_accumulate(plan->dcfg, c, acc, start, chunk, prof, s);
+    if (!plan->counter) {
+        hipError_t e = hipMalloc(&plan->counter, sizeof(uint32_t));
+        if (e != hipSuccess) return e;
+    }
+    uint32_t wg = plan->workgroups ? plan->workgroups : default_workgroups();
+    return fpt::launch_sdf_persistent(plan->dcfg, c, acc, start, chunk, plan->counter, wg, int(plan->event_batch),
+                                      prof, s);
+}
+}  // namespace
 
 extern "C" {
 
@@ -237,15 +263,33 @@ int fpt_hip_plan_render(FptHipPlan *plan, float *device_accum, void *stream, cha
     for (uint32_t start = 0; start < spp; start += chunk) {
         hipError_t e = c.renderer_backend == FPT_RENDERER_VOXEL
                            ? fpt::launch_voxel_accumulate(plan->dcfg, c, plan->voxels, acc, start, chunk, s, nullptr)
-                           : fpt::launch_sdf_accumulate(plan->dcfg, c, acc, start, chunk, nullptr, s);
+                           : launch_sdf(plan, acc, start, chunk, nullptr, s);
         HIP_TRY(e, "render kernel launch");
     }
+    return 0;
+}
+
+int fpt_hip_plan_tune(FptHipPlan *plan, uint32_t kernel, uint32_t workgroups, uint32_t event_batch, char *error,
+                      size_t error_len) {
+    if (!plan) {
+        set_error(error, error_len, "missing plan");
+        return 1;
+    }
+    if (kernel > 1u || event_batch > 64u || workgroups > 65536u) {
+        set_error(error, error_len, "invalid launch shape (kernel %u, workgroups %u, event_batch %u)", kernel,
+                  workgroups, event_batch);
+        return 1;
+    }
+    plan->kernel = kernel;
+    plan->workgroups = workgroups;
+    plan->event_batch = event_batch ? event_batch : 24u;
     return 0;
 }
 
 void fpt_hip_plan_destroy(FptHipPlan *plan) {
     if (!plan) return;
     if (plan->dcfg) (void)hipFree(plan->dcfg);
+    if (plan->counter) (void)hipFree(plan->counter);
     fpt::voxel_free(plan->voxels);
     delete plan;
 }
@@ -328,8 +372,7 @@ int fpt_metal_render(const char *metallib_path, const char *stitch_metallib_path
         HIP_TRY(hipMemsetAsync(prof.p, 0, sizeof(fpt::DeviceProfile), st.s), "profile clear");
         const uint32_t spp = clamp_samples(config->samples), chunk = chunk_for(*config);
         for (uint32_t s0 = 0; s0 < spp; s0 += chunk)
-            HIP_TRY(fpt::launch_sdf_accumulate(plan->dcfg, *config, acc2.as<float4>(), s0, chunk,
-                                               prof.as<fpt::DeviceProfile>(), st.s),
+            HIP_TRY(launch_sdf(plan, acc2.as<float4>(), s0, chunk, prof.as<fpt::DeviceProfile>(), st.s),
                     "profile kernel launch");
         fpt::DeviceProfile hp;
         HIP_TRY(hipMemcpyAsync(&hp, prof.p, sizeof(hp), hipMemcpyDeviceToHost, st.s), "profile readback");

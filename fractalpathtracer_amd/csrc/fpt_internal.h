@@ -30,6 +30,9 @@ hipError_t launch_field_sample(const FptRenderConfig *dcfg, const float4 *pts, s
                                FptMandelbulberFieldSample *out, hipStream_t stream);
 hipError_t launch_surface_sample(const FptRenderConfig *dcfg, const float4 *pts, size_t n, float4 *out,
                                  hipStream_t stream);
+hipError_t launch_sdf_persistent(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, float4 *accum,
+                                 uint32_t start, uint32_t count, uint32_t *counter, uint32_t workgroups,
+                                 int event_batch, DeviceProfile *prof, hipStream_t stream);
 hipError_t launch_present(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, const float4 *accum,
                           uchar4 *out, hipStream_t stream);
 

@@ -229,7 +229,7 @@ EXPORTED_SYMBOLS = (
     "fpt_test_async_stitch_context", "fpt_test_typed_soa", "fpt_metal_preview",
     "fpt_hip_trace_primary", "fpt_hip_plan_create", "fpt_hip_plan_render",
     "fpt_hip_plan_destroy", "fpt_hip_last_error", "fpt_hip_program_surface", "fpt_hip_voxel_build",
-    "fpt_hip_voxel_trace_cells",
+    "fpt_hip_voxel_trace_cells", "fpt_hip_plan_tune",
 )
 
 _lib = None
@@ -277,6 +277,8 @@ def _declare(lib):
     lib.fpt_hip_voxel_trace_cells.restype = C.c_int
     lib.fpt_hip_program_surface.argtypes = [P(FptRenderConfig), P(f32), C.c_size_t, P(f32), cp, C.c_size_t]
     lib.fpt_hip_program_surface.restype = C.c_int
+    lib.fpt_hip_plan_tune.argtypes = [C.c_void_p, u32, u32, u32, cp, C.c_size_t]
+    lib.fpt_hip_plan_tune.restype = C.c_int
     lib.fpt_hip_last_error.argtypes = []
     lib.fpt_hip_last_error.restype = C.c_char_p
 

@@ -692,6 +692,24 @@ int fpt_hip_plan_render(FptHipPlan *plan,
 void fpt_hip_plan_destroy(FptHipPlan *plan);
 
 /*
+ * Launch shape of a plan's SDF render (the autotuner's knobs; every shape
+ * renders the same image):
+ *   kernel       0 = one path per lane (16x16 pixel tiles per workgroup);
+ *                1 = persistent, path-regenerating kernel (default);
+ *   workgroups   persistent grid size in 256-thread workgroups, 0 = two per
+ *                compute unit;
+ *   event_batch  how many lanes of a 64-lane wave must have finished their
+ *                march before the per-hit work (material, analytic normal,
+ *                sun and scatter) runs for all of them, 1..64 (0 = 24).
+ */
+int fpt_hip_plan_tune(FptHipPlan *plan,
+                      uint32_t kernel,
+                      uint32_t workgroups,
+                      uint32_t event_batch,
+                      char *error,
+                      size_t error_len);
+
+/*
  * Voxel tables of `config` (renderer_backend ignored; voxel_* fields used),
  * built on the GPU exactly as fpt_metal_render builds them
  * (Shaders.metal:5388-5424, metal_bridge.mm:1718-1816; pages are numbered in
