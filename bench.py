@@ -76,30 +76,23 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU split of this path "
               "is a design note, not a build target", file=sys.stderr)
         sys.exit(2)
-    import torch
-
     from fractalpathtracer_amd import api, scenes
 
-    assert torch.cuda.is_available(), "bench.py needs a GPU"
-    dev = torch.device("cuda:0")
+    name = api.device_name()
+    assert "gfx" in name, f"bench.py needs a HIP GPU ({name})"
     cfg = scenes.learned_step_scene(args.width, args.height, args.spp)
 
     # Untimed counting pass of the same frame: work counters for the roofline.
     prof = api.render_full(cfg, profile=True).profile
 
-    stream = torch.cuda.current_stream(dev)
-    accum = torch.empty((cfg.height, cfg.width, 4), dtype=torch.float32, device=dev)
+    # Timed region: HIP events on the launch stream around `steps` frames,
+    # inputs (config) resident in HBM, output in a device buffer.
+    stream = api.Stream()
+    accum = api.DeviceBuffer(cfg.width * cfg.height * 4)
     plan = api.Plan(cfg)
-    for _ in range(args.warmup):
-        plan.render_into(accum.data_ptr(), stream.cuda_stream)
-    torch.cuda.synchronize()
-    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    start.record(stream)
-    for _ in range(args.steps):
-        plan.render_into(accum.data_ptr(), stream.cuda_stream)
-    end.record(stream)
-    torch.cuda.synchronize()
-    total_ms = start.elapsed_time(end)
+    if args.warmup:
+        api.time_renders(plan, accum, stream, args.warmup)
+    total_ms = api.time_renders(plan, accum, stream, args.steps)
     plan.close()
     ms_per_step = total_ms / args.steps
     samples_per_step = cfg.width * cfg.height * cfg.samples

@@ -155,6 +155,85 @@ def device_name() -> str:
     return buf.value.decode()
 
 
+class DeviceBuffer:
+    """A float32 device buffer owned by the library's HIP runtime."""
+
+    def __init__(self, count: int):
+        self.lib = ffi.load_library()
+        self.count = int(count)
+        self.ptr = C.c_void_p()
+        err = ffi.error_buffer()
+        ffi.check(self.lib.fpt_hip_device_alloc(self.count * 4, C.byref(self.ptr), err, len(err)), err)
+
+    def fill(self, value: float, stream: "Stream | None" = None) -> None:
+        err = ffi.error_buffer()
+        ffi.check(self.lib.fpt_hip_fill_f32(self.ptr, value, self.count, stream.handle if stream else None, err,
+                                            len(err)), err)
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.count, np.float32)
+        err = ffi.error_buffer()
+        ffi.check(self.lib.fpt_hip_copy(out.ctypes.data_as(C.c_void_p), self.ptr, self.count * 4, 0, err,
+                                        len(err)), err)
+        return out
+
+    @property
+    def address(self) -> int:
+        return self.ptr.value or 0
+
+    def close(self) -> None:
+        if self.ptr:
+            self.lib.fpt_hip_device_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Stream:
+    """A non-blocking HIP stream of the library's runtime."""
+
+    def __init__(self):
+        self.lib = ffi.load_library()
+        self.handle = C.c_void_p()
+        err = ffi.error_buffer()
+        ffi.check(self.lib.fpt_hip_stream_create(C.byref(self.handle), err, len(err)), err)
+
+    @property
+    def address(self) -> int:
+        return self.handle.value or 0
+
+    def synchronize(self) -> None:
+        err = ffi.error_buffer()
+        ffi.check(self.lib.fpt_hip_synchronize(self.handle, err, len(err)), err)
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.fpt_hip_stream_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def time_renders(target, buffer: DeviceBuffer, stream: Stream, steps: int) -> float:
+    """HIP-event time (ms) of `steps` renders of a Plan or Batch on `stream`."""
+    lib = ffi.load_library()
+    ms = C.c_double(0)
+    err = ffi.error_buffer()
+    plan = target.handle if isinstance(target, Plan) else None
+    batch = target.handle if isinstance(target, Batch) else None
+    ffi.check(lib.fpt_hip_time_renders(plan, batch, buffer.ptr, stream.handle, steps, C.byref(ms), err, len(err)),
+              err)
+    return ms.value
+
+
 class Plan:
     """Device-resident render plan (fpt_hip_plan_*): upload once, render many."""
 
@@ -168,6 +247,13 @@ class Plan:
     def tune(self, kernel: int = 1, workgroups: int = 0, event_batch: int = 0) -> None:
         err = ffi.error_buffer()
         ffi.check(self.lib.fpt_hip_plan_tune(self.handle, kernel, workgroups, event_batch, err, len(err)), err)
+
+    def autotune(self, device_ptr: int, stream_ptr: int = 0, rounds: int = 3) -> ffi.FptTuneResult:
+        res = ffi.FptTuneResult()
+        err = ffi.error_buffer()
+        ffi.check(self.lib.fpt_hip_autotune(self.handle, None, C.c_void_p(device_ptr), C.c_void_p(stream_ptr),
+                                            rounds, C.byref(res), err, len(err)), err)
+        return res
 
     def render_into(self, device_ptr: int, stream_ptr: int = 0) -> None:
         err = ffi.error_buffer()
@@ -190,3 +276,77 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """Batched multi-scene render (fpt_hip_batch_*)."""
+
+    def __init__(self, configs):
+        self.lib = ffi.load_library()
+        self.configs = list(configs)
+        arr = (FptRenderConfig * len(self.configs))(*self.configs)
+        self._arr = arr
+        self.handle = C.c_void_p()
+        err = ffi.error_buffer()
+        ffi.check(self.lib.fpt_hip_batch_create(arr, len(self.configs), C.byref(self.handle), err, len(err)), err)
+        self.pixels = int(self.lib.fpt_hip_batch_pixels(self.handle))
+        self.offsets = np.cumsum([0] + [c.width * c.height for c in self.configs])
+
+    def tune(self, workgroups: int = 0, event_batch: int = 0, use_graph: bool = True) -> None:
+        err = ffi.error_buffer()
+        ffi.check(self.lib.fpt_hip_batch_tune(self.handle, workgroups, event_batch, int(use_graph), err, len(err)),
+                  err)
+
+    def autotune(self, device_ptr: int, stream_ptr: int = 0, rounds: int = 3) -> ffi.FptTuneResult:
+        res = ffi.FptTuneResult()
+        err = ffi.error_buffer()
+        ffi.check(self.lib.fpt_hip_autotune(None, self.handle, C.c_void_p(device_ptr), C.c_void_p(stream_ptr),
+                                            rounds, C.byref(res), err, len(err)), err)
+        return res
+
+    def render_into(self, device_ptr: int, stream_ptr: int = 0) -> None:
+        err = ffi.error_buffer()
+        ffi.check(self.lib.fpt_hip_batch_render(self.handle, C.c_void_p(device_ptr), C.c_void_p(stream_ptr), err,
+                                                len(err)), err)
+
+    def split(self, flat: np.ndarray):
+        """Per-scene (H, W, 4) views of a flat (pixels, 4) result."""
+        flat = flat.reshape(-1, 4)
+        return [flat[self.offsets[i]:self.offsets[i + 1]].reshape(c.height, c.width, 4)
+                for i, c in enumerate(self.configs)]
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.fpt_hip_batch_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def render_batch(configs):
+    """fpt_hip_render_batch: list of (H, W, 4) images and the elapsed ms."""
+    lib = ffi.load_library()
+    configs = list(configs)
+    arr = (FptRenderConfig * len(configs))(*configs)
+    n = sum(c.width * c.height for c in configs)
+    out = np.zeros(n * 4, np.float32)
+    ms = C.c_double(0)
+    err = ffi.error_buffer()
+    ffi.check(lib.fpt_hip_render_batch(arr, len(configs), out.ctypes.data_as(C.POINTER(C.c_float)), out.size,
+                                       C.byref(ms), err, len(err)), err)
+    flat = out.reshape(-1, 4)
+    res, o = [], 0
+    for c in configs:
+        res.append(flat[o:o + c.width * c.height].reshape(c.height, c.width, 4))
+        o += c.width * c.height
+    return res, ms.value

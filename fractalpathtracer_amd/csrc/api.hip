@@ -294,6 +294,283 @@ void fpt_hip_plan_destroy(FptHipPlan *plan) {
     delete plan;
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched multi-scene rendering.
+// ---------------------------------------------------------------------------
+struct FptHipBatch {
+    std::vector<FptRenderConfig> host;
+    std::vector<fpt::BatchScene> scenes;
+    FptRenderConfig *dcfgs = nullptr;
+    fpt::BatchScene *dscenes = nullptr;
+    uint32_t *counter = nullptr;
+    uint32_t total_slots = 0;
+    size_t pixels = 0;
+    uint32_t workgroups = 0;
+    uint32_t event_batch = 24;
+    uint32_t use_graph = 1;
+    // Captured launch sequence (counter reset + persistent kernel) for one
+    // output buffer.
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    float *graph_accum = nullptr;
+    uint32_t graph_wg = 0, graph_eb = 0;
+};
+
+namespace {
+void batch_drop_graph(FptHipBatch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->graph_accum = nullptr;
+}
+}  // namespace
+
+extern "C" {
+
+int fpt_hip_batch_create(const FptRenderConfig *configs, uint32_t count, FptHipBatch **out, char *error,
+                         size_t error_len) {
+    if (!out) {
+        set_error(error, error_len, "missing batch out-pointer");
+        return 1;
+    }
+    *out = nullptr;
+    if (!configs || count == 0 || count > 65536) {
+        set_error(error, error_len, "invalid scene count %u (1..65536)", count);
+        return 1;
+    }
+    for (uint32_t i = 0; i < count; ++i) {
+        if (validate_config(&configs[i], error, error_len)) return 1;
+        if (configs[i].renderer_backend != FPT_RENDERER_SDF) {
+            set_error(error, error_len, "scene %u: batched rendering supports the SDF backend only", i);
+            return 1;
+        }
+    }
+    if (check_device(error, error_len)) return 1;
+    FptHipBatch *b = new FptHipBatch();
+    b->host.assign(configs, configs + count);
+    uint64_t slot = 0, off = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        fpt::BatchScene sc;
+        sc.slot_begin = uint32_t(slot);
+        sc.slots = ((configs[i].width + 7u) / 8u) * ((configs[i].height + 7u) / 8u) * 64u;
+        sc.width = configs[i].width;
+        sc.height = configs[i].height;
+        sc.accum_offset = off;
+        slot += sc.slots;
+        off += uint64_t(configs[i].width) * configs[i].height;
+        b->scenes.push_back(sc);
+    }
+    if (slot > 0xffffffffull) {
+        set_error(error, error_len, "batch has too many pixels (%llu slots)", (unsigned long long)slot);
+        fpt_hip_batch_destroy(b);
+        return 1;
+    }
+    b->total_slots = uint32_t(slot);
+    b->pixels = size_t(off);
+    hipError_t e = hipMalloc(&b->dcfgs, sizeof(FptRenderConfig) * count);
+    if (e == hipSuccess) e = hipMalloc(&b->dscenes, sizeof(fpt::BatchScene) * count);
+    if (e == hipSuccess) e = hipMalloc(&b->counter, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemcpy(b->dcfgs, configs, sizeof(FptRenderConfig) * count, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(b->dscenes, b->scenes.data(), sizeof(fpt::BatchScene) * count, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        set_error(error, error_len, "batch upload failed: %s", hipGetErrorString(e));
+        fpt_hip_batch_destroy(b);
+        return 1;
+    }
+    *out = b;
+    return 0;
+}
+
+size_t fpt_hip_batch_pixels(const FptHipBatch *batch) { return batch ? batch->pixels : 0; }
+
+int fpt_hip_batch_tune(FptHipBatch *batch, uint32_t workgroups, uint32_t event_batch, uint32_t use_graph, char *error,
+                       size_t error_len) {
+    if (!batch) {
+        set_error(error, error_len, "missing batch");
+        return 1;
+    }
+    if (event_batch > 64u || workgroups > 65536u) {
+        set_error(error, error_len, "invalid launch shape (workgroups %u, event_batch %u)", workgroups, event_batch);
+        return 1;
+    }
+    batch->workgroups = workgroups;
+    batch->event_batch = event_batch ? event_batch : 24u;
+    batch->use_graph = use_graph;
+    return 0;
+}
+
+int fpt_hip_batch_render(FptHipBatch *batch, float *device_accum, void *stream, char *error, size_t error_len) {
+    if (!batch || !device_accum) {
+        set_error(error, error_len, "missing batch or accumulation buffer");
+        return 1;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const uint32_t wg = batch->workgroups ? batch->workgroups : default_workgroups();
+    float4 *acc = reinterpret_cast<float4 *>(device_accum);
+    if (!batch->use_graph) {
+        HIP_TRY(fpt::launch_sdf_batch(batch->dcfgs, batch->dscenes, uint32_t(batch->host.size()), batch->total_slots, acc,
+                                      batch->counter, wg, int(batch->event_batch), 0u, nullptr, s),
+                "batch kernel launch");
+        return 0;
+    }
+    if (!batch->exec || batch->graph_accum != device_accum || batch->graph_wg != wg ||
+        batch->graph_eb != batch->event_batch) {
+        batch_drop_graph(batch);
+        hipStream_t cap = nullptr;
+        HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "capture stream");
+        hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess)
+            e = fpt::launch_sdf_batch(batch->dcfgs, batch->dscenes, uint32_t(batch->host.size()), batch->total_slots,
+                                      acc, batch->counter, wg, int(batch->event_batch), 0u, nullptr, cap);
+        hipGraph_t g = nullptr;
+        hipError_t e2 = hipStreamEndCapture(cap, &g);
+        (void)hipStreamDestroy(cap);
+        if (e == hipSuccess) e = e2;
+        if (e == hipSuccess) e = hipGraphInstantiate(&batch->exec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            set_error(error, error_len, "batch graph capture failed: %s", hipGetErrorString(e));
+            return 1;
+        }
+        batch->graph = g;
+        batch->graph_accum = device_accum;
+        batch->graph_wg = wg;
+        batch->graph_eb = batch->event_batch;
+    }
+    HIP_TRY(hipGraphLaunch(batch->exec, s), "batch graph launch");
+    return 0;
+}
+
+void fpt_hip_batch_destroy(FptHipBatch *batch) {
+    if (!batch) return;
+    batch_drop_graph(batch);
+    if (batch->dcfgs) (void)hipFree(batch->dcfgs);
+    if (batch->dscenes) (void)hipFree(batch->dscenes);
+    if (batch->counter) (void)hipFree(batch->counter);
+    delete batch;
+}
+
+int fpt_hip_render_batch(const FptRenderConfig *configs, uint32_t count, float *linear_output, size_t linear_output_len,
+                         double *elapsed_ms, char *error, size_t error_len) {
+    FptHipBatch *b = nullptr;
+    if (fpt_hip_batch_create(configs, count, &b, error, error_len)) return 1;
+    struct G {
+        FptHipBatch *b;
+        ~G() { fpt_hip_batch_destroy(b); }
+    } guard{b};
+    if (linear_output && linear_output_len < b->pixels * 4) {
+        set_error(error, error_len, "linear output buffer is too small (%zu < %zu)", linear_output_len, b->pixels * 4);
+        return 1;
+    }
+    Stream st;
+    HIP_TRY(hipStreamCreate(&st.s), "hipStreamCreate");
+    DevBuf acc;
+    HIP_TRY(acc.alloc(b->pixels * sizeof(float4)), "batch accumulation allocation");
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0), "hipEventCreate");
+    HIP_TRY(hipEventCreate(&e1), "hipEventCreate");
+    HIP_TRY(hipEventRecord(e0, st.s), "hipEventRecord");
+    if (fpt_hip_batch_render(b, acc.as<float>(), st.s, error, error_len)) return 1;
+    HIP_TRY(hipEventRecord(e1, st.s), "hipEventRecord");
+    HIP_TRY(hipStreamSynchronize(st.s), "batch render");
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (elapsed_ms) *elapsed_ms = ms;
+    if (linear_output)
+        HIP_TRY(hipMemcpy(linear_output, acc.p, b->pixels * sizeof(float4), hipMemcpyDeviceToHost), "batch readback");
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: candidate launch shapes timed on a proxy of the workload (the
+// plan's first sample / the batch capped at one sample per pixel), probed in
+// a rotated order every round so drift does not favour a position, median of
+// the rounds, fastest median wins.
+// ---------------------------------------------------------------------------
+int fpt_hip_autotune(FptHipPlan *plan, FptHipBatch *batch, float *device_accum, void *stream, uint32_t rounds,
+                     FptTuneResult *result, char *error, size_t error_len) {
+    if ((!plan) == (!batch) || !device_accum) {
+        set_error(error, error_len, "autotune needs exactly one of plan/batch and a device buffer");
+        return 1;
+    }
+    if (plan && plan->host.renderer_backend != FPT_RENDERER_SDF) {
+        set_error(error, error_len, "autotune applies to the SDF backend");
+        return 1;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const uint32_t cus = default_workgroups() / 2u;
+    std::vector<std::pair<uint32_t, uint32_t>> cand;
+    for (uint32_t per_cu : {1u, 2u, 3u, 4u})
+        for (uint32_t eb : {8u, 16u, 24u, 32u, 40u}) cand.push_back({per_cu * cus, eb});
+    rounds = std::max<uint32_t>(1u, std::min<uint32_t>(rounds, 16u));
+    std::vector<std::vector<float>> times(cand.size());
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0), "hipEventCreate");
+    HIP_TRY(hipEventCreate(&e1), "hipEventCreate");
+    float4 *acc = reinterpret_cast<float4 *>(device_accum);
+    if (plan && !plan->counter) HIP_TRY(hipMalloc(&plan->counter, sizeof(uint32_t)), "counter allocation");
+    auto launch = [&](uint32_t wg, uint32_t eb) -> hipError_t {
+        if (plan) return fpt::launch_sdf_persistent(plan->dcfg, plan->host, acc, 0u, 1u, plan->counter, wg, int(eb),
+                                                    nullptr, s);
+        return fpt::launch_sdf_batch(batch->dcfgs, batch->dscenes, uint32_t(batch->host.size()), batch->total_slots, acc,
+                                     batch->counter, wg, int(eb), 1u, nullptr, s);
+    };
+    HIP_TRY(launch(cand[0].first, cand[0].second), "autotune warm-up");
+    for (uint32_t r = 0; r < rounds; ++r) {
+        for (size_t j = 0; j < cand.size(); ++j) {
+            size_t c = (j + r * 7u) % cand.size();  // rotated probe order
+            HIP_TRY(hipEventRecord(e0, s), "hipEventRecord");
+            HIP_TRY(launch(cand[c].first, cand[c].second), "autotune launch");
+            HIP_TRY(hipEventRecord(e1, s), "hipEventRecord");
+            HIP_TRY(hipEventSynchronize(e1), "autotune");
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+            times[c].push_back(ms);
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    size_t best = 0;
+    float best_ms = 1e30f, default_ms = 0.f;
+    for (size_t c = 0; c < cand.size(); ++c) {
+        std::vector<float> t = times[c];
+        std::sort(t.begin(), t.end());
+        float med = t[t.size() / 2];
+        if (cand[c].first == 2u * cus && cand[c].second == 24u) default_ms = med;
+        if (med < best_ms) {
+            best_ms = med;
+            best = c;
+        }
+    }
+    if (plan) {
+        plan->kernel = 1u;
+        plan->workgroups = cand[best].first;
+        plan->event_batch = cand[best].second;
+    } else {
+        batch->workgroups = cand[best].first;
+        batch->event_batch = cand[best].second;
+    }
+    if (result) {
+        result->workgroups = cand[best].first;
+        result->event_batch = cand[best].second;
+        result->candidates = uint32_t(cand.size());
+        result->rounds = rounds;
+        result->best_ms = best_ms;
+        result->default_ms = default_ms;
+    }
+    return 0;
+}
+
+}  // extern "C"
+
+extern "C" {
+
 int fpt_metal_device_name(char *name, size_t name_len) {
     if (!name || name_len == 0) return 1;
     int dev = 0;
@@ -618,6 +895,95 @@ int fpt_metal_preview(const char *, const char *, const char *const *, const cha
                       size_t error_len) {
     set_error(error, error_len, "the interactive preview has no counterpart in this library");
     return 1;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Device plumbing for callers without their own HIP runtime handle (tests,
+// bench): buffers, streams, a graph-safe fill, and HIP-event timing of
+// repeated renders on the launch stream.
+// ---------------------------------------------------------------------------
+namespace {
+__global__ void fill_f32_kernel(float *dst, float v, size_t n) {
+    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (; i < n; i += stride) dst[i] = v;
+}
+}  // namespace
+
+extern "C" {
+
+int fpt_hip_device_alloc(size_t bytes, void **ptr, char *error, size_t error_len) {
+    if (!ptr) {
+        set_error(error, error_len, "missing pointer out-parameter");
+        return 1;
+    }
+    *ptr = nullptr;
+    if (check_device(error, error_len)) return 1;
+    HIP_TRY(hipMalloc(ptr, bytes ? bytes : 1), "device allocation");
+    return 0;
+}
+
+void fpt_hip_device_free(void *ptr) {
+    if (ptr) (void)hipFree(ptr);
+}
+
+int fpt_hip_copy(void *dst, const void *src, size_t bytes, uint32_t to_device, char *error, size_t error_len) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost), "copy");
+    return 0;
+}
+
+int fpt_hip_fill_f32(float *dst, float value, size_t count, void *stream, char *error, size_t error_len) {
+    if (!count) return 0;
+    unsigned blocks = unsigned(std::min<size_t>((count + 255) / 256, 65536));
+    hipLaunchKernelGGL(fill_f32_kernel, dim3(blocks), dim3(256), 0, static_cast<hipStream_t>(stream), dst, value, count);
+    HIP_TRY(hipGetLastError(), "fill kernel launch");
+    return 0;
+}
+
+int fpt_hip_stream_create(void **stream, char *error, size_t error_len) {
+    hipStream_t s = nullptr;
+    if (check_device(error, error_len)) return 1;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+    *stream = s;
+    return 0;
+}
+
+void fpt_hip_stream_destroy(void *stream) {
+    if (stream) (void)hipStreamDestroy(static_cast<hipStream_t>(stream));
+}
+
+int fpt_hip_synchronize(void *stream, char *error, size_t error_len) {
+    HIP_TRY(stream ? hipStreamSynchronize(static_cast<hipStream_t>(stream)) : hipDeviceSynchronize(), "synchronize");
+    return 0;
+}
+
+int fpt_hip_time_renders(FptHipPlan *plan, FptHipBatch *batch, float *device_accum, void *stream, uint32_t steps,
+                         double *elapsed_ms, char *error, size_t error_len) {
+    if ((!plan) == (!batch) || !device_accum || !elapsed_ms) {
+        set_error(error, error_len, "time_renders needs exactly one of plan/batch, a buffer and an output");
+        return 1;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipEvent_t e0, e1;
+    HIP_TRY(hipStreamSynchronize(s), "synchronize");
+    HIP_TRY(hipEventCreate(&e0), "hipEventCreate");
+    HIP_TRY(hipEventCreate(&e1), "hipEventCreate");
+    HIP_TRY(hipEventRecord(e0, s), "hipEventRecord");
+    for (uint32_t i = 0; i < steps; ++i) {
+        int rc = plan ? fpt_hip_plan_render(plan, device_accum, stream, error, error_len)
+                      : fpt_hip_batch_render(batch, device_accum, stream, error, error_len);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipEventRecord(e1, s), "hipEventRecord");
+    HIP_TRY(hipEventSynchronize(e1), "timed renders");
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *elapsed_ms = ms;
+    return 0;
 }
 
 }  // extern "C"

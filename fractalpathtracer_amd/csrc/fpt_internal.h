@@ -22,6 +22,21 @@ struct DeviceProfile {
     unsigned long long pixels;
 };
 
+// One scene of a batched launch: its work slots are
+// [slot_begin, slot_begin + slots) of the global slot range (8x8 tiles of its
+// pixels), its image is accum[accum_offset .. accum_offset + width*height).
+struct BatchScene {
+    uint32_t slot_begin;
+    uint32_t slots;
+    uint32_t width;
+    uint32_t height;
+    unsigned long long accum_offset;
+};
+
+hipError_t launch_sdf_batch(const FptRenderConfig *dcfgs, const BatchScene *dscenes, uint32_t nscenes,
+                            uint32_t total_slots, float4 *accum, uint32_t *counter, uint32_t workgroups,
+                            int event_batch, uint32_t spp_cap, DeviceProfile *prof, hipStream_t stream);
+
 hipError_t launch_sdf_accumulate(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, float4 *accum,
                                  uint32_t start, uint32_t count, DeviceProfile *prof, hipStream_t stream);
 hipError_t launch_trace_primary(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, FptRayHit *hits,

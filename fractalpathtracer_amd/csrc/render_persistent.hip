@@ -1,4 +1,5 @@
-// Persistent, path-regenerating SDF path tracer for gfx950.
+// Persistent, path-regenerating SDF path tracer for gfx950, for one scene or
+// a batch of scenes in one launch.
 //
 // The straightforward kernel (render.hip) runs one path per lane from camera
 // to termination: a 64-lane wave then waits for its slowest ray at every
@@ -15,11 +16,18 @@
 //   IDLE   no pixel left.
 //
 // When a path ends the lane starts the pixel's next sample; when a pixel's
-// samples are done it writes the running mean and takes the next pixel from
-// a global counter (one atomic per wave, pixels in 8x8 tile order so a wave
-// keeps working on neighbouring pixels).  Samples of a pixel are processed
-// in order by one lane, so the running mean mix(c, s, 1/(i+1)) and every
-// RNG key are exactly those of the straightforward kernel.
+// samples are done it writes the running mean and takes the next work slot
+// from a global counter (one atomic per wave).  Slots are pixels in 8x8 tile
+// order, scene after scene for a batch, so the lanes of a wave work on
+// neighbouring pixels of one scene except at scene boundaries.  Samples of a
+// pixel are processed in order by one lane, so the running mean
+// mix(c, s, 1/(i+1)) and every RNG key are exactly those of the
+// straightforward kernel and the images are bit-identical.
+//
+// MULTI = false: one scene, its config read through a kernel-argument pointer
+// (wave-uniform scalar loads).  MULTI = true: every lane carries the pointer
+// to its scene's config (vector loads, cached), scenes may differ in size,
+// spp and program.
 #include <hip/hip_runtime.h>
 
 #include "fpt_device.h"
@@ -35,9 +43,9 @@ struct PersistentCounters {
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 
-// Wave-level fetch of `want` pixels: one atomic per wave, consecutive indices
-// to the requesting lanes in lane order.  Must be reached by every lane.
-__device__ __forceinline__ uint32_t fetch_pixel(bool want, uint32_t *counter) {
+// Wave-level fetch of work slots: one atomic per wave, consecutive indices to
+// the requesting lanes in lane order.  Must be reached by every lane.
+__device__ __forceinline__ uint32_t fetch_slot(bool want, uint32_t *counter) {
     const unsigned long long m = __ballot(want);
     uint32_t base = 0;
     if (m) {
@@ -49,7 +57,7 @@ __device__ __forceinline__ uint32_t fetch_pixel(bool want, uint32_t *counter) {
     return base + uint32_t(__popcll(below));
 }
 
-// Pixel index -> (x, y): 8x8 tiles, tiles row-major.
+// Slot index within a scene -> (x, y): 8x8 tiles, tiles row-major.
 __device__ __forceinline__ void pixel_of(uint32_t k, uint32_t w, uint32_t &x, uint32_t &y) {
     const uint32_t tiles_x = (w + 7u) >> 3;
     const uint32_t tile = k >> 6, in = k & 63u;
@@ -57,56 +65,80 @@ __device__ __forceinline__ void pixel_of(uint32_t k, uint32_t w, uint32_t &x, ui
     y = (tile / tiles_x) * 8u + (in >> 3);
 }
 
-template <bool PROFILE>
-__global__ __launch_bounds__(256) void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgp,
+__host__ __device__ __forceinline__ uint32_t slot_count(uint32_t w, uint32_t h) {
+    return ((w + 7u) / 8u) * ((h + 7u) / 8u) * 64u;
+}
+
+template <bool PROFILE, bool MULTI>
+__global__ __launch_bounds__(256) void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
+                                                             const BatchScene *__restrict__ scenes, uint32_t nscenes,
                                                              float4 *__restrict__ accum, uint32_t start,
                                                              uint32_t count, uint32_t *__restrict__ counter,
-                                                             uint32_t slots, int event_batch, DeviceProfile *prof) {
-    const FptRenderConfig &cfg = *cfgp;
-    const uint32_t W = cfg.width, H = cfg.height;
-    const uint32_t total = cfg.samples < 1u ? 1u : (cfg.samples > 512u ? 512u : cfg.samples);
-    const uint32_t s_end = start + count < total ? start + count : total;
-    const f3 cam = ld3(cfg.camera_position);
-    const float focal = 1.0f / tanf(cfg.camera_fov / 2.0f * kPi / 180.0f);
-    const float aa = 0.3f / fmaxf(float(W), float(H));
-    const float maxd = cfg.render[4];
-    const float maxd2 = maxd * maxd;
-    const float far = maxd * 0.99f;
-    const float far2 = far * far;
-    const float min_dist = cfg.render[3];
-    const float adapt = cfg.render[5];
-    const bool check_t = cfg.sdf_id == FPT_SDF_PROGRAM && cfg.program_material[5] > 0.0f;
-    int bounces = int(cfg.render[0]) < 8 ? int(cfg.render[0]) : 8;
-    if (cfg.sdf_bounce_cap > 0u && int(cfg.sdf_bounce_cap) < bounces) bounces = int(cfg.sdf_bounce_cap);
-    const int ni_first = int(cfg.render[1]);
-    const int ni_next = int(cfg.render[1] / (cfg.render[5] * 2.0f + 1.0f));
-    const bool sun_on = cfg.sun[0] == 1.0f;
-    const f3 sun_dir0 = rotate_camera(mk(0.f, 0.f, 1.f), cfg.sun[1] * kPi / 180.0f, cfg.sun[2] * kPi / 180.0f);
+                                                             uint32_t total_slots, int event_batch,
+                                                             DeviceProfile *prof) {
+    const FptRenderConfig *cp = cfgs;  // the lane's scene (fixed when !MULTI)
+    float4 *out = accum;               // the lane's scene accumulation buffer
 
     Work w = {0, 0};
     PersistentCounters pc = {0, 0, 0, 0, 0};
     int pixels_done = 0;
+
+    // Per-scene constants, refreshed when a lane takes a pixel.
+    uint32_t W = 0, H = 0, s_end = 0, s_begin = 0;
+    f3 cam = mk(0.f, 0.f, 0.f);
+    float focal = 1.f, aa = 0.f, maxd = 0.f, maxd2 = 0.f, far2 = 0.f, min_dist = 0.f, adapt = 0.f;
+    bool check_t = false, sun_on = false;
+    int bounces = 0, ni_first = 0, ni_next = 0;
+    f3 sun_dir0 = mk(0.f, 0.f, 1.f);
+    auto load_scene = [&]() {
+        const FptRenderConfig &cfg = *cp;
+        const uint32_t total = cfg.samples < 1u ? 1u : (cfg.samples > 512u ? 512u : cfg.samples);
+        s_begin = MULTI ? 0u : start;
+        // MULTI: `count` caps the samples per pixel (0 = all; autotuning proxies).
+        s_end = MULTI ? (count && count < total ? count : total) : (start + count < total ? start + count : total);
+        cam = ld3(cfg.camera_position);
+        focal = 1.0f / tanf(cfg.camera_fov / 2.0f * kPi / 180.0f);
+        aa = 0.3f / fmaxf(float(W), float(H));
+        maxd = cfg.render[4];
+        maxd2 = maxd * maxd;
+        const float far = maxd * 0.99f;
+        far2 = far * far;
+        min_dist = cfg.render[3];
+        adapt = cfg.render[5];
+        check_t = cfg.sdf_id == FPT_SDF_PROGRAM && cfg.program_material[5] > 0.0f;
+        bounces = int(cfg.render[0]) < 8 ? int(cfg.render[0]) : 8;
+        if (cfg.sdf_bounce_cap > 0u && int(cfg.sdf_bounce_cap) < bounces) bounces = int(cfg.sdf_bounce_cap);
+        ni_first = int(cfg.render[1]);
+        ni_next = int(cfg.render[1] / (cfg.render[5] * 2.0f + 1.0f));
+        sun_on = cfg.sun[0] == 1.0f;
+        sun_dir0 = rotate_camera(mk(0.f, 0.f, 1.f), cfg.sun[1] * kPi / 180.0f, cfg.sun[2] * kPi / 180.0f);
+    };
+    if (!MULTI) {
+        W = cfgs->width;
+        H = cfgs->height;
+        load_scene();
+    }
 
     // Lane state.
     int phase = PH_IDLE;
     uint32_t pix = 0, s = 0;
     UV uv = {0.f, 0.f};
     f3 acc = mk(0.f, 0.f, 0.f);
-    // path
-    int bounce = 0, local_ni = ni_first;
+    int bounce = 0, local_ni = 0;
     float side = 1.0f, sky_mask = 0.0f;
     f3 light = mk(0.f, 0.f, 0.f), color = mk(1.f, 1.f, 1.f), gcol = mk(0.f, 0.f, 0.f), dr = mk(0.f, 0.f, 0.f);
     f3 hit_p = mk(0.f, 0.f, 0.f), hit_n = mk(0.f, 0.f, 0.f);
     Material mat;
     mat.rgb = mk(1.f, 1.f, 1.f);
     mat.roughness = 1.f; mat.specular = 0.f; mat.translucency = 0.f; mat.ior = 1.5f; mat.emission = 0.f;
-    // march
     f3 p = mk(0.f, 0.f, 0.f), mdir = mk(0.f, 0.f, 1.f), mstart = mk(0.f, 0.f, 0.f);
     int it = 0, max_iter = 0;
     bool shadow = false;
+    bool retry = false;  // slot was tile padding: take another
 
     // Start the camera path of sample `s` of the current pixel.
     auto start_path = [&]() {
+        const FptRenderConfig &cfg = *cp;
         const float frame = float(s);
         float jx, jy;
         random_point(aa, uv, frame, jx, jy);
@@ -129,26 +161,43 @@ __global__ __launch_bounds__(256) void sdf_persistent_kernel(const FptRenderConf
         phase = PH_MARCH;
     };
 
-    // Take a pixel (or go idle).  Wave-uniform call.
-    auto take_pixel = [&](bool want) {
-        const uint32_t k = fetch_pixel(want, counter);
+    // Take a work slot (or go idle).  Wave-uniform call.
+    auto take_slot = [&](bool want) {
+        const uint32_t k = fetch_slot(want, counter);
+        retry = false;
         if (!want) return;
-        if (k >= slots) {
+        if (k >= total_slots) {
             phase = PH_IDLE;
             return;
         }
+        uint32_t local = k;
+        if (MULTI) {
+            // Binary search of the scene owning slot k.
+            uint32_t lo = 0, hi = nscenes - 1u;
+            while (lo < hi) {
+                uint32_t mid = (lo + hi + 1u) >> 1;
+                if (scenes[mid].slot_begin <= k) lo = mid;
+                else hi = mid - 1u;
+            }
+            const BatchScene sc = scenes[lo];
+            cp = cfgs + lo;
+            out = accum + sc.accum_offset;
+            W = sc.width;
+            H = sc.height;
+            local = k - sc.slot_begin;
+            load_scene();
+        }
         uint32_t x, y;
-        pixel_of(k, W, x, y);
-        if (x >= W || y >= H) {  // tile padding: nothing to render, take another
-            phase = PH_EVENT;     // handled as "needs a pixel" below
-            pix = 0xffffffffu;
+        pixel_of(local, W, x, y);
+        if (x >= W || y >= H) {
+            retry = true;
             return;
         }
         pix = y * W + x;
         uv = screen_uv(x, y, W, H);
-        s = start;
-        if (start != 0u) {
-            float4 prev = accum[pix];
+        s = s_begin;
+        if (s_begin != 0u) {
+            float4 prev = out[pix];
             acc = mk(prev.x, prev.y, prev.z);
         } else {
             acc = mk(0.f, 0.f, 0.f);
@@ -156,27 +205,20 @@ __global__ __launch_bounds__(256) void sdf_persistent_kernel(const FptRenderConf
         start_path();
     };
 
-    take_pixel(true);
-    // Tile-padding lanes ask again until every lane has a real pixel or none.
-    while (true) {
-        bool need = phase == PH_EVENT && pix == 0xffffffffu;
-        if (!__any(need)) break;
-        take_pixel(need);
-    }
+    take_slot(true);
+    while (__any(retry)) take_slot(retry);
 
     while (true) {
         if (phase == PH_MARCH) {
+            const FptRenderConfig &cfg = *cp;
             const float d = map_sdf(p, cfg, w);
             ++it;
             bool stop = !isfinite(d);
             if (!stop) {
                 const float o = fabsf(d) * 0.99f;
                 p = p + mdir * o;
-                float lod = min_dist;
-                {
-                    const f3 df = mstart - p;  // lod_falloff 0.0002 (SM:3771-3776)
-                    lod = mixf(min_dist, mixf(lod, 0.1f, dot(df, df) * 0.0002f), adapt);
-                }
+                const f3 df = mstart - p;  // lod_falloff 0.0002 (SM:3771-3776)
+                float lod = mixf(min_dist, mixf(min_dist, 0.1f, dot(df, df) * 0.0002f), adapt);
                 if (check_t && user_material(p, cfg, w).translucency > 0.0f) lod = 0.0001f;
                 stop = o < lod || maxd < o || it >= max_iter;
             }
@@ -187,8 +229,9 @@ __global__ __launch_bounds__(256) void sdf_persistent_kernel(const FptRenderConf
         if (!ev && !mar) break;
         if (!(__popcll(ev) >= event_batch || (ev && !mar))) continue;
 
-        bool need_pixel = false;
+        bool need_slot = false;
         if (phase == PH_EVENT) {
+            const FptRenderConfig &cfg = *cp;
             const float frame = float(s);
             bool path_done = false;
             bool do_scatter = false;
@@ -302,19 +345,15 @@ __global__ __launch_bounds__(256) void sdf_persistent_kernel(const FptRenderConf
                 if (s < s_end) {
                     start_path();
                 } else {
-                    accum[pix] = make_float4(acc.x, acc.y, acc.z, 1.0f);
+                    out[pix] = make_float4(acc.x, acc.y, acc.z, 1.0f);
                     ++pixels_done;
-                    need_pixel = true;
+                    need_slot = true;
                 }
             }
         }
-        // Wave-uniform pixel hand-out for lanes whose pixel is finished.
-        take_pixel(need_pixel);
-        while (true) {
-            bool need = phase == PH_EVENT && pix == 0xffffffffu;
-            if (!__any(need)) break;
-            take_pixel(need);
-        }
+        // Wave-uniform hand-out for lanes whose pixel is finished.
+        take_slot(need_slot);
+        while (__any(retry)) take_slot(retry);
     }
 
     if (PROFILE) {
@@ -333,20 +372,35 @@ __global__ __launch_bounds__(256) void sdf_persistent_kernel(const FptRenderConf
     }
 }
 
+// Counter reset as a kernel (not a memset node), so a captured graph replays it.
+__global__ void zero_counter_kernel(uint32_t *c) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0u;
+}
+
 hipError_t launch_sdf_persistent(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, float4 *accum,
                                  uint32_t start, uint32_t count, uint32_t *counter, uint32_t workgroups,
                                  int event_batch, DeviceProfile *prof, hipStream_t stream) {
-    // Pixel slots in 8x8 tile order (tiles padded to whole 8x8 blocks).
-    const uint32_t tiles = ((hcfg.width + 7u) / 8u) * ((hcfg.height + 7u) / 8u);
-    const uint32_t slots = tiles * 64u;
-    hipError_t e = hipMemsetAsync(counter, 0, sizeof(uint32_t), stream);
-    if (e != hipSuccess) return e;
+    const uint32_t slots = slot_count(hcfg.width, hcfg.height);
+    hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, counter);
     if (prof)
-        hipLaunchKernelGGL(sdf_persistent_kernel<true>, dim3(workgroups), dim3(256), 0, stream, dcfg, accum, start,
-                           count, counter, slots, event_batch, prof);
+        hipLaunchKernelGGL((sdf_persistent_kernel<true, false>), dim3(workgroups), dim3(256), 0, stream, dcfg, nullptr,
+                           1u, accum, start, count, counter, slots, event_batch, prof);
     else
-        hipLaunchKernelGGL(sdf_persistent_kernel<false>, dim3(workgroups), dim3(256), 0, stream, dcfg, accum, start,
-                           count, counter, slots, event_batch, nullptr);
+        hipLaunchKernelGGL((sdf_persistent_kernel<false, false>), dim3(workgroups), dim3(256), 0, stream, dcfg, nullptr,
+                           1u, accum, start, count, counter, slots, event_batch, nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_sdf_batch(const FptRenderConfig *dcfgs, const BatchScene *dscenes, uint32_t nscenes,
+                            uint32_t total_slots, float4 *accum, uint32_t *counter, uint32_t workgroups,
+                            int event_batch, uint32_t spp_cap, DeviceProfile *prof, hipStream_t stream) {
+    hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, counter);
+    if (prof)
+        hipLaunchKernelGGL((sdf_persistent_kernel<true, true>), dim3(workgroups), dim3(256), 0, stream, dcfgs, dscenes,
+                           nscenes, accum, 0u, spp_cap, counter, total_slots, event_batch, prof);
+    else
+        hipLaunchKernelGGL((sdf_persistent_kernel<false, true>), dim3(workgroups), dim3(256), 0, stream, dcfgs,
+                           dscenes, nscenes, accum, 0u, spp_cap, counter, total_slots, event_batch, nullptr);
     return hipGetLastError();
 }
 

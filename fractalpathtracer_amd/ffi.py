@@ -211,12 +211,18 @@ class FptVoxelBuildInfo(C.Structure):
                 ("active_cells", C.c_uint64), ("memory_bytes", C.c_uint64), ("build_ms", C.c_double)]
 
 
+class FptTuneResult(C.Structure):
+    _fields_ = [("workgroups", u32), ("event_batch", u32), ("candidates", u32), ("rounds", u32),
+                ("best_ms", f32), ("default_ms", f32)]
+
+
 # Expected sizes (src/ffi.rs:494-507 plus the new FptRayHit).
 EXPECTED_SIZES = {
     FptSdfInstruction: 32, FptPrimitiveInstance: 80, FptAffineTransform: 64,
     FptIndexedPrimitive: 32, FptTypedSoAProgram: 5136, FptStitchPipelineStats: 72,
     FptRenderConfig: 30448, FptSdfProfileStats: 288, FptDiagnosticConfig: 24,
     FptMandelbulberFieldSample: 16, FptAsyncJitStats: 40, FptRayHit: 40, FptVoxelBuildInfo: 40,
+    FptTuneResult: 24,
 }
 
 PKG_DIR = Path(__file__).resolve().parent
@@ -229,7 +235,10 @@ EXPORTED_SYMBOLS = (
     "fpt_test_async_stitch_context", "fpt_test_typed_soa", "fpt_metal_preview",
     "fpt_hip_trace_primary", "fpt_hip_plan_create", "fpt_hip_plan_render",
     "fpt_hip_plan_destroy", "fpt_hip_last_error", "fpt_hip_program_surface", "fpt_hip_voxel_build",
-    "fpt_hip_voxel_trace_cells", "fpt_hip_plan_tune",
+    "fpt_hip_voxel_trace_cells", "fpt_hip_plan_tune", "fpt_hip_batch_create", "fpt_hip_batch_pixels",
+    "fpt_hip_batch_tune", "fpt_hip_batch_render", "fpt_hip_batch_destroy", "fpt_hip_render_batch",
+    "fpt_hip_autotune", "fpt_hip_device_alloc", "fpt_hip_device_free", "fpt_hip_copy", "fpt_hip_fill_f32",
+    "fpt_hip_stream_create", "fpt_hip_stream_destroy", "fpt_hip_synchronize", "fpt_hip_time_renders",
 )
 
 _lib = None
@@ -279,6 +288,39 @@ def _declare(lib):
     lib.fpt_hip_program_surface.restype = C.c_int
     lib.fpt_hip_plan_tune.argtypes = [C.c_void_p, u32, u32, u32, cp, C.c_size_t]
     lib.fpt_hip_plan_tune.restype = C.c_int
+    lib.fpt_hip_batch_create.argtypes = [P(FptRenderConfig), u32, P(C.c_void_p), cp, C.c_size_t]
+    lib.fpt_hip_batch_create.restype = C.c_int
+    lib.fpt_hip_batch_pixels.argtypes = [C.c_void_p]
+    lib.fpt_hip_batch_pixels.restype = C.c_size_t
+    lib.fpt_hip_batch_tune.argtypes = [C.c_void_p, u32, u32, u32, cp, C.c_size_t]
+    lib.fpt_hip_batch_tune.restype = C.c_int
+    lib.fpt_hip_batch_render.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, cp, C.c_size_t]
+    lib.fpt_hip_batch_render.restype = C.c_int
+    lib.fpt_hip_batch_destroy.argtypes = [C.c_void_p]
+    lib.fpt_hip_batch_destroy.restype = None
+    lib.fpt_hip_render_batch.argtypes = [P(FptRenderConfig), u32, P(f32), C.c_size_t, P(C.c_double), cp,
+                                         C.c_size_t]
+    lib.fpt_hip_render_batch.restype = C.c_int
+    lib.fpt_hip_autotune.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, u32, P(FptTuneResult), cp,
+                                     C.c_size_t]
+    lib.fpt_hip_autotune.restype = C.c_int
+    lib.fpt_hip_device_alloc.argtypes = [C.c_size_t, P(C.c_void_p), cp, C.c_size_t]
+    lib.fpt_hip_device_alloc.restype = C.c_int
+    lib.fpt_hip_device_free.argtypes = [C.c_void_p]
+    lib.fpt_hip_device_free.restype = None
+    lib.fpt_hip_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, u32, cp, C.c_size_t]
+    lib.fpt_hip_copy.restype = C.c_int
+    lib.fpt_hip_fill_f32.argtypes = [C.c_void_p, f32, C.c_size_t, C.c_void_p, cp, C.c_size_t]
+    lib.fpt_hip_fill_f32.restype = C.c_int
+    lib.fpt_hip_stream_create.argtypes = [P(C.c_void_p), cp, C.c_size_t]
+    lib.fpt_hip_stream_create.restype = C.c_int
+    lib.fpt_hip_stream_destroy.argtypes = [C.c_void_p]
+    lib.fpt_hip_stream_destroy.restype = None
+    lib.fpt_hip_synchronize.argtypes = [C.c_void_p, cp, C.c_size_t]
+    lib.fpt_hip_synchronize.restype = C.c_int
+    lib.fpt_hip_time_renders.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, u32, P(C.c_double), cp,
+                                         C.c_size_t]
+    lib.fpt_hip_time_renders.restype = C.c_int
     lib.fpt_hip_last_error.argtypes = []
     lib.fpt_hip_last_error.restype = C.c_char_p
 

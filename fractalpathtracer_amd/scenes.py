@@ -261,6 +261,65 @@ def voxel_mandelbox_scene(width=1920, height=1080, spp=4, resolution=1024) -> Fp
     return c
 
 
+def _splitmix64(x: int) -> int:
+    x = (x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return x ^ (x >> 31)
+
+
+def _uniform(seed: int, scene: int, field: int) -> float:
+    """Counter-based uniform in [0, 1) of (seed, scene, field)."""
+    return _splitmix64((seed << 40) ^ (scene << 16) ^ field) / 2.0 ** 64
+
+
+def look_at(c: FptRenderConfig, position, target=(0.0, 0.0, 0.0)) -> None:
+    """Camera yaw/pitch so that rotateCamera((0,0,1)) points at `target`
+    (Shaders.metal:460-471: dir = (cos p sin y, sin p, cos p cos y))."""
+    f = [t - p for t, p in zip(target, position)]
+    n = math.sqrt(sum(v * v for v in f))
+    f = [v / n for v in f]
+    _set(c.camera_position, position)
+    _set(c.camera_yaw_pitch, [math.atan2(f[0], f[2]), math.asin(f[1])])
+
+
+def batch_scenes(count=64, seed=0, min_tile=64, max_tile=512, max_spp=16):
+    """Batched uneven workload: `count` scenes, family i mod 3 (Mandelbulb
+    power U[2,10], Mandelbox scale U[-2.5,2.5], Menger U{3..6} iterations),
+    tile width/height U{min_tile..max_tile}, spp U{1..max_spp}, 2 bounces;
+    every parameter from a counter-based hash of (seed, scene)."""
+    out = []
+    for i in range(count):
+        u = lambda f: _uniform(seed, i, f)  # noqa: E731
+        w = min_tile + int(u(1) * (max_tile - min_tile + 1))
+        h = min_tile + int(u(2) * (max_tile - min_tile + 1))
+        spp = 1 + int(u(3) * max_spp)
+        c = default_config()
+        c.width, c.height, c.samples = w, h, spp
+        c.camera_fov, c.camera_dof = 60.0, 0.0
+        _set(c.render, [3.0, 256.0, 0.0005, 1e-4, 100.0, 0.0, 0.0, 0.0])
+        _c1_lighting(c)
+        _set(c.world, [3.0, 1.0, 0.0, 0.0, 0.3, 1.0, 0.0])
+        _set(c.world_one_color, [0.4, 0.5, 0.65])
+        fam = i % 3
+        if fam == 0:
+            power = 2.0 + 8.0 * u(4)
+            set_program(c, [(ffi.OP_MANDELBULB, 0, [power, 12.0, 2.0, 0.0])])
+            look_at(c, (0.3, 0.4, -2.8))
+        elif fam == 1:
+            scale = -2.5 + 5.0 * u(4)
+            set_program(c, [(ffi.OP_MANDELBOX, 0, [scale, 10.0, 0.5, 1.0])])
+            look_at(c, (1.0, 1.5, -6.5))
+        else:
+            iters = 3 + int(u(4) * 4)
+            set_program(c, [(ffi.OP_ROTATE_Y, 0, [0.5, 0, 0, 0]), (ffi.OP_MENGER, 0, [float(iters), 1.0, 3.0, 0.0])])
+            look_at(c, (1.4, 1.1, -3.2))
+        c.material_mode = 1
+        _set(c.program_material, [0.8, 0.8, 0.8, 0.9, 0.1, 0.0, 1.5, 0.0])
+        out.append(c)
+    return out
+
+
 def clone(c: FptRenderConfig) -> FptRenderConfig:
     out = FptRenderConfig()
     C.memmove(C.byref(out), C.byref(c), C.sizeof(FptRenderConfig))

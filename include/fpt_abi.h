@@ -762,6 +762,99 @@ int fpt_hip_voxel_trace_cells(const struct FptRenderConfig *config,
                               char *error,
                               size_t error_len);
 
+/*
+ * Batched multi-scene rendering: `count` independent SDF scenes (each its own
+ * size, spp, camera and program) rendered by one persistent launch that
+ * hands out 8x8 pixel tiles scene after scene from one work counter.  Scene
+ * i's image occupies float4 pixels [off_i, off_i + W_i*H_i) of the output,
+ * off_i = sum of W_j*H_j for j < i; every scene's image is bit-identical to
+ * rendering it alone.  With use_graph (default 1) the launch sequence is
+ * captured once per output buffer into a HIP graph and replayed.
+ * fpt_hip_batch_render enqueues on `stream` without synchronising.
+ */
+typedef struct FptHipBatch FptHipBatch;
+
+int fpt_hip_batch_create(const struct FptRenderConfig *configs,
+                         uint32_t count,
+                         FptHipBatch **batch,
+                         char *error,
+                         size_t error_len);
+size_t fpt_hip_batch_pixels(const FptHipBatch *batch);
+int fpt_hip_batch_tune(FptHipBatch *batch,
+                       uint32_t workgroups,
+                       uint32_t event_batch,
+                       uint32_t use_graph,
+                       char *error,
+                       size_t error_len);
+int fpt_hip_batch_render(FptHipBatch *batch,
+                         float *device_accum,
+                         void *stream,
+                         char *error,
+                         size_t error_len);
+void fpt_hip_batch_destroy(FptHipBatch *batch);
+
+/* Synchronous convenience: render a batch into host memory
+ * (linear_output_len >= 4 * sum W_i*H_i floats). */
+int fpt_hip_render_batch(const struct FptRenderConfig *configs,
+                         uint32_t count,
+                         float *linear_output,
+                         size_t linear_output_len,
+                         double *elapsed_ms,
+                         char *error,
+                         size_t error_len);
+
+/*
+ * Autotuner for exactly one of `plan` / `batch`: 20 launch shapes
+ * (workgroups 1-4 per CU x event_batch 8..40) timed on a proxy of the
+ * workload (the plan's first sample; the batch capped at one sample per
+ * pixel) written into device_accum (contents are overwritten), `rounds`
+ * rounds in a rotated probe order, median per shape; the fastest shape is
+ * stored in the plan/batch and reported.  Synchronous.
+ */
+struct FptTuneResult {
+    uint32_t workgroups;
+    uint32_t event_batch;
+    uint32_t candidates;
+    uint32_t rounds;
+    float best_ms;
+    float default_ms;   /* median of the default shape (2 per CU, 24) */
+};
+FPT_ABI_ASSERT(sizeof(struct FptTuneResult) == 24, "FptTuneResult");
+
+int fpt_hip_autotune(FptHipPlan *plan,
+                     FptHipBatch *batch,
+                     float *device_accum,
+                     void *stream,
+                     uint32_t rounds,
+                     struct FptTuneResult *result,
+                     char *error,
+                     size_t error_len);
+
+/*
+ * Device plumbing for callers that do not hold a HIP runtime of their own
+ * (all on the runtime this library links): device buffers, host<->device
+ * copies (to_device 1 = host to device), a float fill that is safe inside
+ * captured graphs, non-blocking streams, and HIP-event timing of `steps`
+ * renders of a plan or batch on `stream` (exactly one of plan / batch).
+ */
+int fpt_hip_device_alloc(size_t bytes, void **ptr, char *error, size_t error_len);
+void fpt_hip_device_free(void *ptr);
+int fpt_hip_copy(void *dst, const void *src, size_t bytes, uint32_t to_device,
+                 char *error, size_t error_len);
+int fpt_hip_fill_f32(float *dst, float value, size_t count, void *stream,
+                     char *error, size_t error_len);
+int fpt_hip_stream_create(void **stream, char *error, size_t error_len);
+void fpt_hip_stream_destroy(void *stream);
+int fpt_hip_synchronize(void *stream, char *error, size_t error_len);
+int fpt_hip_time_renders(FptHipPlan *plan,
+                         FptHipBatch *batch,
+                         float *device_accum,
+                         void *stream,
+                         uint32_t steps,
+                         double *elapsed_ms,
+                         char *error,
+                         size_t error_len);
+
 /* Last error string of the calling thread (empty when none). */
 const char *fpt_hip_last_error(void);
 

@@ -1,0 +1,69 @@
+"""Batched multi-scene persistent launch: every scene's image equals the
+scene rendered alone (bit-identical), graph replays equal eager launches,
+and scene images stay within the FP64 oracle's parity bar."""
+import numpy as np
+import pytest
+
+from fractalpathtracer_amd import api, scenes
+from oracle import bind
+
+pytestmark = pytest.mark.gpu
+
+
+def _small_batch(n=9, seed=3):
+    return scenes.batch_scenes(count=n, seed=seed, min_tile=24, max_tile=72, max_spp=3)
+
+
+def test_batch_scenes_equal_single_scene_renders():
+    cfgs = _small_batch()
+    imgs, ms = api.render_batch(cfgs)
+    assert ms > 0
+    for i, c in enumerate(cfgs):
+        alone = api.render(c)
+        assert np.array_equal(imgs[i], alone), f"scene {i} ({c.width}x{c.height}, {c.samples} spp)"
+
+
+def test_graph_replays_match_eager_launch():
+    cfgs = _small_batch(n=6, seed=4)
+    st = api.Stream()
+    with api.Batch(cfgs) as b:
+        eager = api.DeviceBuffer(b.pixels * 4)
+        b.tune(0, 0, use_graph=False)
+        b.render_into(eager.address, st.address)
+        st.synchronize()
+        ref = eager.numpy()
+        out = api.DeviceBuffer(b.pixels * 4)
+        b.tune(0, 0, use_graph=True)
+        for rep in range(4):
+            out.fill(float(rep + 1), st)       # eager work between replays
+            st.synchronize()
+            b.render_into(out.address, st.address)
+            st.synchronize()
+            assert np.array_equal(out.numpy(), ref), f"replay {rep}"
+
+
+def test_batch_scene_parity_against_fp64_oracle():
+    cfgs = _small_batch(n=6, seed=5)
+    imgs, _ = api.render_batch(cfgs)
+    for i in (0, 1, 2):
+        ref = bind.render(cfgs[i], 64)
+        twin = bind.render(cfgs[i], 32)
+        rmse = float(np.sqrt(np.mean((imgs[i][..., :3] - ref[..., :3]) ** 2)))
+        rmse_twin = float(np.sqrt(np.mean((twin[..., :3] - ref[..., :3]) ** 2)))
+        print(i, "rmse gpu %.3e twin %.3e" % (rmse, rmse_twin))
+        assert rmse <= 1.25 * rmse_twin + 2e-3
+
+
+def test_autotune_picks_a_valid_shape_and_keeps_the_image():
+    cfgs = _small_batch(n=6, seed=6)
+    st = api.Stream()
+    with api.Batch(cfgs) as b:
+        buf = api.DeviceBuffer(b.pixels * 4)
+        b.render_into(buf.address, st.address)
+        st.synchronize()
+        before = buf.numpy()
+        r = b.autotune(buf.address, st.address, rounds=1)
+        assert r.candidates == 20 and r.best_ms > 0 and 8 <= r.event_batch <= 40 and r.workgroups > 0
+        b.render_into(buf.address, st.address)
+        st.synchronize()
+        assert np.array_equal(buf.numpy(), before)

@@ -105,10 +105,19 @@ def test_program_gradient_matches_fp64():
     for cfg in (scenes.mandelbulb_scene(8, 8), scenes.typed_program_scene(8, 8)):
         got = api.program_surface(cfg, pts)
         want = bind.program_surface(cfg, pts, 64)
-        rel = np.linalg.norm(got[:, 1:] - want[:, 1:], axis=1) / np.linalg.norm(want[:, 1:], axis=1).clip(1e-9)
-        print("gradient rel err median %.2e p99 %.2e" % (np.median(rel), np.percentile(rel, 99)))
-        # fold planes and CSG seams make a few points pick the other branch
-        assert np.mean(rel <= 1e-4) >= 0.98 and np.median(rel) <= 1e-5
+        gn = np.linalg.norm(want[:, 1:], axis=1).clip(1e-9)
+        rel = np.linalg.norm(got[:, 1:] - want[:, 1:], axis=1) / gn
+        # one-ulp sensitivity of the FP64 gradient (fold planes, CSG seams and
+        # non-escaping orbits make it large at some points)
+        sens = np.zeros(len(pts))
+        for ax in range(3):
+            q = pts.copy()
+            q[:, ax] += np.abs(pts).max(axis=1) * 2.0 ** -23
+            sens = np.maximum(sens, np.linalg.norm(bind.program_surface(cfg, q, 64)[:, 1:] - want[:, 1:], axis=1) / gn)
+        ok = rel <= np.maximum(1e-4, 4.0 * sens)
+        print("gradient rel err median %.2e p99 %.2e within conditioning %.4f" % (
+            np.median(rel), np.percentile(rel, 99), np.mean(ok)))
+        assert np.mean(ok) >= 0.985 and np.median(rel) <= 1e-5
 
 
 @pytest.mark.parametrize("case", ["mandelbulb_p8", "typed_program", "exact_csg"])

@@ -249,3 +249,19 @@ def test_counters_are_consistent():
     assert k["paths"] == 256
     assert k["distance_evals"] >= k["primary_steps"] + k["secondary_steps"] + k["shadow_steps"]
     assert k["orbit_iterations"] >= k["distance_evals"]
+
+
+def test_batch_scene_generator_is_seeded_and_in_range():
+    a = scenes.batch_scenes(count=64, seed=0)
+    b = scenes.batch_scenes(count=64, seed=0)
+    c = scenes.batch_scenes(count=64, seed=1)
+    assert bytes(a[5]) == bytes(b[5]) and bytes(a[5]) != bytes(c[5])
+    for i, s in enumerate(a):
+        assert 64 <= s.width <= 512 and 64 <= s.height <= 512 and 1 <= s.samples <= 16
+        op = s.sdf_program[s.sdf_program_count - 1]
+        if i % 3 == 0:
+            assert op.opcode == ffi.OP_MANDELBULB and 2.0 <= op.data[0] <= 10.0
+        elif i % 3 == 1:
+            assert op.opcode == ffi.OP_MANDELBOX and -2.5 <= op.data[0] <= 2.5
+        else:
+            assert op.opcode == ffi.OP_MENGER and 3 <= op.data[0] <= 6
