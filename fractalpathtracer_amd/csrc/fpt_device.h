@@ -141,6 +141,22 @@ __device__ __forceinline__ Df s_min(Df x, Df y) { return y.v < x.v ? y : x; }
 __device__ __forceinline__ float s_scale(float x, float k) { return x * k; }
 __device__ __forceinline__ Df s_scale(Df x, float k) { return x * k; }
 
+// Native single-instruction reciprocal / square root (v_rcp_f32, v_sqrt_f32,
+// about 1 ulp) for the orbit loop, where the correctly rounded sequences would
+// cost ~10 instructions each.
+__device__ __forceinline__ float f_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ Df f_rcp(Df x) {
+    float r = __builtin_amdgcn_rcpf(x.v);
+    float k = -r * r;
+    return {r, x.a * k, x.b * k, x.c * k};
+}
+__device__ __forceinline__ float f_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ Df f_sqrt(Df x) {
+    float r = __builtin_amdgcn_sqrtf(x.v);
+    float k = 0.5f * __builtin_amdgcn_rcpf(r);
+    return {r, x.a * k, x.b * k, x.c * k};
+}
+
 template <class S>
 struct v3 {
     S x, y, z;
@@ -239,7 +255,7 @@ struct Leaf {
 template <class S>
 __device__ __forceinline__ S orbit_radius(v3<S> z) {
     S r2 = dot3(z);
-    if (__builtin_expect(isfinite(val(r2)), 1)) return s_sqrt(r2);
+    if (__builtin_expect(isfinite(val(r2)), 1)) return f_sqrt(r2);
     float m = fmaxf(fabsf(val(z.x)), fmaxf(fabsf(val(z.y)), fabsf(val(z.z))));
     if (!isfinite(m)) return r2;
     float im = 1.0f / m;
@@ -300,16 +316,51 @@ template <class S> __device__ __forceinline__ S sconst(float x);
 template <> __device__ __forceinline__ float sconst<float>(float x) { return x; }
 template <> __device__ __forceinline__ Df sconst<Df>(float x) { return dconst(x); }
 
-// Mandelbulb leaf: orbit loop (ORB:530-585) with the triplex kernel
-// (ORB:766-778), +c (mandelbulber.rs:1421-1440) and the Logarithmic
-// finalizer (ORB:654-660).
-template <class S>
-__device__ Leaf<S> leaf_mandelbulb(v3<S> p, const float *data, int *orbit_counter) {
-    const float power = data[0];
-    const int iters = int(data[1]);
-    const float bailout = data[2];
-    const int ip = int(power);
-    const bool integral = float(ip) == power && ip >= 1 && ip <= 32;
+// One triplex power step z <- r^p (cos t cos f, cos t sin f, sin t),
+// dr <- p r^(p-1) dr + 1 (ORB:766-778).  P > 0: the integer power is a
+// compile-time constant (straight-line complex squarings); P == 0 with
+// ip > 0: runtime integer power; ip == 0: trigonometric form.
+template <class S, int P>
+__device__ __forceinline__ void bulb_step(v3<S> &z, S r, S &dr, float power, int ip) {
+    if (P > 0 || ip > 0) {
+        const int n = P > 0 ? P : ip;
+        S rho2 = z.x * z.x + z.y * z.y;
+        S rho = f_sqrt(rho2);
+        S ir = f_rcp(r);
+        S st = z.z * ir;
+        S ct = rho * ir;
+        S cp, sp;
+        if (val(rho) > 0.0f) {
+            S irho = f_rcp(rho);
+            cp = z.x * irho;
+            sp = z.y * irho;
+        } else {  // atan2(0, +-0) = 0 or pi: cos = sign(x) (x = +0 -> 1)
+            cp = sconst<S>(signbit(val(z.x)) ? -1.0f : 1.0f);
+            sp = sconst<S>(0.0f);
+        }
+        S cpt, spt, cpp, spp;
+        cpow_int(ct, st, n, cpt, spt);
+        cpow_int(cp, sp, n, cpp, spp);
+        S rp = ipow_int(r, n - 1);
+        dr = rp * dr * power + sconst<S>(1.0f);
+        rp = rp * r;
+        S k = cpt * rp;
+        z = {k * cpp, k * spp, spt * rp};
+    } else {
+        S u = z.z / r;
+        S theta = s_asin(u) * power;
+        S phi = s_atan2(z.y, z.x) * power;
+        S rp = s_pow(r, power - 1.0f);
+        dr = rp * dr * power + sconst<S>(1.0f);
+        rp = rp * r;
+        S ctt = s_cos(theta);
+        z = {ctt * s_cos(phi) * rp, ctt * s_sin(phi) * rp, s_sin(theta) * rp};
+    }
+}
+
+template <class S, int P>
+__device__ __forceinline__ Leaf<S> bulb_orbit(v3<S> p, float power, int ip, int iters, float bailout,
+                                              int *orbit_counter) {
     v3<S> z = p;
     S r = orbit_radius(z);
     S dr = sconst<S>(1.0f);
@@ -317,39 +368,7 @@ __device__ Leaf<S> leaf_mandelbulb(v3<S> p, const float *data, int *orbit_counte
     int done = 0;
     for (int i = 0; i < iters; ++i) {
         v3<S> old = z;
-        if (integral) {
-            S rho2 = z.x * z.x + z.y * z.y;
-            S rho = s_sqrt(rho2);
-            S ir = s_rcp(r);
-            S st = z.z * ir;
-            S ct = rho * ir;
-            S cp, sp;
-            if (val(rho) > 0.0f) {
-                S irho = s_rcp(rho);
-                cp = z.x * irho;
-                sp = z.y * irho;
-            } else {  // atan2(0, +-0) = 0 or pi: cos = sign(x) (x = +0 -> 1)
-                cp = sconst<S>(signbit(val(z.x)) ? -1.0f : 1.0f);
-                sp = sconst<S>(0.0f);
-            }
-            S cpt, spt, cpp, spp;
-            cpow_int(ct, st, ip, cpt, spt);
-            cpow_int(cp, sp, ip, cpp, spp);
-            S rp = ipow_int(r, ip - 1);
-            dr = rp * dr * power + sconst<S>(1.0f);
-            rp = rp * r;
-            S k = cpt * rp;
-            z = {k * cpp, k * spp, spt * rp};
-        } else {
-            S u = z.z / r;
-            S theta = s_asin(u) * power;
-            S phi = s_atan2(z.y, z.x) * power;
-            S rp = s_pow(r, power - 1.0f);
-            dr = rp * dr * power + sconst<S>(1.0f);
-            rp = rp * r;
-            S ctt = s_cos(theta);
-            z = {ctt * s_cos(phi) * rp, ctt * s_sin(phi) * rp, s_sin(theta) * rp};
-        }
+        bulb_step<S, P>(z, r, dr, power, ip);
         z = add(z, p);
         r = orbit_radius(z);
         done = i + 1;
@@ -371,6 +390,21 @@ __device__ Leaf<S> leaf_mandelbulb(v3<S> p, const float *data, int *orbit_counte
     out.trap = trap;
     out.iterations = done;
     return out;
+}
+
+// Mandelbulb leaf: orbit loop (ORB:530-585) with the triplex kernel
+// (ORB:766-778), +c (mandelbulber.rs:1421-1440) and the Logarithmic
+// finalizer (ORB:654-660).  Power 8 (the common case) gets a specialised
+// loop; the dispatch is on a config value, so it is wave-uniform.
+template <class S>
+__device__ Leaf<S> leaf_mandelbulb(v3<S> p, const float *data, int *orbit_counter) {
+    const float power = data[0];
+    const int iters = int(data[1]);
+    const float bailout = data[2];
+    const int ipw = int(power);
+    const int ip = (float(ipw) == power && ipw >= 1 && ipw <= 32) ? ipw : 0;
+    if (ip == 8) return bulb_orbit<S, 8>(p, power, 8, iters, bailout, orbit_counter);
+    return bulb_orbit<S, 0>(p, power, ip, iters, bailout, orbit_counter);
 }
 
 // Mandelbox leaf: ORB:1636-1687 (fold limit 1, value 2, no rotations,

@@ -138,10 +138,10 @@ def render(cfg, precision: int = 64, threads: int = 0, rows=None) -> np.ndarray:
 
 
 def counters() -> dict:
-    a = (C.c_uint64 * 7)()
+    a = (C.c_uint64 * 9)()
     lib().orc_last_counters(a)
     keys = ("distance_evals", "orbit_iterations", "primary_steps", "secondary_steps",
-            "shadow_steps", "normal_evals", "paths")
+            "shadow_steps", "normal_evals", "paths", "far_steps", "far_orbit_iterations")
     return dict(zip(keys, list(a)))
 
 

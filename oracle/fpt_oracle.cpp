@@ -48,7 +48,12 @@ constexpr float kPiF = 3.14159265359f;
 struct Counters {
     uint64_t distance_evals = 0, orbit_iterations = 0, primary_steps = 0,
              secondary_steps = 0, shadow_steps = 0, normal_evals = 0, paths = 0;
+    // march steps whose distance exceeded 0.02 (far field) and their orbit
+    // iterations: the work a far-field step proposer could replace
+    uint64_t far_steps = 0, far_orbit_iterations = 0;
     void add(const Counters &o) {
+        far_steps += o.far_steps;
+        far_orbit_iterations += o.far_orbit_iterations;
         distance_evals += o.distance_evals;
         orbit_iterations += o.orbit_iterations;
         primary_steps += o.primary_steps;
@@ -892,8 +897,13 @@ V3<T> march(V3<T> dr, V3<T> rp, int ni, T min_dist, T lod_falloff, const FptRend
     bool check_t = has_translucent<T>(cfg);
     int steps = 0;
     for (int i = 0; i < max_iter; ++i) {
+        uint64_t it0 = tl_counters ? tl_counters->orbit_iterations : 0;
         T d = map_sdf(rp, cfg);
         ++steps;
+        if (tl_counters && std::fabs(d) > T(0.02)) {
+            tl_counters->far_steps++;
+            tl_counters->far_orbit_iterations += tl_counters->orbit_iterations - it0;
+        }
         if (!std::isfinite(d)) break;
         T thr = min_dist;
         T o = std::fabs(d) * T(0.99f);
@@ -1277,7 +1287,9 @@ void orc_mandelbox_step(double s[5], double scale, double min_r2, double fixed_r
 uint32_t orc_hash13_bits(float x, float y, float z) { return hash13bits(x, y, z); }
 float orc_hash13(float x, float y, float z) { return hash13(x, y, z); }
 
-void orc_last_counters(uint64_t out[7]) {
+void orc_last_counters(uint64_t out[9]) {
+    out[7] = g_last.far_steps;
+    out[8] = g_last.far_orbit_iterations;
     out[0] = g_last.distance_evals;
     out[1] = g_last.orbit_iterations;
     out[2] = g_last.primary_steps;

@@ -72,8 +72,9 @@ float orc_hash13(float x, float y, float z);
 
 /* Work counters of one orc_render call (summed over threads):
  * {distance_evals, orbit_iterations, primary_steps, secondary_steps,
- *  shadow_steps, normal_evals, paths}. */
-void orc_last_counters(uint64_t out[7]);
+ *  shadow_steps, normal_evals, paths, far_steps, far_orbit_iterations};
+ * far = march steps whose |distance| exceeded 0.02. */
+void orc_last_counters(uint64_t out[9]);
 
 #ifdef __cplusplus
 }
