@@ -1,17 +1,35 @@
-"""Throughput benchmark: path samples/s of the HIP path tracer on MI355X.
+"""Throughput benchmark: path samples/s of the HIP path tracer on MI355X,
+exact-distance march vs learned-step march at equal image RMSE.
 
 Workload (learned-step scene at full size): Mandelbulb power 8, 12 DE
 iterations, bailout 2, 1920x1080, 64 spp, 4 bounces, one directional light,
 grey albedo 0.7.  One step = one full frame (64 spp) rendered from the
-device-resident plan into a device accumulation buffer; inputs are resident
-in HBM before timing starts.
+device-resident plan into a device accumulation buffer; inputs (config,
+proposer weights) are resident in HBM before timing starts.
 
-Also reported on the same JSON line:
-  roofline      algorithmic FLOPs of the dominant kernel per launch (from an
-                untimed counting pass of the same frame) / its average launch
-                time measured with HIP events on the launch stream;
-  cpu_baseline  the FP64 CPU oracle on the host cores, on a bounded band of
-                rows of the same frame.
+Both march methods are timed on the same plan:
+  exact    every march step on the exact distance estimate;
+  learned  far-field steps (|q| > 0.02 inside the [-1.5, 1.5]^3 training box)
+           at 0.5 x the learned proposal q (fractalpathtracer_amd/assets/
+           proposer_mandelbulb_p8.npz), exact near the surface.
+Image quality of each method is measured on the linear accumulation buffer
+over a band of rows against the FP64 CPU oracle twice:
+  same samples    the oracle's samples 0..spp-1 (identical RNG keys): parity,
+                  how closely the GPU path follows the reference's paths;
+  converged       the oracle's samples spp..spp+ref_spp-1 (independent RNG
+                  keys, ref_spp = 4 x spp): RMSE against an independent
+                  higher-spp estimate, i.e. Monte-Carlo error + bias.
+`value` is the throughput of the faster method whose converged RMSE is
+within 10% of the exact method's (equal image RMSE); both methods' figures
+are on the line.
+
+Also on the JSON line:
+  roofline      algorithmic FLOPs of the dominant kernel (the persistent
+                path-tracing kernel of the reported method) per launch, from
+                an untimed counting pass of the same frame, / its average
+                launch time measured with HIP events on the launch stream;
+  cpu_baseline  the FP64 CPU oracle on the host cores, timed on the converged
+                reference band (same scene, ref_spp samples per pixel).
 """
 from __future__ import annotations
 
@@ -21,6 +39,8 @@ import os
 import sys
 import time
 from pathlib import Path
+
+import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
@@ -39,8 +59,12 @@ FLOPS_PER_DISTANCE_EVAL = 17
 #  Analytic normal = forward-mode gradient: 4 lanes of value+3 tangents,
 #  counted as 4 distance evaluations of the same orbit.
 NORMAL_DE_FACTOR = 4
+#  Proposer network per point: 2 x (128*64 + 3*64*64 + 64) multiply-adds.
+FLOPS_PER_PROPOSAL = 2 * (128 * 64 + 3 * 64 * 64 + 64)
 
-FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 peak
+FP32_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
+BF16_PEAK_TFLOPS = 2500.0  # MI355X published dense BF16 matrix peak
+KERNEL = "sdf_persistent_kernel"
 
 
 def parse():
@@ -52,22 +76,48 @@ def parse():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-rows", type=int, default=4)
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--cpu-rows", type=int, default=8)
+    ap.add_argument("--ref-spp-factor", type=int, default=4)
     return ap.parse_args()
 
 
-def cpu_baseline(cfg, rows: int) -> dict:
-    """FP64 oracle on the host cores over `rows` rows at the frame centre."""
+def oracle_band(cfg, rows: int, ref_factor: int):
+    """FP64 oracle over `rows` rows at the frame centre: the same-sample band
+    and the converged band (timed: the CPU baseline)."""
     from oracle import bind
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     y0 = cfg.height // 2 - rows // 2
+    y1 = y0 + rows
+    same = bind.render(cfg, 64, threads=threads, rows=(y0, y1))[y0:y1]
+    s0, s1 = cfg.samples, cfg.samples * (1 + ref_factor)
     t = time.perf_counter()
-    bind.render(cfg, 64, threads=threads, rows=(y0, y0 + rows))
+    conv = bind.render_samples(cfg, (s0, s1), rows=(y0, y1), threads=threads)[y0:y1]
     dt = time.perf_counter() - t
-    samples = rows * cfg.width * cfg.samples
-    return {"value": samples / dt, "unit": "path samples/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 C++ oracle, rows {y0}-{y0 + rows - 1} of the {cfg.width}x{cfg.height} frame at "
-                      f"{cfg.samples} spp ({samples} path samples, {dt:.1f} s)"}
+    samples = rows * cfg.width * (s1 - s0)
+    base = {"value": samples / dt, "unit": "path samples/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 C++ oracle (OpenMP), rows {y0}-{y1 - 1} of the {cfg.width}x{cfg.height} frame, "
+                      f"samples {s0}-{s1 - 1} per pixel ({samples} path samples, {dt:.1f} s)"}
+    return base, same, conv, (y0, y1)
+
+
+def frame_flops(prof: dict) -> float:
+    iters = prof.get("march_orbit_iterations", 0)
+    de = prof.get("distance_evals", 0)
+    normals = prof.get("normal_evals", 0)
+    avg_iter = iters / max(de, 1)
+    return (iters * FLOPS_PER_ORBIT_ITERATION + de * FLOPS_PER_DISTANCE_EVAL +
+            normals * NORMAL_DE_FACTOR * (avg_iter * FLOPS_PER_ORBIT_ITERATION + FLOPS_PER_DISTANCE_EVAL))
+
+
+def run_method(api, plan, cfg, stream, accum, steps, warmup):
+    """Timed frames, then an untimed counting pass and the final image."""
+    if warmup:
+        api.time_renders(plan, accum, stream, warmup)
+    total_ms = api.time_renders(plan, accum, stream, steps)
+    img = accum.numpy().reshape(cfg.height, cfg.width, 4)
+    prof, learned = plan.profile(accum)
+    return total_ms / steps, img, prof, learned
 
 
 def main():
@@ -76,66 +126,91 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU split of this path "
               "is a design note, not a build target", file=sys.stderr)
         sys.exit(2)
-    from fractalpathtracer_amd import api, scenes
+    from fractalpathtracer_amd import api, proposer, scenes
 
     name = api.device_name()
     assert "gfx" in name, f"bench.py needs a HIP GPU ({name})"
     cfg = scenes.learned_step_scene(args.width, args.height, args.spp)
+    samples_per_step = cfg.width * cfg.height * cfg.samples
+    chunk = 8 if cfg.samples > 16 else cfg.samples   # samples per launch (chunked policy)
+    launches = (cfg.samples + chunk - 1) // chunk
 
-    # Untimed counting pass of the same frame: work counters for the roofline.
-    prof = api.render_full(cfg, profile=True).profile
-
-    # Timed region: HIP events on the launch stream around `steps` frames,
-    # inputs (config) resident in HBM, output in a device buffer.
     stream = api.Stream()
     accum = api.DeviceBuffer(cfg.width * cfg.height * 4)
     plan = api.Plan(cfg)
-    if args.warmup:
-        api.time_renders(plan, accum, stream, args.warmup)
-    total_ms = api.time_renders(plan, accum, stream, args.steps)
+    methods = {}
+    ms, img, prof, _ = run_method(api, plan, cfg, stream, accum, args.steps, args.warmup)
+    methods["exact"] = {"ms_per_step": ms, "value": samples_per_step / (ms / 1e3), "img": img, "prof": prof}
+    if not args.no_learned:
+        params, meta = proposer.load(proposer.asset_path("mandelbulb_p8"))
+        plan.set_proposer(params, box_half=meta.get("box_half", proposer.BOX_HALF),
+                          switch_distance=proposer.SWITCH_DISTANCE, resume_distance=proposer.RESUME_DISTANCE,
+                          step_scale=proposer.STEP_SCALE)
+        ms, img, prof, learned = run_method(api, plan, cfg, stream, accum, args.steps, args.warmup)
+        plan.set_proposer(None)
+        methods["learned"] = {"ms_per_step": ms, "value": samples_per_step / (ms / 1e3), "img": img,
+                              "prof": prof, "learned": learned, "train": meta}
     plan.close()
-    ms_per_step = total_ms / args.steps
-    samples_per_step = cfg.width * cfg.height * cfg.samples
-    value = samples_per_step / (ms_per_step / 1e3)
 
-    # One launch renders `chunk` samples of the frame (8 at 64 spp).
-    chunk = 8 if cfg.samples > 16 else cfg.samples
-    launches = (cfg.samples + chunk - 1) // chunk
-    launch_ms = ms_per_step / launches
-    iters = prof.get("march_orbit_iterations", 0)
-    de = prof.get("distance_evals", 0)
-    normals = prof.get("normal_evals", 0)
-    avg_iter = iters / max(de, 1)
-    flops_frame = (iters * FLOPS_PER_ORBIT_ITERATION + de * FLOPS_PER_DISTANCE_EVAL +
-                   normals * NORMAL_DE_FACTOR * (avg_iter * FLOPS_PER_ORBIT_ITERATION + FLOPS_PER_DISTANCE_EVAL))
-    flops_launch = flops_frame / launches
+    base = None
+    gate = None
+    chosen = "exact"
+    if not args.no_cpu_baseline:
+        base, same, conv, (y0, y1) = oracle_band(cfg, args.cpu_rows, args.ref_spp_factor)
+        for m in methods.values():
+            band = m["img"][y0:y1, :, :3]
+            m["rmse_same"] = float(np.sqrt(np.mean((band - same[..., :3]) ** 2)))
+            m["rmse"] = float(np.sqrt(np.mean((band - conv[..., :3]) ** 2)))
+            m["mean_bias"] = float(np.mean(band) - np.mean(conv[..., :3]))
+        gate = 1.10 * methods["exact"]["rmse"]
+        ok = [k for k, m in methods.items() if m["rmse"] <= gate]
+        chosen = max(ok, key=lambda k: methods[k]["value"])
+
+    m = methods[chosen]
+    launch_ms = m["ms_per_step"] / launches
+    flops_launch = frame_flops(m["prof"]) / launches
     achieved = flops_launch / (launch_ms / 1e3) / 1e12
+    summary = {}
+    for k, mm in methods.items():
+        s = {"value": mm["value"], "ms_per_step": mm["ms_per_step"],
+             "rmse_vs_converged_fp64": mm.get("rmse"), "rmse_vs_fp64_same_samples": mm.get("rmse_same"),
+             "mean_minus_converged": mm.get("mean_bias"),
+             "distance_evals": mm["prof"]["distance_evals"],
+             "orbit_iterations": mm["prof"]["march_orbit_iterations"]}
+        if k == "learned":
+            L = mm["learned"]
+            mlp_flops = L["proposer_calls"] * 64 * FLOPS_PER_PROPOSAL
+            s.update(L)
+            s["mfma_tflops"] = mlp_flops / (mm["ms_per_step"] / 1e3) / 1e12
+            s["mfma_frac_of_bf16_peak"] = s["mfma_tflops"] / BF16_PEAK_TFLOPS
+            s["speedup_vs_exact"] = mm["value"] / methods["exact"]["value"]
+            s["training"] = {k2: mm["train"].get(k2) for k2 in
+                             ("samples", "steps", "sample_seconds", "train_seconds", "mae_far")}
+        summary[k] = s
     line = {
         "metric": METRIC,
-        "value": value,
+        "value": m["value"],
         "unit": "path samples/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": m["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"exact-DE march, Mandelbulb p8/12it/bailout2, {cfg.width}x{cfg.height}, "
-                               f"{cfg.samples} spp, 4 bounces", "method": "exact-DE",
+        "config": {"workload": f"Mandelbulb p8/12it/bailout2, {cfg.width}x{cfg.height}, {cfg.samples} spp, "
+                               f"4 bounces", "method": chosen, "rmse_gate": gate,
                    "width": cfg.width, "height": cfg.height, "spp": cfg.samples, "bounces": 4,
-                   "learned_step": None},
+                   "methods": summary},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "sdf_accumulate_kernel", "launch_ms": launch_ms,
-                     "flops_per_launch": flops_launch,
-                     "counts_per_frame": {"orbit_iterations": iters, "distance_evals": de,
-                                          "normal_evals": normals}},
+                     "kernel": KERNEL, "launches_per_step": launches, "launch_ms": launch_ms,
+                     "flops_per_launch": flops_launch},
     }
-    if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(cfg, args.cpu_rows)
+    if base is not None:
+        line["cpu_baseline"] = base
     print(json.dumps(line), flush=True)
 
 

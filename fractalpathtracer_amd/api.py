@@ -148,6 +148,35 @@ def test_voxel_dda() -> None:
     ffi.check(lib.fpt_test_voxel_dda(None, err, len(err)), err)
 
 
+def proposer_samples(cfg: FptRenderConfig, count: int, seed: int = 0, box_half: float = 1.5,
+                     band: float = 0.05, max_attempts: int = 1 << 16) -> tuple[np.ndarray, int]:
+    """Training samples (count, 4) = (x, y, z, exact d) and the number of
+    band points that found no candidate (fpt_hip_proposer_samples)."""
+    lib = ffi.load_library()
+    out = np.zeros((int(count), 4), np.float32)
+    fails = C.c_uint32(0)
+    err = ffi.error_buffer()
+    ffi.check(lib.fpt_hip_proposer_samples(C.byref(cfg), box_half, band, int(count), int(seed), int(max_attempts),
+                                           out.ctypes.data_as(C.POINTER(C.c_float)), C.byref(fails), err,
+                                           len(err)), err)
+    return out, int(fails.value)
+
+
+def proposer_eval(params, points, box_half: float = 1.5) -> np.ndarray:
+    """Proposer network on the GPU at (n, 3) points (fpt_hip_proposer_eval)."""
+    lib = ffi.load_library()
+    pts = np.asarray(points, np.float32).reshape(-1, 3)
+    p4 = np.zeros((pts.shape[0], 4), np.float32)
+    p4[:, :3] = pts
+    w = np.ascontiguousarray(params, np.float32).ravel()
+    out = np.zeros(pts.shape[0], np.float32)
+    err = ffi.error_buffer()
+    ffi.check(lib.fpt_hip_proposer_eval(w.ctypes.data_as(C.POINTER(C.c_float)), w.size, box_half,
+                                        p4.ctypes.data_as(C.POINTER(C.c_float)), pts.shape[0],
+                                        out.ctypes.data_as(C.POINTER(C.c_float)), err, len(err)), err)
+    return out
+
+
 def device_name() -> str:
     lib = ffi.load_library()
     buf = C.create_string_buffer(256)
@@ -259,6 +288,30 @@ class Plan:
         err = ffi.error_buffer()
         ffi.check(self.lib.fpt_hip_plan_render(self.handle, C.c_void_p(device_ptr), C.c_void_p(stream_ptr),
                                                err, len(err)), err)
+
+    def set_proposer(self, params, box_half: float = 1.5, switch_distance: float = 0.02,
+                     resume_distance: float = 0.1, step_scale: float = 0.5) -> None:
+        """Attach a learned step proposer (params: FPT_PROPOSER_PARAM_COUNT
+        floats), or detach it with params=None."""
+        err = ffi.error_buffer()
+        if params is None:
+            ffi.check(self.lib.fpt_hip_plan_set_proposer(self.handle, None, 0, None, err, len(err)), err)
+            return
+        w = np.ascontiguousarray(params, np.float32).ravel()
+        st = ffi.FptProposerSettings(box_half, switch_distance, resume_distance, step_scale)
+        ffi.check(self.lib.fpt_hip_plan_set_proposer(self.handle, w.ctypes.data_as(C.POINTER(C.c_float)), w.size,
+                                                     C.byref(st), err, len(err)), err)
+
+    def profile(self, buffer: "DeviceBuffer") -> tuple[dict, dict]:
+        """Counting pass of the full frame: (profile counters, learned-step
+        counters)."""
+        prof = ffi.FptSdfProfileStats()
+        lst = ffi.FptProposerStats()
+        err = ffi.error_buffer()
+        ffi.check(self.lib.fpt_hip_plan_profile(self.handle, buffer.ptr, C.byref(prof), C.byref(lst), err,
+                                                len(err)), err)
+        return ({k: getattr(prof, k) for k, _ in ffi.FptSdfProfileStats._fields_[:13]},
+                {k: getattr(lst, k) for k, _ in ffi.FptProposerStats._fields_})
 
     def close(self) -> None:
         if self.handle:

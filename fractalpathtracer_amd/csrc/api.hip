@@ -18,6 +18,7 @@
 
 #include "../../include/fpt_abi.h"
 #include "fpt_internal.h"
+#include "learned.h"
 #include "voxel_api.h"
 
 namespace {
@@ -193,6 +194,7 @@ struct FptHipPlan {
     uint32_t workgroups = 0;     // persistent grid; 0 = 2 per compute unit
     uint32_t event_batch = 24;   // waiting lanes before per-hit work runs
     uint32_t *counter = nullptr; // persistent pixel counter
+    fpt::ProposerHost proposer;  // learned stepping (fpt_hip_plan_set_proposer)
 };
 
 namespace {
@@ -212,7 +214,7 @@ hipError_t launch_sdf(FptHipPlan *plan, float4 *acc, uint32_t start, uint32_t ch
     }
     uint32_t wg = plan->workgroups ? plan->workgroups : default_workgroups();
     return fpt::launch_sdf_persistent(plan->dcfg, c, acc, start, chunk, plan->counter, wg, int(plan->event_batch),
-                                      prof, s);
+                                      prof, plan->proposer.enabled ? &plan->proposer.dev : nullptr, s);
 }
 }  // namespace
 
@@ -280,6 +282,10 @@ int fpt_hip_plan_tune(FptHipPlan *plan, uint32_t kernel, uint32_t workgroups, ui
                   workgroups, event_batch);
         return 1;
     }
+    if (kernel == 0u && plan->proposer.enabled) {
+        set_error(error, error_len, "learned stepping runs on the persistent kernel (kernel 1)");
+        return 1;
+    }
     plan->kernel = kernel;
     plan->workgroups = workgroups;
     plan->event_batch = event_batch ? event_batch : 24u;
@@ -291,7 +297,140 @@ void fpt_hip_plan_destroy(FptHipPlan *plan) {
     if (plan->dcfg) (void)hipFree(plan->dcfg);
     if (plan->counter) (void)hipFree(plan->counter);
     fpt::voxel_free(plan->voxels);
+    fpt::proposer_release(plan->proposer);
     delete plan;
+}
+
+int fpt_hip_plan_set_proposer(FptHipPlan *plan, const float *params, size_t param_count,
+                              const FptProposerSettings *settings, char *error, size_t error_len) {
+    if (!plan) {
+        set_error(error, error_len, "missing plan");
+        return 1;
+    }
+    if (!settings) {
+        fpt::proposer_release(plan->proposer);
+        return 0;
+    }
+    if (plan->host.renderer_backend != FPT_RENDERER_SDF) {
+        set_error(error, error_len, "learned stepping applies to the SDF backend");
+        return 1;
+    }
+    if (plan->kernel == 0u) {
+        set_error(error, error_len, "learned stepping runs on the persistent kernel (kernel 1)");
+        return 1;
+    }
+    if (const char *msg = fpt::proposer_upload(plan->proposer, params, param_count, settings->box_half,
+                                               settings->switch_distance, settings->resume_distance,
+                                               settings->step_scale)) {
+        set_error(error, error_len, "%s", msg);
+        return 1;
+    }
+    return 0;
+}
+
+int fpt_hip_plan_profile(FptHipPlan *plan, float *device_accum, FptSdfProfileStats *stats,
+                         FptProposerStats *learned, char *error, size_t error_len) {
+    if (!plan || !device_accum || !stats) {
+        set_error(error, error_len, "missing plan, accumulation buffer or stats");
+        return 1;
+    }
+    if (plan->host.renderer_backend != FPT_RENDERER_SDF) {
+        set_error(error, error_len, "the counting pass applies to the SDF backend");
+        return 1;
+    }
+    std::memset(stats, 0, sizeof(*stats));
+    if (learned) std::memset(learned, 0, sizeof(*learned));
+    Stream st;
+    HIP_TRY(hipStreamCreate(&st.s), "hipStreamCreate");
+    DevBuf prof;
+    HIP_TRY(prof.alloc(sizeof(fpt::DeviceProfile)), "profile buffer");
+    HIP_TRY(hipMemsetAsync(prof.p, 0, sizeof(fpt::DeviceProfile), st.s), "profile clear");
+    const uint32_t spp = clamp_samples(plan->host.samples), chunk = chunk_for(plan->host);
+    float4 *acc = reinterpret_cast<float4 *>(device_accum);
+    for (uint32_t s0 = 0; s0 < spp; s0 += chunk)
+        HIP_TRY(launch_sdf(plan, acc, s0, chunk, prof.as<fpt::DeviceProfile>(), st.s), "profile kernel launch");
+    fpt::DeviceProfile hp;
+    HIP_TRY(hipMemcpyAsync(&hp, prof.p, sizeof(hp), hipMemcpyDeviceToHost, st.s), "profile readback");
+    HIP_TRY(hipStreamSynchronize(st.s), "profile pass");
+    stats->primary_steps = hp.primary_steps;
+    stats->secondary_steps = hp.secondary_steps;
+    stats->shadow_steps = hp.shadow_steps;
+    stats->normal_evals = hp.normal_evals;
+    stats->bounces = hp.bounces;
+    stats->pixels = hp.pixels;
+    stats->distance_evals = hp.distance_evals;
+    stats->march_orbit_iterations = hp.orbit_iterations;
+    if (learned) {
+        learned->proposer_calls = hp.proposer_calls;
+        learned->proposals = hp.proposals;
+        learned->far_steps = hp.far_steps;
+        learned->backtracks = hp.backtracks;
+    }
+    return 0;
+}
+
+int fpt_hip_proposer_samples(const FptRenderConfig *config, float box_half, float band, uint32_t count,
+                             uint64_t seed, uint32_t max_attempts, float *out_xyzd, uint32_t *failures,
+                             char *error, size_t error_len) {
+    if (validate_config(config, error, error_len)) return 1;
+    if (!out_xyzd || !failures) {
+        set_error(error, error_len, "missing sample or failure-count buffer");
+        return 1;
+    }
+    if (!(box_half > 0.0f) || !(band > 0.0f) || max_attempts == 0u) {
+        set_error(error, error_len, "box_half and band must be positive and max_attempts non-zero");
+        return 1;
+    }
+    *failures = 0;
+    if (count == 0) return 0;
+    if (check_device(error, error_len)) return 1;
+    Stream st;
+    HIP_TRY(hipStreamCreate(&st.s), "hipStreamCreate");
+    DevBuf dcfg, dout, dfail;
+    HIP_TRY(dcfg.alloc(sizeof(FptRenderConfig)), "config allocation");
+    HIP_TRY(dout.alloc(size_t(count) * sizeof(float4)), "sample buffer allocation");
+    HIP_TRY(dfail.alloc(sizeof(uint32_t)), "counter allocation");
+    HIP_TRY(hipMemcpyAsync(dcfg.p, config, sizeof(FptRenderConfig), hipMemcpyHostToDevice, st.s), "config upload");
+    HIP_TRY(hipMemsetAsync(dfail.p, 0, sizeof(uint32_t), st.s), "counter clear");
+    HIP_TRY(fpt::launch_proposer_samples(dcfg.as<FptRenderConfig>(), box_half, band, count, seed, max_attempts,
+                                         dout.as<float4>(), dfail.as<uint32_t>(), st.s),
+            "sample kernel launch");
+    HIP_TRY(hipMemcpyAsync(out_xyzd, dout.p, size_t(count) * sizeof(float4), hipMemcpyDeviceToHost, st.s),
+            "sample readback");
+    HIP_TRY(hipMemcpyAsync(failures, dfail.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st.s), "counter readback");
+    HIP_TRY(hipStreamSynchronize(st.s), "sampling");
+    return 0;
+}
+
+int fpt_hip_proposer_eval(const float *params, size_t param_count, float box_half, const float *points_xyzw,
+                          size_t point_count, float *out, char *error, size_t error_len) {
+    if (!points_xyzw || !out) {
+        set_error(error, error_len, "missing point or output buffer");
+        return 1;
+    }
+    if (check_device(error, error_len)) return 1;
+    fpt::ProposerHost ph;
+    struct Release {
+        fpt::ProposerHost &h;
+        ~Release() { fpt::proposer_release(h); }
+    } release{ph};
+    if (const char *msg = fpt::proposer_upload(ph, params, param_count, box_half, 0.02f, 0.02f, 0.5f)) {
+        set_error(error, error_len, "%s", msg);
+        return 1;
+    }
+    if (point_count == 0) return 0;
+    Stream st;
+    HIP_TRY(hipStreamCreate(&st.s), "hipStreamCreate");
+    DevBuf dp, dout;
+    HIP_TRY(dp.alloc(point_count * sizeof(float4)), "point buffer allocation");
+    HIP_TRY(dout.alloc(point_count * sizeof(float)), "output allocation");
+    HIP_TRY(hipMemcpyAsync(dp.p, points_xyzw, point_count * sizeof(float4), hipMemcpyHostToDevice, st.s),
+            "point upload");
+    HIP_TRY(fpt::launch_proposer_eval(ph.dev, dp.as<float4>(), point_count, dout.as<float>(), st.s),
+            "proposer kernel launch");
+    HIP_TRY(hipMemcpyAsync(out, dout.p, point_count * sizeof(float), hipMemcpyDeviceToHost, st.s), "output readback");
+    HIP_TRY(hipStreamSynchronize(st.s), "proposer evaluation");
+    return 0;
 }
 
 }  // extern "C"
@@ -516,8 +655,9 @@ int fpt_hip_autotune(FptHipPlan *plan, FptHipBatch *batch, float *device_accum, 
     float4 *acc = reinterpret_cast<float4 *>(device_accum);
     if (plan && !plan->counter) HIP_TRY(hipMalloc(&plan->counter, sizeof(uint32_t)), "counter allocation");
     auto launch = [&](uint32_t wg, uint32_t eb) -> hipError_t {
-        if (plan) return fpt::launch_sdf_persistent(plan->dcfg, plan->host, acc, 0u, 1u, plan->counter, wg, int(eb),
-                                                    nullptr, s);
+        if (plan)
+            return fpt::launch_sdf_persistent(plan->dcfg, plan->host, acc, 0u, 1u, plan->counter, wg, int(eb), nullptr,
+                                              plan->proposer.enabled ? &plan->proposer.dev : nullptr, s);
         return fpt::launch_sdf_batch(batch->dcfgs, batch->dscenes, uint32_t(batch->host.size()), batch->total_slots, acc,
                                      batch->counter, wg, int(eb), 1u, nullptr, s);
     };

@@ -20,7 +20,15 @@ struct DeviceProfile {
     unsigned long long normal_evals;
     unsigned long long bounces;
     unsigned long long pixels;
+    // Learned stepping (zero otherwise): wave-collective proposer
+    // evaluations, lanes that used a proposal, far steps taken, backtracks.
+    unsigned long long proposer_calls;
+    unsigned long long proposals;
+    unsigned long long far_steps;
+    unsigned long long backtracks;
 };
+
+struct ProposerDevice;
 
 // One scene of a batched launch: its work slots are
 // [slot_begin, slot_begin + slots) of the global slot range (8x8 tiles of its
@@ -47,7 +55,8 @@ hipError_t launch_surface_sample(const FptRenderConfig *dcfg, const float4 *pts,
                                  hipStream_t stream);
 hipError_t launch_sdf_persistent(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, float4 *accum,
                                  uint32_t start, uint32_t count, uint32_t *counter, uint32_t workgroups,
-                                 int event_batch, DeviceProfile *prof, hipStream_t stream);
+                                 int event_batch, DeviceProfile *prof, const ProposerDevice *prop,
+                                 hipStream_t stream);
 hipError_t launch_present(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, const float4 *accum,
                           uchar4 *out, hipStream_t stream);
 

@@ -28,10 +28,22 @@
 // (wave-uniform scalar loads).  MULTI = true: every lane carries the pointer
 // to its scene's config (vector loads, cached), scenes may differ in size,
 // spp and program.
+//
+// LEARNED = true (single scene): march steps inside the proposer's training
+// box may use the learned distance proposal q (learned.h) instead of the exact
+// distance.  A march is in "far" mode from the camera, and again whenever an
+// exact step saw |d| > resume_distance; in far mode, while q >
+// switch_distance, the step is step_scale * q (no exact evaluation).  The
+// first proposal at or below switch_distance hands the march to the exact
+// distance; if that exact distance is negative the far step overshot into the
+// surface and the march backs up to the point before it.  Far steps have their
+// own budget (the march's step limit); when it runs out the march stays
+// exact.  Hits, normals, shading and termination tests are the exact ones.
 #include <hip/hip_runtime.h>
 
 #include "fpt_device.h"
 #include "fpt_internal.h"
+#include "learned.h"
 
 namespace fpt {
 
@@ -39,6 +51,10 @@ enum : int { PH_IDLE = 0, PH_MARCH = 1, PH_EVENT = 2 };
 
 struct PersistentCounters {
     int primary_steps, secondary_steps, shadow_steps, normal_evals, bounces;
+};
+
+struct LearnedCounters {
+    int proposer_calls, proposals, far_steps, backtracks;
 };
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
@@ -69,18 +85,19 @@ __host__ __device__ __forceinline__ uint32_t slot_count(uint32_t w, uint32_t h) 
     return ((w + 7u) / 8u) * ((h + 7u) / 8u) * 64u;
 }
 
-template <bool PROFILE, bool MULTI>
+template <bool PROFILE, bool MULTI, bool LEARNED>
 __global__ __launch_bounds__(256) void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
                                                              const BatchScene *__restrict__ scenes, uint32_t nscenes,
                                                              float4 *__restrict__ accum, uint32_t start,
                                                              uint32_t count, uint32_t *__restrict__ counter,
                                                              uint32_t total_slots, int event_batch,
-                                                             DeviceProfile *prof) {
+                                                             DeviceProfile *prof, ProposerDevice prop) {
     const FptRenderConfig *cp = cfgs;  // the lane's scene (fixed when !MULTI)
     float4 *out = accum;               // the lane's scene accumulation buffer
 
     Work w = {0, 0};
     PersistentCounters pc = {0, 0, 0, 0, 0};
+    LearnedCounters lc = {0, 0, 0, 0};
     int pixels_done = 0;
 
     // Per-scene constants, refreshed when a lane takes a pixel.
@@ -134,6 +151,10 @@ __global__ __launch_bounds__(256) void sdf_persistent_kernel(const FptRenderConf
     f3 p = mk(0.f, 0.f, 0.f), mdir = mk(0.f, 0.f, 1.f), mstart = mk(0.f, 0.f, 0.f);
     int it = 0, max_iter = 0;
     bool shadow = false;
+    // Learned stepping state (LEARNED only).
+    bool lfar = false, lastfar = false;
+    int fit = 0;
+    f3 psafe = mk(0.f, 0.f, 0.f);
     bool retry = false;  // slot was tile padding: take another
 
     // Start the camera path of sample `s` of the current pixel.
@@ -158,6 +179,9 @@ __global__ __launch_bounds__(256) void sdf_persistent_kernel(const FptRenderConf
         it = 0;
         max_iter = local_ni < 360 ? local_ni : 360;
         shadow = false;
+        lfar = LEARNED;
+        lastfar = false;
+        fit = 0;
         phase = PH_MARCH;
     };
 
@@ -209,20 +233,52 @@ __global__ __launch_bounds__(256) void sdf_persistent_kernel(const FptRenderConf
     while (__any(retry)) take_slot(retry);
 
     while (true) {
+        float q = 0.f;
+        bool use_q = false;
+        if (LEARNED) {
+            use_q = phase == PH_MARCH && lfar && proposer_in_box(prop, p);
+            if (__any(use_q)) {  // wave-uniform: the MLP needs every lane
+                q = proposer_eval(prop, p);
+                if (PROFILE) {
+                    lc.proposer_calls += lane_id() == 0u;
+                    lc.proposals += use_q;
+                }
+            }
+        }
         if (phase == PH_MARCH) {
             const FptRenderConfig &cfg = *cp;
-            const float d = map_sdf(p, cfg, w);
-            ++it;
-            bool stop = !isfinite(d);
-            if (!stop) {
-                const float o = fabsf(d) * 0.99f;
-                p = p + mdir * o;
-                const f3 df = mstart - p;  // lod_falloff 0.0002 (SM:3771-3776)
-                float lod = mixf(min_dist, mixf(min_dist, 0.1f, dot(df, df) * 0.0002f), adapt);
-                if (check_t && user_material(p, cfg, w).translucency > 0.0f) lod = 0.0001f;
-                stop = o < lod || maxd < o || it >= max_iter;
+            if (LEARNED && use_q && q > prop.switch_distance) {
+                // Far step on the proposal.
+                psafe = p;
+                p = p + mdir * (q * prop.step_scale);
+                lastfar = true;
+                if (++fit >= max_iter) lfar = false;
+                if (PROFILE) lc.far_steps++;
+            } else {
+                if (LEARNED && use_q) lfar = false;  // within switch_distance: exact from here
+                const float d = map_sdf(p, cfg, w);
+                ++it;
+                if (LEARNED && lastfar && use_q && d < 0.0f) {
+                    // The last far step crossed the surface: back up.
+                    p = psafe;
+                    lastfar = false;
+                    if (PROFILE) lc.backtracks++;
+                    if (it >= max_iter) phase = PH_EVENT;
+                } else {
+                    lastfar = false;
+                    bool stop = !isfinite(d);
+                    if (!stop) {
+                        const float o = fabsf(d) * 0.99f;
+                        p = p + mdir * o;
+                        const f3 df = mstart - p;  // lod_falloff 0.0002 (SM:3771-3776)
+                        float lod = mixf(min_dist, mixf(min_dist, 0.1f, dot(df, df) * 0.0002f), adapt);
+                        if (check_t && user_material(p, cfg, w).translucency > 0.0f) lod = 0.0001f;
+                        stop = o < lod || maxd < o || it >= max_iter;
+                        if (LEARNED && fabsf(d) > prop.resume_distance && fit < max_iter) lfar = true;
+                    }
+                    if (stop) phase = PH_EVENT;
+                }
             }
-            if (stop) phase = PH_EVENT;
         }
         const unsigned long long ev = __ballot(phase == PH_EVENT);
         const unsigned long long mar = __ballot(phase == PH_MARCH);
@@ -273,6 +329,9 @@ __global__ __launch_bounds__(256) void sdf_persistent_kernel(const FptRenderConf
                         it = 0;
                         max_iter = ni_first < 360 ? ni_first : 360;
                         shadow = true;
+                        lfar = false;
+                        lastfar = false;
+                        fit = 0;
                         phase = PH_MARCH;
                     } else {
                         do_scatter = true;
@@ -333,6 +392,9 @@ __global__ __launch_bounds__(256) void sdf_persistent_kernel(const FptRenderConf
                     mdir = dr;
                     it = 0;
                     max_iter = local_ni < 360 ? local_ni : 360;
+                    lfar = false;
+                    lastfar = false;
+                    fit = 0;
                     phase = PH_MARCH;
                 }
             }
@@ -369,6 +431,12 @@ __global__ __launch_bounds__(256) void sdf_persistent_kernel(const FptRenderConf
         wadd(&prof->normal_evals, (unsigned long long)pc.normal_evals);
         wadd(&prof->bounces, (unsigned long long)pc.bounces);
         wadd(&prof->pixels, (unsigned long long)pixels_done);
+        if (LEARNED) {
+            wadd(&prof->proposer_calls, (unsigned long long)lc.proposer_calls);
+            wadd(&prof->proposals, (unsigned long long)lc.proposals);
+            wadd(&prof->far_steps, (unsigned long long)lc.far_steps);
+            wadd(&prof->backtracks, (unsigned long long)lc.backtracks);
+        }
     }
 }
 
@@ -379,15 +447,26 @@ __global__ void zero_counter_kernel(uint32_t *c) {
 
 hipError_t launch_sdf_persistent(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, float4 *accum,
                                  uint32_t start, uint32_t count, uint32_t *counter, uint32_t workgroups,
-                                 int event_batch, DeviceProfile *prof, hipStream_t stream) {
+                                 int event_batch, DeviceProfile *prof, const ProposerDevice *prop,
+                                 hipStream_t stream) {
     const uint32_t slots = slot_count(hcfg.width, hcfg.height);
     hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, counter);
-    if (prof)
-        hipLaunchKernelGGL((sdf_persistent_kernel<true, false>), dim3(workgroups), dim3(256), 0, stream, dcfg, nullptr,
-                           1u, accum, start, count, counter, slots, event_batch, prof);
-    else
-        hipLaunchKernelGGL((sdf_persistent_kernel<false, false>), dim3(workgroups), dim3(256), 0, stream, dcfg, nullptr,
-                           1u, accum, start, count, counter, slots, event_batch, nullptr);
+    const ProposerDevice none{};
+    const dim3 grid(workgroups), block(256);
+    if (prop) {
+        if (prof)
+            hipLaunchKernelGGL((sdf_persistent_kernel<true, false, true>), grid, block, 0, stream, dcfg, nullptr, 1u,
+                               accum, start, count, counter, slots, event_batch, prof, *prop);
+        else
+            hipLaunchKernelGGL((sdf_persistent_kernel<false, false, true>), grid, block, 0, stream, dcfg, nullptr, 1u,
+                               accum, start, count, counter, slots, event_batch, nullptr, *prop);
+    } else if (prof) {
+        hipLaunchKernelGGL((sdf_persistent_kernel<true, false, false>), grid, block, 0, stream, dcfg, nullptr, 1u,
+                           accum, start, count, counter, slots, event_batch, prof, none);
+    } else {
+        hipLaunchKernelGGL((sdf_persistent_kernel<false, false, false>), grid, block, 0, stream, dcfg, nullptr, 1u,
+                           accum, start, count, counter, slots, event_batch, nullptr, none);
+    }
     return hipGetLastError();
 }
 
@@ -395,12 +474,13 @@ hipError_t launch_sdf_batch(const FptRenderConfig *dcfgs, const BatchScene *dsce
                             uint32_t total_slots, float4 *accum, uint32_t *counter, uint32_t workgroups,
                             int event_batch, uint32_t spp_cap, DeviceProfile *prof, hipStream_t stream) {
     hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, counter);
+    const ProposerDevice none{};
     if (prof)
-        hipLaunchKernelGGL((sdf_persistent_kernel<true, true>), dim3(workgroups), dim3(256), 0, stream, dcfgs, dscenes,
-                           nscenes, accum, 0u, spp_cap, counter, total_slots, event_batch, prof);
+        hipLaunchKernelGGL((sdf_persistent_kernel<true, true, false>), dim3(workgroups), dim3(256), 0, stream, dcfgs,
+                           dscenes, nscenes, accum, 0u, spp_cap, counter, total_slots, event_batch, prof, none);
     else
-        hipLaunchKernelGGL((sdf_persistent_kernel<false, true>), dim3(workgroups), dim3(256), 0, stream, dcfgs,
-                           dscenes, nscenes, accum, 0u, spp_cap, counter, total_slots, event_batch, nullptr);
+        hipLaunchKernelGGL((sdf_persistent_kernel<false, true, false>), dim3(workgroups), dim3(256), 0, stream, dcfgs,
+                           dscenes, nscenes, accum, 0u, spp_cap, counter, total_slots, event_batch, nullptr, none);
     return hipGetLastError();
 }
 

@@ -216,13 +216,25 @@ class FptTuneResult(C.Structure):
                 ("best_ms", f32), ("default_ms", f32)]
 
 
+class FptProposerSettings(C.Structure):
+    _fields_ = [("box_half", f32), ("switch_distance", f32), ("resume_distance", f32), ("step_scale", f32)]
+
+
+class FptProposerStats(C.Structure):
+    _fields_ = [("proposer_calls", C.c_uint64), ("proposals", C.c_uint64), ("far_steps", C.c_uint64),
+                ("backtracks", C.c_uint64)]
+
+
+PROPOSER_PARAM_COUNT = 20801
+
+
 # Expected sizes (src/ffi.rs:494-507 plus the new FptRayHit).
 EXPECTED_SIZES = {
     FptSdfInstruction: 32, FptPrimitiveInstance: 80, FptAffineTransform: 64,
     FptIndexedPrimitive: 32, FptTypedSoAProgram: 5136, FptStitchPipelineStats: 72,
     FptRenderConfig: 30448, FptSdfProfileStats: 288, FptDiagnosticConfig: 24,
     FptMandelbulberFieldSample: 16, FptAsyncJitStats: 40, FptRayHit: 40, FptVoxelBuildInfo: 40,
-    FptTuneResult: 24,
+    FptTuneResult: 24, FptProposerSettings: 16, FptProposerStats: 32,
 }
 
 PKG_DIR = Path(__file__).resolve().parent
@@ -239,6 +251,7 @@ EXPORTED_SYMBOLS = (
     "fpt_hip_batch_tune", "fpt_hip_batch_render", "fpt_hip_batch_destroy", "fpt_hip_render_batch",
     "fpt_hip_autotune", "fpt_hip_device_alloc", "fpt_hip_device_free", "fpt_hip_copy", "fpt_hip_fill_f32",
     "fpt_hip_stream_create", "fpt_hip_stream_destroy", "fpt_hip_synchronize", "fpt_hip_time_renders",
+    "fpt_hip_proposer_samples", "fpt_hip_proposer_eval", "fpt_hip_plan_set_proposer", "fpt_hip_plan_profile",
 )
 
 _lib = None
@@ -321,6 +334,17 @@ def _declare(lib):
     lib.fpt_hip_time_renders.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, u32, P(C.c_double), cp,
                                          C.c_size_t]
     lib.fpt_hip_time_renders.restype = C.c_int
+    lib.fpt_hip_proposer_samples.argtypes = [P(FptRenderConfig), f32, f32, u32, C.c_uint64, u32, P(f32), P(u32),
+                                             cp, C.c_size_t]
+    lib.fpt_hip_proposer_samples.restype = C.c_int
+    lib.fpt_hip_proposer_eval.argtypes = [P(f32), C.c_size_t, f32, P(f32), C.c_size_t, P(f32), cp, C.c_size_t]
+    lib.fpt_hip_proposer_eval.restype = C.c_int
+    lib.fpt_hip_plan_set_proposer.argtypes = [C.c_void_p, P(f32), C.c_size_t, P(FptProposerSettings), cp,
+                                              C.c_size_t]
+    lib.fpt_hip_plan_set_proposer.restype = C.c_int
+    lib.fpt_hip_plan_profile.argtypes = [C.c_void_p, C.c_void_p, P(FptSdfProfileStats), P(FptProposerStats), cp,
+                                         C.c_size_t]
+    lib.fpt_hip_plan_profile.restype = C.c_int
     lib.fpt_hip_last_error.argtypes = []
     lib.fpt_hip_last_error.restype = C.c_char_p
 

@@ -855,6 +855,98 @@ int fpt_hip_time_renders(FptHipPlan *plan,
                          char *error,
                          size_t error_len);
 
+/*
+ * Learned far-field step proposer.  No counterpart in the reference (its
+ * march takes every step on the exact distance, Shaders.metal:3753-3783):
+ * a per-scene MLP distance estimate used for march steps far from the
+ * surface, evaluated with BF16 matrix instructions.
+ *
+ * Network: v = p / box_half; 128 Fourier features F[32a + 8q + 2i + c] =
+ * sin (c = 0) / cos (c = 1) of pi * 2^((4q + i) / 2) * v_a for axis a < 3,
+ * q < 4, i < 4; F[96..98] = v; F[99..127] = 0; then four SiLU layers
+ * 128->64->64->64->64 and a linear 64->1 output.  Weights and the inputs of
+ * the four hidden layers are rounded to BF16 (nearest even), sums are FP32,
+ * the last hidden layer and the output stay FP32.
+ *
+ * Parameters: FPT_PROPOSER_PARAM_COUNT floats, row-major (unit, input):
+ *   W1[64][128] b1[64]  W2[64][64] b2[64]  W3[64][64] b3[64]
+ *   W4[64][64] b4[64]   w5[64] b5
+ */
+#define FPT_PROPOSER_PARAM_COUNT 20801
+
+/* How a plan's marches use the proposal q (all distances in scene units):
+ * inside the centred box |x|,|y|,|z| < box_half, a march in far mode steps
+ * step_scale * q while q > switch_distance, then continues on the exact
+ * distance (backing up one far step if the exact distance there is
+ * negative); an exact step with |d| > resume_distance re-enters far mode.
+ * Camera marches start in far mode, bounce and shadow marches do not. */
+struct FptProposerSettings {
+    float box_half;         /* 1.5 for the Mandelbulb training box */
+    float switch_distance;  /* 0.02 */
+    float resume_distance;  /* >= switch_distance, e.g. 0.1 */
+    float step_scale;       /* (0, 1]; 0.5 */
+};
+FPT_ABI_ASSERT(sizeof(struct FptProposerSettings) == 16, "FptProposerSettings");
+
+/* Learned-stepping work counters of one counting pass. */
+struct FptProposerStats {
+    uint64_t proposer_calls;  /* wave-wide (64-lane) network evaluations */
+    uint64_t proposals;       /* lanes that used a proposal */
+    uint64_t far_steps;       /* steps taken on a proposal */
+    uint64_t backtracks;      /* far steps undone after crossing the surface */
+};
+FPT_ABI_ASSERT(sizeof(struct FptProposerStats) == 32, "FptProposerStats");
+
+/*
+ * Training samples of `config`'s scene: count float4 (x, y, z, d), d the
+ * exact FP32 scene distance.  Points [0, count/2) are uniform in
+ * [-box_half, box_half]^3; points [count/2, count) are uniform in the same
+ * box restricted to |d| < band (rejection sampling, at most max_attempts
+ * candidates per point; *failures counts points that found none and kept
+ * their last candidate).  Candidates come from a splitmix64 sequence keyed by
+ * (seed, point index), so the set is reproducible.  out_xyzd is host memory.
+ */
+int fpt_hip_proposer_samples(const struct FptRenderConfig *config,
+                             float box_half,
+                             float band,
+                             uint32_t count,
+                             uint64_t seed,
+                             uint32_t max_attempts,
+                             float *out_xyzd,
+                             uint32_t *failures,
+                             char *error,
+                             size_t error_len);
+
+/* Proposals at points_xyzw[4*i..4*i+2] (host memory) -> out[i]. */
+int fpt_hip_proposer_eval(const float *params,
+                          size_t param_count,
+                          float box_half,
+                          const float *points_xyzw,
+                          size_t point_count,
+                          float *out,
+                          char *error,
+                          size_t error_len);
+
+/* Attach (settings non-NULL) or detach (NULL) a proposer to an SDF plan;
+ * the plan then renders with learned stepping on the persistent kernel. */
+int fpt_hip_plan_set_proposer(FptHipPlan *plan,
+                              const float *params,
+                              size_t param_count,
+                              const struct FptProposerSettings *settings,
+                              char *error,
+                              size_t error_len);
+
+/* Untimed counting pass of a plan's full frame (same samples as a render):
+ * the reference's profile categories (Shaders.metal:7650-7704) into stats
+ * and, when learned (optional) is non-NULL, the learned-stepping counters.
+ * The image goes to device_accum. */
+int fpt_hip_plan_profile(FptHipPlan *plan,
+                         float *device_accum,
+                         struct FptSdfProfileStats *stats,
+                         struct FptProposerStats *learned,
+                         char *error,
+                         size_t error_len);
+
 /* Last error string of the calling thread (empty when none). */
 const char *fpt_hip_last_error(void);
 

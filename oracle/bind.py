@@ -35,6 +35,8 @@ def lib():
         P = C.POINTER
         dp = P(C.c_double)
         L.orc_render.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_uint32, C.c_uint32, dp]
+        L.orc_render_samples.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32,
+                                         C.c_uint32, dp]
         L.orc_trace_primary.argtypes = [C.c_void_p, C.c_int, C.c_int, dp]
         L.orc_map_distance.argtypes = [C.c_void_p, C.c_int, dp, C.c_size_t, dp]
         L.orc_normal.argtypes = [C.c_void_p, C.c_int, dp, C.c_size_t, dp]
@@ -150,6 +152,18 @@ def trace_primary(cfg, precision: int = 64, threads: int = 0) -> np.ndarray:
     out = np.zeros((cfg.height, cfg.width, 9), np.float64)
     if lib().orc_trace_primary(C.addressof(cfg), precision, threads, _dp(out)) != 0:
         raise RuntimeError("orc_trace_primary failed")
+    return out
+
+
+def render_samples(cfg, samples, rows=None, precision: int = 64, threads: int = 0) -> np.ndarray:
+    """Mean of samples range(*samples) per pixel, (H, W, 4) float64 (rows
+    outside `rows` stay zero)."""
+    y0, y1 = (0, cfg.height) if rows is None else rows
+    out = np.zeros((cfg.height, cfg.width, 4), np.float64)
+    rc = lib().orc_render_samples(C.addressof(cfg), precision, threads, y0, y1, int(samples[0]), int(samples[1]),
+                                  _dp(out))
+    if rc != 0:
+        raise RuntimeError("orc_render_samples failed")
     return out
 
 

@@ -1122,9 +1122,9 @@ V3<T> render_path(UV xy, uint32_t sample_idx, const FptRenderConfig &cfg) {
 }
 
 template <class T>
-void render_rows(const FptRenderConfig &cfg, int threads, uint32_t y0, uint32_t y1, double *accum) {
+void render_rows(const FptRenderConfig &cfg, int threads, uint32_t y0, uint32_t y1, uint32_t s0, uint32_t s1,
+                 double *accum) {
     const uint32_t W = cfg.width;
-    const uint32_t samples = std::min<uint32_t>(std::max<uint32_t>(cfg.samples, 1u), 512u);
     const uint32_t tiles_x = (W + 15) / 16, tiles_y = (y1 - y0 + 15) / 16;
     const long ntiles = long(tiles_x) * long(tiles_y);
     Counters total;
@@ -1144,9 +1144,9 @@ void render_rows(const FptRenderConfig &cfg, int threads, uint32_t y0, uint32_t 
                 for (uint32_t x = tx * 16; x < std::min(W, tx * 16 + 16); ++x) {
                     UV uv = screen_uv(x, y, cfg);
                     V3<T> c;
-                    for (uint32_t s = 0; s < samples; ++s) {
+                    for (uint32_t s = s0; s < s1; ++s) {
                         V3<T> sc = render_path<T>(uv, s, cfg);
-                        T a = T(1) / T(s + 1);
+                        T a = T(1) / T(s - s0 + 1);
                         c = vmix(c, sc, a);
                     }
                     double *o = accum + (size_t(y) * W + x) * 4;
@@ -1202,8 +1202,18 @@ int orc_render(const FptRenderConfig *cfg, int precision, int threads, uint32_t 
                uint32_t row_end, double *accum) {
     if (!cfg || !accum || cfg->width == 0 || cfg->height == 0) return 1;
     if (row_end > cfg->height || row_begin >= row_end) return 1;
-    if (precision == 64) render_rows<double>(*cfg, threads, row_begin, row_end, accum);
-    else render_rows<float>(*cfg, threads, row_begin, row_end, accum);
+    const uint32_t samples = std::min<uint32_t>(std::max<uint32_t>(cfg->samples, 1u), 512u);
+    if (precision == 64) render_rows<double>(*cfg, threads, row_begin, row_end, 0u, samples, accum);
+    else render_rows<float>(*cfg, threads, row_begin, row_end, 0u, samples, accum);
+    return 0;
+}
+
+int orc_render_samples(const FptRenderConfig *cfg, int precision, int threads, uint32_t row_begin,
+                       uint32_t row_end, uint32_t sample_begin, uint32_t sample_end, double *accum) {
+    if (!cfg || !accum || cfg->width == 0 || cfg->height == 0) return 1;
+    if (row_end > cfg->height || row_begin >= row_end || sample_begin >= sample_end) return 1;
+    if (precision == 64) render_rows<double>(*cfg, threads, row_begin, row_end, sample_begin, sample_end, accum);
+    else render_rows<float>(*cfg, threads, row_begin, row_end, sample_begin, sample_end, accum);
     return 0;
 }
 

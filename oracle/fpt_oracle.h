@@ -33,6 +33,12 @@ extern "C" {
 int orc_render(const struct FptRenderConfig *cfg, int precision, int threads,
                uint32_t row_begin, uint32_t row_end, double *accum);
 
+/* Mean of samples [sample_begin, sample_end) per pixel (the RNG keys of
+ * sample s are those of sample s of a full render), rows as orc_render. */
+int orc_render_samples(const struct FptRenderConfig *cfg, int precision, int threads,
+                       uint32_t row_begin, uint32_t row_end, uint32_t sample_begin,
+                       uint32_t sample_end, double *accum);
+
 /* Un-jittered primary ray per pixel: march + normal (Shaders.metal:
  * 3753-3783, 3871-3971).  out: 9 doubles per pixel {t, px, py, pz, nx, ny,
  * nz, hit, steps}. */

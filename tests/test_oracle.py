@@ -265,3 +265,18 @@ def test_batch_scene_generator_is_seeded_and_in_range():
             assert op.opcode == ffi.OP_MANDELBOX and -2.5 <= op.data[0] <= 2.5
         else:
             assert op.opcode == ffi.OP_MENGER and 3 <= op.data[0] <= 6
+
+
+def test_sample_range_render_composes():
+    """orc_render_samples over [0, spp) is orc_render; the running means of
+    [0, k) and [k, n) combine to the mean of [0, n); rows outside the band
+    stay zero."""
+    c = scenes.mandelbulb_scene(12, 10, 6, bounces=1)
+    full = bind.render(c, 64)
+    assert np.array_equal(bind.render_samples(c, (0, 6)), full)
+    a = bind.render_samples(c, (0, 2))
+    b = bind.render_samples(c, (2, 6))
+    assert np.allclose((2 * a + 4 * b) / 6, full, atol=1e-12)
+    band = bind.render_samples(c, (3, 5), rows=(4, 6))
+    assert np.all(band[:4] == 0) and np.all(band[6:] == 0)
+    assert np.array_equal(band[4:6], bind.render_samples(c, (3, 5))[4:6])
