@@ -62,6 +62,22 @@ NORMAL_DE_FACTOR = 4
 #  Proposer network per point: 2 x (128*64 + 3*64*64 + 64) multiply-adds.
 FLOPS_PER_PROPOSAL = 2 * (128 * 64 + 3 * 64 * 64 + 64)
 
+# HBM traffic of one exact-march frame launch, measured with rocprofv3 PMC
+# passes (profiles/sdf_persistent_mandelbulb_1080p_64spp.md): FETCH_SIZE and
+# WRITE_SIZE KiB per dispatch, scaled by what the same counters read for a
+# 1 GiB copy kernel (x2.000 and x1.000).
+TRAFFIC_FETCH_KIB, TRAFFIC_WRITE_KIB = 2567.875, 117916.5
+TRAFFIC_FETCH_SCALE, TRAFFIC_WRITE_SCALE = 2.0, 1.0
+TRAFFIC_WORKLOAD = (1920, 1080, 64)
+
+# HBM traffic of one exact-march frame launch, measured with rocprofv3 PMC
+# passes (profiles/sdf_persistent_mandelbulb_1080p_64spp.md): FETCH_SIZE and
+# WRITE_SIZE KiB per dispatch, scaled by what the same counters read for a
+# 1 GiB copy kernel (x2.000 and x1.000).
+TRAFFIC_FETCH_KIB, TRAFFIC_WRITE_KIB = 2567.875, 117916.5
+TRAFFIC_FETCH_SCALE, TRAFFIC_WRITE_SCALE = 2.0, 1.0
+TRAFFIC_WORKLOAD = (1920, 1080, 64)
+
 FP32_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
 BF16_PEAK_TFLOPS = 2500.0  # MI355X published dense BF16 matrix peak
 KERNEL = "sdf_persistent_kernel"
@@ -79,6 +95,9 @@ def parse():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=8)
     ap.add_argument("--ref-spp-factor", type=int, default=4)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run one device-to-device copy kernel of MIB MiB (the known-size reference for "
+                         "HBM byte counters) and exit")
     return ap.parse_args()
 
 
@@ -130,10 +149,16 @@ def main():
 
     name = api.device_name()
     assert "gfx" in name, f"bench.py needs a HIP GPU ({name})"
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 4
+        src, dst = api.DeviceBuffer(n), api.DeviceBuffer(n)
+        src.fill(1.0)
+        api.device_copy(dst, src)
+        print(json.dumps({"copy_kernel_bytes_read": n * 4, "copy_kernel_bytes_written": n * 4}), flush=True)
+        return
     cfg = scenes.learned_step_scene(args.width, args.height, args.spp)
     samples_per_step = cfg.width * cfg.height * cfg.samples
-    chunk = 8 if cfg.samples > 16 else cfg.samples   # samples per launch (chunked policy)
-    launches = (cfg.samples + chunk - 1) // chunk
+    launches = 1   # the persistent kernel renders all samples of the frame in one launch
 
     stream = api.Stream()
     accum = api.DeviceBuffer(cfg.width * cfg.height * 4)
@@ -167,6 +192,9 @@ def main():
         chosen = max(ok, key=lambda k: methods[k]["value"])
 
     m = methods[chosen]
+    traffic = None
+    if chosen == "exact" and (cfg.width, cfg.height, cfg.samples) == TRAFFIC_WORKLOAD:
+        traffic = 1024 * (TRAFFIC_FETCH_KIB * TRAFFIC_FETCH_SCALE + TRAFFIC_WRITE_KIB * TRAFFIC_WRITE_SCALE)
     launch_ms = m["ms_per_step"] / launches
     flops_launch = frame_flops(m["prof"]) / launches
     achieved = flops_launch / (launch_ms / 1e3) / 1e12
@@ -205,7 +233,7 @@ def main():
                    "width": cfg.width, "height": cfg.height, "spp": cfg.samples, "bounces": 4,
                    "methods": summary},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": KERNEL, "launches_per_step": launches, "launch_ms": launch_ms,
                      "flops_per_launch": flops_launch},
     }

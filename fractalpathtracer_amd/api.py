@@ -222,6 +222,13 @@ class DeviceBuffer:
             pass
 
 
+def device_copy(dst: DeviceBuffer, src: DeviceBuffer) -> None:
+    """Device-to-device copy of src into dst by the library's copy kernel."""
+    n = min(dst.count, src.count) * 4
+    err = ffi.error_buffer()
+    ffi.check(dst.lib.fpt_hip_copy(dst.ptr, src.ptr, n - n % 16, 2, err, len(err)), err)
+
+
 class Stream:
     """A non-blocking HIP stream of the library's runtime."""
 

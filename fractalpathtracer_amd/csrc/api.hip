@@ -216,6 +216,16 @@ hipError_t launch_sdf(FptHipPlan *plan, float4 *acc, uint32_t start, uint32_t ch
     return fpt::launch_sdf_persistent(plan->dcfg, c, acc, start, chunk, plan->counter, wg, int(plan->event_batch),
                                       prof, plan->proposer.enabled ? &plan->proposer.dev : nullptr, s);
 }
+// Samples per SDF launch.  The one-path-per-lane kernel follows the
+// reference's accumulation policy (chunk_for); the persistent kernel keeps
+// every pixel's running mean in a register across its samples, so one launch
+// covers the whole frame with the same sequence of mix() steps (the float4
+// round trip between chunked launches is exact): the image is bit-identical
+// and the accumulation buffer is read and written once per frame.
+uint32_t sdf_launch_chunk(const FptHipPlan *plan) {
+    const uint32_t spp = clamp_samples(plan->host.samples);
+    return plan->kernel == 1u ? spp : chunk_for(plan->host);
+}
 }  // namespace
 
 extern "C" {
@@ -260,7 +270,7 @@ int fpt_hip_plan_render(FptHipPlan *plan, float *device_accum, void *stream, cha
     hipStream_t s = static_cast<hipStream_t>(stream);
     const FptRenderConfig &c = plan->host;
     const uint32_t spp = clamp_samples(c.samples);
-    const uint32_t chunk = chunk_for(c);
+    const uint32_t chunk = c.renderer_backend == FPT_RENDERER_VOXEL ? chunk_for(c) : sdf_launch_chunk(plan);
     float4 *acc = reinterpret_cast<float4 *>(device_accum);
     for (uint32_t start = 0; start < spp; start += chunk) {
         hipError_t e = c.renderer_backend == FPT_RENDERER_VOXEL
@@ -345,7 +355,7 @@ int fpt_hip_plan_profile(FptHipPlan *plan, float *device_accum, FptSdfProfileSta
     DevBuf prof;
     HIP_TRY(prof.alloc(sizeof(fpt::DeviceProfile)), "profile buffer");
     HIP_TRY(hipMemsetAsync(prof.p, 0, sizeof(fpt::DeviceProfile), st.s), "profile clear");
-    const uint32_t spp = clamp_samples(plan->host.samples), chunk = chunk_for(plan->host);
+    const uint32_t spp = clamp_samples(plan->host.samples), chunk = sdf_launch_chunk(plan);
     float4 *acc = reinterpret_cast<float4 *>(device_accum);
     for (uint32_t s0 = 0; s0 < spp; s0 += chunk)
         HIP_TRY(launch_sdf(plan, acc, s0, chunk, prof.as<fpt::DeviceProfile>(), st.s), "profile kernel launch");
@@ -787,7 +797,7 @@ int fpt_metal_render(const char *metallib_path, const char *stitch_metallib_path
         HIP_TRY(prof.alloc(sizeof(fpt::DeviceProfile)), "profile buffer");
         HIP_TRY(acc2.alloc(pixels * sizeof(float4)), "profile accumulation buffer");
         HIP_TRY(hipMemsetAsync(prof.p, 0, sizeof(fpt::DeviceProfile), st.s), "profile clear");
-        const uint32_t spp = clamp_samples(config->samples), chunk = chunk_for(*config);
+        const uint32_t spp = clamp_samples(config->samples), chunk = sdf_launch_chunk(plan);
         for (uint32_t s0 = 0; s0 < spp; s0 += chunk)
             HIP_TRY(launch_sdf(plan, acc2.as<float4>(), s0, chunk, prof.as<fpt::DeviceProfile>(), st.s),
                     "profile kernel launch");
@@ -1050,6 +1060,14 @@ __global__ void fill_f32_kernel(float *dst, float v, size_t n) {
     size_t stride = size_t(gridDim.x) * blockDim.x;
     for (; i < n; i += stride) dst[i] = v;
 }
+
+// Device-to-device copy as a kernel (16-byte elements, grid-stride), so its
+// traffic shows up in the kernel's own counters.
+__global__ void copy_u4_kernel(uint4 *__restrict__ dst, const uint4 *__restrict__ src, size_t n) {
+    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
 }  // namespace
 
 extern "C" {
@@ -1070,6 +1088,20 @@ void fpt_hip_device_free(void *ptr) {
 }
 
 int fpt_hip_copy(void *dst, const void *src, size_t bytes, uint32_t to_device, char *error, size_t error_len) {
+    if (to_device == 2u) {
+        if (bytes % 16u != 0u || (reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) % 16u != 0u) {
+            set_error(error, error_len, "device copies take 16-byte aligned pointers and sizes");
+            return 1;
+        }
+        const size_t n = bytes / 16u;
+        if (!n) return 0;
+        const unsigned blocks = unsigned(std::min<size_t>((n + 255) / 256, 8192));
+        hipLaunchKernelGGL(copy_u4_kernel, dim3(blocks), dim3(256), 0, nullptr, static_cast<uint4 *>(dst),
+                           static_cast<const uint4 *>(src), n);
+        HIP_TRY(hipGetLastError(), "copy kernel launch");
+        HIP_TRY(hipDeviceSynchronize(), "device copy");
+        return 0;
+    }
     HIP_TRY(hipMemcpy(dst, src, bytes, to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost), "copy");
     return 0;
 }

@@ -85,8 +85,19 @@ __host__ __device__ __forceinline__ uint32_t slot_count(uint32_t w, uint32_t h) 
     return ((w + 7u) / 8u) * ((h + 7u) / 8u) * 64u;
 }
 
+// Minimum resident waves per SIMD the register allocation must allow (exact
+// and learned variants); the compiler caps VGPRs at 512 / waves.
+#ifndef FPT_PERSISTENT_WAVES
+#define FPT_PERSISTENT_WAVES 2
+#endif
+#ifndef FPT_LEARNED_WAVES
+#define FPT_LEARNED_WAVES 1
+#endif
+
 template <bool PROFILE, bool MULTI, bool LEARNED>
-__global__ __launch_bounds__(256) void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LEARNED ? FPT_LEARNED_WAVES
+                                                                             : FPT_PERSISTENT_WAVES)))
+void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
                                                              const BatchScene *__restrict__ scenes, uint32_t nscenes,
                                                              float4 *__restrict__ accum, uint32_t start,
                                                              uint32_t count, uint32_t *__restrict__ counter,

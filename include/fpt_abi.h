@@ -833,7 +833,9 @@ int fpt_hip_autotune(FptHipPlan *plan,
 /*
  * Device plumbing for callers that do not hold a HIP runtime of their own
  * (all on the runtime this library links): device buffers, host<->device
- * copies (to_device 1 = host to device), a float fill that is safe inside
+ * copies (to_device 1 = host to device, 0 = device to host, 2 = device to
+ * device by a copy kernel: 16-byte aligned pointers and size), a float fill
+ * that is safe inside
  * captured graphs, non-blocking streams, and HIP-event timing of `steps`
  * renders of a plan or batch on `stream` (exactly one of plan / batch).
  */

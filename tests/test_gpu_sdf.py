@@ -135,17 +135,30 @@ def test_image_parity_against_fp64_and_fp32_twin(case):
     assert rmse <= 1.25 * rmse_twin + 2e-3
 
 
+def _plan_render(cfg, kernel):
+    with api.Plan(cfg) as plan:
+        plan.tune(kernel=kernel)
+        buf = api.DeviceBuffer(cfg.width * cfg.height * 4)
+        plan.render_into(buf.address)
+        return buf.numpy().reshape(cfg.height, cfg.width, 4)
+
+
 def test_accumulation_policy_does_not_change_the_image():
-    """Batch vs chunked vs per-sample (metal_bridge.mm:4531-4564; the reference's
-    capability test requires batch == chunked, run_capability_tests.sh:22-28)."""
+    """Batch vs chunked vs per-sample launches of the one-path-per-lane kernel
+    (metal_bridge.mm:4531-4564; the reference's capability test requires
+    batch == chunked, run_capability_tests.sh:22-28) and the persistent
+    kernel's single launch per frame give one image."""
     base = scenes.mandelbulb_scene(64, 48, 6)
     imgs = []
     for mode in (1, 2, 3):
         c = scenes.clone(base)
         c.sdf_accumulation_mode = mode
         c.sdf_chunk_samples = 4
-        imgs.append(api.render(c))
-    assert np.array_equal(imgs[0], imgs[1]) and np.array_equal(imgs[1], imgs[2])
+        imgs.append(_plan_render(c, 0))
+        imgs.append(_plan_render(c, 1))
+    imgs.append(api.render(base))
+    for i, im in enumerate(imgs[1:]):
+        assert np.array_equal(imgs[0], im), i
 
 
 def test_spp_changes_the_image():
