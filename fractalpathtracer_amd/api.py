@@ -177,6 +177,74 @@ def proposer_eval(params, points, box_half: float = 1.5) -> np.ndarray:
     return out
 
 
+def diagnostic(cfg: FptRenderConfig, mode, max_distance: float = 0.0, bounce: int = 0,
+               output_png: str | None = None) -> tuple[np.ndarray, float]:
+    """Diagnostic image (H, W, 4) uint8 in PNG row order and the GPU time in
+    ms (fpt_hip_diagnostic_render; with output_png, fpt_metal_diagnostic_render
+    writes the PNG instead and the image is read back from it)."""
+    lib = ffi.load_library()
+    m = ffi.DIAGNOSTIC_MODES[mode] if isinstance(mode, str) else int(mode)
+    dg = ffi.FptDiagnosticConfig(m, int(bounce), float(max_distance), 0.0)
+    ms = C.c_double(0)
+    err = ffi.error_buffer()
+    if output_png:
+        ffi.check(lib.fpt_metal_diagnostic_render(None, output_png.encode(), None, 0, C.byref(cfg), C.byref(dg),
+                                                  C.byref(ms), err, len(err)), err)
+        return read_png(output_png), ms.value
+    out = np.zeros((cfg.height, cfg.width, 4), np.uint8)
+    ffi.check(lib.fpt_hip_diagnostic_render(C.byref(cfg), C.byref(dg), out.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                            out.size, C.byref(ms), err, len(err)), err)
+    return out, ms.value
+
+
+def read_png(path: str) -> np.ndarray:
+    """Decode an 8-bit RGBA, non-interlaced PNG into (H, W, 4) uint8 (all five
+    row filters; the library's writer uses filter 0)."""
+    import struct
+    import zlib
+    data = open(path, "rb").read()
+    if data[:8] != b"\x89PNG\r\n\x1a\n":
+        raise ValueError(f"{path}: not a PNG")
+    pos, idat, w, h = 8, b"", 0, 0
+    while pos < len(data):
+        n, kind = struct.unpack(">I4s", data[pos:pos + 8])
+        body = data[pos + 8:pos + 8 + n]
+        if kind == b"IHDR":
+            w, h, depth, ctype = struct.unpack(">IIBB", body[:10])
+            if depth != 8 or ctype != 6 or body[12] != 0:
+                raise ValueError(f"{path}: only 8-bit RGBA non-interlaced PNGs are read")
+        elif kind == b"IDAT":
+            idat += body
+        pos += 12 + n
+    raw = np.frombuffer(zlib.decompress(idat), np.uint8).reshape(h, 1 + 4 * w)
+    out = np.zeros((h, 4 * w), np.uint8)
+    prev = np.zeros(4 * w, np.int32)
+    for y in range(h):
+        f, line = int(raw[y, 0]), raw[y, 1:].astype(np.int32)
+        if f == 0:
+            cur = line
+        elif f == 2:
+            cur = (line + prev) & 255
+        else:
+            cur = np.zeros(4 * w, np.int32)
+            for x in range(4 * w):
+                a = int(cur[x - 4]) if x >= 4 else 0
+                b = int(prev[x])
+                c = int(prev[x - 4]) if x >= 4 else 0
+                if f == 1:
+                    pred = a
+                elif f == 3:
+                    pred = (a + b) // 2
+                else:
+                    p = a + b - c
+                    pa, pb, pc = abs(p - a), abs(p - b), abs(p - c)
+                    pred = a if (pa <= pb and pa <= pc) else (b if pb <= pc else c)
+                cur[x] = (int(line[x]) + pred) & 255
+        out[y] = cur
+        prev = cur.astype(np.int32)
+    return out.reshape(h, w, 4)
+
+
 def device_name() -> str:
     lib = ffi.load_library()
     buf = C.create_string_buffer(256)

@@ -1011,14 +1011,95 @@ int fpt_hip_voxel_trace_cells(const FptRenderConfig *config, const float *rays, 
     return 0;
 }
 
+}  // extern "C"
+
+namespace {
+// Diagnostic image of `config` into host rgba (width*height*4, PNG row
+// order), on its own stream; elapsed = GPU time of the diagnostic kernel (and
+// voxel build for the voxel backend), metal_bridge.mm:5865-6011.
+int diagnostic_image(const FptRenderConfig *config, const FptDiagnosticConfig *diagnostic, uint8_t *rgba,
+                     double *elapsed_ms, char *error, size_t error_len) {
+    if (!config || !diagnostic) {
+        set_error(error, error_len, "missing diagnostic render config");
+        return 1;
+    }
+    if (validate_config(config, error, error_len)) return 1;
+    if (diagnostic->mode > FPT_DIAGNOSTIC_SDF_BOUNCE_CONTRIBUTION) {
+        set_error(error, error_len, "unknown diagnostic mode %u", diagnostic->mode);
+        return 1;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    FptHipPlan *plan = nullptr;
+    if (fpt_hip_plan_create(config, &plan, error, error_len)) return 1;
+    struct PlanGuard {
+        FptHipPlan *p;
+        ~PlanGuard() { fpt_hip_plan_destroy(p); }
+    } guard{plan};
+    const size_t pixels = size_t(config->width) * config->height;
+    Stream st;
+    HIP_TRY(hipStreamCreate(&st.s), "hipStreamCreate");
+    DevBuf out;
+    HIP_TRY(out.alloc(pixels * 4), "diagnostic buffer allocation");
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0), "hipEventCreate");
+    HIP_TRY(hipEventCreate(&e1), "hipEventCreate");
+    HIP_TRY(hipEventRecord(e0, st.s), "hipEventRecord");
+    hipError_t e = config->renderer_backend == FPT_RENDERER_VOXEL
+                       ? fpt::launch_voxel_diagnostic(plan->dcfg, *config, plan->voxels, *diagnostic,
+                                                      out.as<uchar4>(), st.s)
+                       : fpt::launch_sdf_diagnostic(plan->dcfg, *config, *diagnostic, out.as<uchar4>(), st.s);
+    HIP_TRY(e, "diagnostic kernel launch");
+    HIP_TRY(hipEventRecord(e1, st.s), "hipEventRecord");
+    HIP_TRY(hipMemcpyAsync(rgba, out.p, pixels * 4, hipMemcpyDeviceToHost, st.s), "diagnostic readback");
+    HIP_TRY(hipStreamSynchronize(st.s), "diagnostic render");
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (elapsed_ms) {
+        const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        *elapsed_ms = ms > 0.f ? double(ms) + plan->build_ms : wall;
+    }
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
 int fpt_metal_diagnostic_render(const char *metallib_path, const char *output_path, const char *shader_source,
                                 size_t shader_source_len, const FptRenderConfig *config,
                                 const FptDiagnosticConfig *diagnostic, double *elapsed_ms, char *error,
                                 size_t error_len) {
-    (void)metallib_path; (void)output_path; (void)shader_source; (void)shader_source_len; (void)config;
-    (void)diagnostic; (void)elapsed_ms;
-    set_error(error, error_len, "diagnostic images are not implemented yet; use fpt_hip_trace_primary");
-    return 1;
+    (void)metallib_path; (void)shader_source; (void)shader_source_len;
+    if (!config || !diagnostic) {
+        set_error(error, error_len, "missing diagnostic render config");
+        return 1;
+    }
+    if (!output_path || !output_path[0]) {
+        set_error(error, error_len, "missing diagnostic output path");
+        return 1;
+    }
+    if (config->width == 0 || config->height == 0 || config->width > 16384 || config->height > 16384) {
+        set_error(error, error_len, "invalid render size %ux%u", config->width, config->height);
+        return 1;
+    }
+    std::vector<uint8_t> rgba(size_t(config->width) * config->height * 4);
+    if (diagnostic_image(config, diagnostic, rgba.data(), elapsed_ms, error, error_len)) return 1;
+    return write_png(output_path, config->width, config->height, rgba, error, error_len) ? 0 : 1;
+}
+
+int fpt_hip_diagnostic_render(const FptRenderConfig *config, const FptDiagnosticConfig *diagnostic, uint8_t *rgba,
+                              size_t rgba_len, double *elapsed_ms, char *error, size_t error_len) {
+    if (!config || !rgba) {
+        set_error(error, error_len, "missing config or output buffer");
+        return 1;
+    }
+    const size_t need = size_t(config->width) * config->height * 4;
+    if (rgba_len < need) {
+        set_error(error, error_len, "diagnostic buffer is too small (%zu < %zu)", rgba_len, need);
+        return 1;
+    }
+    return diagnostic_image(config, diagnostic, rgba, elapsed_ms, error, error_len);
 }
 
 int fpt_compare_images(const char *baseline_path, const char *candidate_path, const char *report_path, char *error,

@@ -57,6 +57,8 @@ hipError_t launch_sdf_persistent(const FptRenderConfig *dcfg, const FptRenderCon
                                  uint32_t start, uint32_t count, uint32_t *counter, uint32_t workgroups,
                                  int event_batch, DeviceProfile *prof, const ProposerDevice *prop,
                                  hipStream_t stream);
+hipError_t launch_sdf_diagnostic(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg,
+                                 const FptDiagnosticConfig &dg, uchar4 *out, hipStream_t stream);
 hipError_t launch_present(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, const float4 *accum,
                           uchar4 *out, hipStream_t stream);
 

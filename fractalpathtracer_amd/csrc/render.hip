@@ -10,6 +10,7 @@
 
 #include "fpt_device.h"
 #include "fpt_internal.h"
+#include "post.h"
 
 namespace fpt {
 
@@ -260,6 +261,266 @@ __global__ __launch_bounds__(256) void surface_sample_kernel(const FptRenderConf
 // Host launchers.
 // ---------------------------------------------------------------------------
 static dim3 tile_grid(uint32_t w, uint32_t h) { return dim3((w + 15u) / 16u, (h + 15u) / 16u, 1u); }
+
+// ---------------------------------------------------------------------------
+// Diagnostic images (sdfDiagnostic, Shaders.metal:6827-6985, 7210-7373):
+// one 8-bit RGBA value per pixel, flipped like the present kernel.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ f3 diag_scalar(float v, float scale) {  // SM:6842-6845
+    const float t = clampf(v / fmaxf(scale, 1.0f), 0.f, 1.f);
+    return mk(t, t, t);
+}
+__device__ __forceinline__ f3 diag_hdr(f3 c, const FptRenderConfig &cfg) {  // SM:6838-6840
+    return post_process(vmin(vmax0(c), 8.0f), cfg);
+}
+__device__ __forceinline__ f3 diag_normal(f3 n) {  // SM:6834-6836
+    return mk(clampf(n.x * 0.5f + 0.5f, 0.f, 1.f), clampf(n.y * 0.5f + 0.5f, 0.f, 1.f),
+              clampf(n.z * 0.5f + 0.5f, 0.f, 1.f));
+}
+__device__ __forceinline__ f3 diag_depth(float travel, bool hit, const FptRenderConfig &cfg,
+                                         const FptDiagnosticConfig &dg) {  // SM:6827-6832
+    if (!hit) return mk(0.f, 0.f, 0.f);
+    const float md = dg.max_distance > 0.001f ? dg.max_distance : cfg.render[4];
+    const float v = 1.0f - clampf(travel / fmaxf(md, 0.001f), 0.f, 1.f);
+    return mk(v, v, v);
+}
+
+__device__ __forceinline__ f3 jittered_camera_ray(UV xy, float frame, const FptRenderConfig &cfg) {
+    const float focal = 1.0f / tanf(cfg.camera_fov / 2.0f * kPi / 180.0f);
+    const float aa = 0.3f / fmaxf(float(cfg.width), float(cfg.height));
+    float jx, jy;
+    random_point(aa, xy, frame, jx, jy);
+    return rotate_camera_roll(normalize(mk(xy.x + jx, xy.y + jy, focal)), cfg.camera_yaw_pitch[0],
+                              cfg.camera_yaw_pitch[1], cfg.camera_roll);
+}
+
+__device__ __forceinline__ int bounce_limit(const FptRenderConfig &cfg) {
+    int b = int(cfg.render[0]) < 8 ? int(cfg.render[0]) : 8;
+    if (cfg.sdf_bounce_cap > 0u && int(cfg.sdf_bounce_cap) < b) b = int(cfg.sdf_bounce_cap);
+    return b;
+}
+
+// sdfPathCostDiagnostic (SM:6919-6985): mean march steps, shadow steps,
+// normal evaluations or bounces per path over min(spp, 8) samples.
+__device__ f3 path_cost_diagnostic(UV xy, const FptRenderConfig &cfg, const FptDiagnosticConfig &dg, Work &w) {
+    const uint32_t samples = cfg.samples < 1u ? 1u : (cfg.samples > 8u ? 8u : cfg.samples);
+    float t_primary = 0.f, t_shadow = 0.f, t_normal = 0.f, t_bounces = 0.f;
+    const f3 cam = ld3(cfg.camera_position);
+    const float far = cfg.render[4] * 0.99f, far2 = far * far;
+    const int bounces = bounce_limit(cfg);
+    for (uint32_t si = 0; si < samples; ++si) {
+        const float frame = float(si);
+        f3 rp = cam;
+        f3 dr = jittered_camera_ray(xy, frame, cfg);
+        int local_ni = int(cfg.render[1]);
+        for (int i = 0; i < bounces; ++i) {
+            int steps = 0;
+            rp = march(dr, rp, local_ni, cfg.render[3], 0.0002f, cfg, w, steps);
+            t_primary += float(steps);
+            local_ni = int(cfg.render[1] / (cfg.render[5] * 2.0f + 1.0f));
+            const f3 tv = rp - cam;
+            if (dot(tv, tv) > far2) break;
+            const Material mat = user_material(rp, cfg, w);
+            t_bounces += 1.0f;
+            t_normal += 1.0f;
+            const f3 n = normal_at(rp, cfg, w);
+            if (cfg.sun[0] == 1.0f) {  // sunStepCountWithNormal (SM:6910-6917)
+                f3 ld = rotate_camera(mk(0.f, 0.f, 1.f), cfg.sun[1] * kPi / 180.0f, cfg.sun[2] * kPi / 180.0f);
+                const float h1 = hash13(xy.x, xy.y, madd_rn(frame, 5.0f, 1.0f));
+                const float h2 = hash13(xy.x, xy.y, madd_rn(frame, 3.0f, 5.0f));
+                float sn, cs;
+                sincosf(h1 * 2.0f * kPi, &sn, &cs);
+                const float m = sqrtf(h2) * cfg.sun[4];
+                ld = rotate_camera(ld, cs * m, sn * m);
+                int sst = 0;
+                (void)march(ld, rp + n * 0.001f, int(cfg.render[1]), cfg.render[3], 0.0002f, cfg, w, sst);
+                t_shadow += float(sst);
+            }
+            const float fi = float(i);
+            const float r1 = hash13(xy.x, xy.y, madd_rn(frame, 1.37f, fi));
+            const float r2 = hash13(xy.x, xy.y, madd_rn(frame, 7.91f, fi));
+            const float q = (mat.ior - 1.0f) / (mat.ior + 1.0f);
+            const float f0 = q * q;
+            const float om = 1.0f - clampf(dot(n, -dr), 0.f, 1.f);
+            const float om2 = om * om;
+            const float fres = f0 + (1.0f - f0) * (om2 * om2 * om);
+            if (r1 > mat.translucency) {
+                const f3 metal = reflect(dr, n);
+                const f3 diffuse = random_vector(n, xy, madd_rn(frame, 13.37f, fi));
+                dr = normalize(vmix(metal, diffuse, mat.roughness));
+                if (r2 < fres * mat.specular) dr = metal;
+            } else {
+                const f3 refr = refract(dr, n, 1.0f / mat.ior);
+                const f3 refl = reflect(dr, n);
+                dr = (dot(refr, refr) < 0.000001f || !isfinite(refr.x) || r2 < fres) ? normalize(refl)
+                                                                                    : normalize(refr);
+            }
+            rp = rp + n * (0.001f * signf(dot(dr, n)));
+        }
+    }
+    const float inv = 1.0f / float(samples);
+    const float steps_scale = fmaxf(cfg.render[1] * fmaxf(cfg.render[0], 1.0f), 1.0f);
+    if (dg.mode == FPT_DIAGNOSTIC_SDF_PRIMARY_STEPS) return diag_scalar(t_primary * inv, steps_scale);
+    if (dg.mode == FPT_DIAGNOSTIC_SDF_SHADOW_STEPS) return diag_scalar(t_shadow * inv, steps_scale);
+    if (dg.mode == FPT_DIAGNOSTIC_SDF_NORMAL_EVALS)
+        return diag_scalar(t_normal * inv, fmaxf(cfg.render[0] * 2.0f, 1.0f));
+    return diag_scalar(t_bounces * inv, fmaxf(cfg.render[0], 1.0f));
+}
+
+// sdfBounceContributionDiagnostic (SM:7210-7279): emission + sun light of
+// bounce `_pad0` only, times the throughput that reaches it.
+__device__ f3 bounce_contribution_diagnostic(UV xy, const FptRenderConfig &cfg, const FptDiagnosticConfig &dg,
+                                             Work &w) {
+    const uint32_t target = dg._pad0;
+    const uint32_t samples = cfg.samples < 1u ? 1u : (cfg.samples > 8u ? 8u : cfg.samples);
+    const f3 cam = ld3(cfg.camera_position);
+    const float far = cfg.render[4] * 0.99f, far2 = far * far;
+    const int bounces = bounce_limit(cfg);
+    f3 total = mk(0.f, 0.f, 0.f);
+    int unused = 0;
+    for (uint32_t si = 0; si < samples; ++si) {
+        const float frame = float(si);
+        f3 rp = cam;
+        f3 dr = jittered_camera_ray(xy, frame, cfg);
+        int local_ni = int(cfg.render[1]);
+        f3 thr = mk(1.f, 1.f, 1.f);
+        for (int i = 0; i < bounces; ++i) {
+            int steps = 0;
+            rp = march(dr, rp, local_ni, cfg.render[3], 0.0002f, cfg, w, steps);
+            const f3 tv = rp - cam;
+            local_ni = int(cfg.render[1] / (cfg.render[5] * 2.0f + 1.0f));
+            if (dot(tv, tv) > far2) {
+                if (uint32_t(i) == target) total = total + environment(dr, cfg) * thr;
+                break;
+            }
+            const Material mat = user_material(rp, cfg, w);
+            const f3 n = normal_at(rp, cfg, w);
+            f3 contrib = mk(0.f, 0.f, 0.f);
+            if (mat.emission > 0.001f) contrib = contrib + mat.rgb * mat.emission;
+            if (cfg.sun[0] == 1.0f) contrib = contrib + sun_contribution(rp, xy, frame, mat, n, cfg, w, unused);
+            if (uint32_t(i) == target) total = total + contrib * thr;
+            const float fi = float(i);
+            const float r1 = hash13(xy.x, xy.y, madd_rn(frame, 1.37f, fi));
+            const float r2 = hash13(xy.x, xy.y, madd_rn(frame, 7.91f, fi));
+            const float q = (mat.ior - 1.0f) / (mat.ior + 1.0f);
+            const float f0 = q * q;
+            const float om = 1.0f - clampf(dot(n, -dr), 0.f, 1.f);
+            const float om2 = om * om;
+            const float fres = f0 + (1.0f - f0) * (om2 * om2 * om);
+            if (r1 > mat.translucency) {
+                const f3 metal = reflect(dr, n);
+                const f3 diffuse = random_vector(n, xy, madd_rn(frame, 13.37f, fi));
+                dr = normalize(vmix(metal, diffuse, mat.roughness));
+                if (r2 >= fres * mat.specular) thr = thr * mat.rgb;
+                else dr = metal;
+            } else {
+                const f3 refr = refract(dr, n, 1.0f / mat.ior);
+                const f3 refl = reflect(dr, n);
+                if (dot(refr, refr) < 0.000001f || !isfinite(refr.x) || r2 < fres) {
+                    dr = normalize(refl);
+                } else {
+                    dr = normalize(refr);
+                    thr = thr * vmix(mk(1.f, 1.f, 1.f), mat.rgb, 0.35f) * (1.0f - fres * 0.5f);
+                }
+            }
+            rp = rp + n * (0.001f * signf(dot(dr, n)));
+        }
+    }
+    return diag_hdr(total / float(samples), cfg);
+}
+
+__device__ f3 sdf_diagnostic(UV xy, const FptRenderConfig &cfg, const FptDiagnosticConfig &dg, Work &w) {
+    const uint32_t mode = dg.mode;
+    if (mode == FPT_DIAGNOSTIC_SDF_BOUNCE_CONTRIBUTION) return bounce_contribution_diagnostic(xy, cfg, dg, w);
+    if (mode == FPT_DIAGNOSTIC_SDF_PRIMARY_STEPS || mode == FPT_DIAGNOSTIC_SDF_SHADOW_STEPS ||
+        mode == FPT_DIAGNOSTIC_SDF_NORMAL_EVALS || mode == FPT_DIAGNOSTIC_SDF_BOUNCES)
+        return path_cost_diagnostic(xy, cfg, dg, w);
+    if (mode == FPT_DIAGNOSTIC_PATH_FINAL) {
+        const uint32_t samples = cfg.samples < 16u ? 16u : (cfg.samples > 64u ? 64u : cfg.samples);
+        f3 c = mk(0.f, 0.f, 0.f);
+        PathCounts pc = {0, 0, 0, 0, 0};
+        for (uint32_t i = 0; i < samples; ++i) c = c + render_path(xy, i, cfg, w, pc);
+        return post_process(c / float(samples), cfg);
+    }
+    // Un-jittered camera ray with the diagnostic's own march (SM:7304-7339).
+    const f3 ro = ld3(cfg.camera_position);
+    const float focal = 1.0f / tanf(cfg.camera_fov / 2.0f * kPi / 180.0f);
+    const f3 rd = rotate_camera_roll(normalize(mk(xy.x, xy.y, focal)), cfg.camera_yaw_pitch[0],
+                                     cfg.camera_yaw_pitch[1], cfg.camera_roll);
+    const int requested = int(fmaxf(cfg.render[1], 32.0f));
+    const int steps = requested < 420 ? requested : 420;
+    const float thr = fmaxf(cfg.render[3], 0.0005f);
+    f3 rp = ro;
+    bool hit = false;
+    for (int i = 0; i < steps; ++i) {
+        const float d = map_sdf(rp, cfg, w);
+        if (!isfinite(d)) break;
+        if (d < thr && length(rp - ro) > 0.001f) {
+            hit = true;
+            break;
+        }
+        rp = rp + rd * (fabsf(d) * 0.99f);
+        if (length(rp - ro) > cfg.render[4]) break;
+    }
+    if (mode == FPT_DIAGNOSTIC_HIT_MASK) return hit ? mk(1.f, 1.f, 1.f) : mk(0.f, 0.f, 0.f);
+    if (mode == FPT_DIAGNOSTIC_DEPTH) return diag_depth(length(rp - ro), hit, cfg, dg);
+    if (!hit) return mk(0.f, 0.f, 0.f);
+    if (mode == FPT_DIAGNOSTIC_DIFFUSE_NORMAL) {
+        const f3 n = normal_at(rp, cfg, w);
+        const f3 fwd = rotate_camera_roll(mk(0.f, 0.f, 1.f), cfg.camera_yaw_pitch[0], cfg.camera_yaw_pitch[1],
+                                          cfg.camera_roll);
+        const float k = 0.12f + 0.88f * fmaxf(dot(n, -fwd), 0.0f);
+        return mk(k, k, k);
+    }
+    // Colour index / palette position need the reference's generated
+    // Mandelbulber material code; without it the reference returns black.
+    if (mode == FPT_DIAGNOSTIC_MANDEL_COLOR_INDEX || mode == FPT_DIAGNOSTIC_MANDEL_PALETTE_POSITION)
+        return mk(0.f, 0.f, 0.f);
+    const Material mat = user_material(rp, cfg, w);
+    if (mode == FPT_DIAGNOSTIC_MATERIAL)
+        return mk(clampf(mat.rgb.x, 0.f, 1.f), clampf(mat.rgb.y, 0.f, 1.f), clampf(mat.rgb.z, 0.f, 1.f));
+    if (mode == FPT_DIAGNOSTIC_PATH_DIRECT) {
+        f3 c = mk(0.f, 0.f, 0.f);
+        if (cfg.sun[0] == 1.0f) {  // sunContribution (SM:5863-5867)
+            int unused = 0;
+            c = sun_contribution(rp, xy, 0.0f, user_material(rp, cfg, w), normal_at(rp, cfg, w), cfg, w, unused);
+        }
+        return diag_hdr(c, cfg);
+    }
+    if (mode == FPT_DIAGNOSTIC_PATH_ENVIRONMENT) return diag_hdr(environment(rd, cfg), cfg);
+    if (mode == FPT_DIAGNOSTIC_PATH_THROUGHPUT) {
+        const f3 n = normal_at(rp, cfg, w);
+        const f3 next = normalize(vmix(reflect(rd, n), random_vector(n, xy, 0.0f), mat.roughness));
+        const float al = clampf(dot(next, n) * 0.5f + 0.5f, 0.f, 1.f);
+        const f3 c = mat.rgb * al;
+        return mk(clampf(c.x, 0.f, 1.f), clampf(c.y, 0.f, 1.f), clampf(c.z, 0.f, 1.f));
+    }
+    return diag_normal(normal_at(rp, cfg, w));
+}
+
+__device__ __forceinline__ uchar4 encode_rgba8(f3 c) {
+    return make_uchar4((unsigned char)(clampf(c.x, 0.f, 1.f) * 255.0f), (unsigned char)(clampf(c.y, 0.f, 1.f) * 255.0f),
+                       (unsigned char)(clampf(c.z, 0.f, 1.f) * 255.0f), 255);
+}
+
+// sdf_diagnostic_kernel (SM:7375-7388): 16x16 pixel tiles, 8x8 per wave.
+__global__ __launch_bounds__(256) void sdf_diagnostic_kernel(const FptRenderConfig *__restrict__ cfgp,
+                                                             FptDiagnosticConfig dg, uchar4 *__restrict__ out) {
+    const FptRenderConfig &cfg = *cfgp;
+    uint32_t x, y;
+    tile_pixel(x, y);
+    if (x >= cfg.width || y >= cfg.height) return;
+    Work w = {0, 0};
+    const f3 c = sdf_diagnostic(screen_uv(x, y, cfg.width, cfg.height), cfg, dg, w);
+    out[size_t(cfg.height - 1u - y) * cfg.width + x] = encode_rgba8(c);
+}
+
+hipError_t launch_sdf_diagnostic(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg,
+                                 const FptDiagnosticConfig &dg, uchar4 *out, hipStream_t stream) {
+    dim3 grid((hcfg.width + 15u) / 16u, (hcfg.height + 15u) / 16u, 1u);
+    hipLaunchKernelGGL(sdf_diagnostic_kernel, grid, dim3(256), 0, stream, dcfg, dg, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_sdf_accumulate(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, float4 *accum,
                                  uint32_t start, uint32_t count, DeviceProfile *prof, hipStream_t stream) {

@@ -36,6 +36,7 @@
 #include <vector>
 
 #include "fpt_device.h"
+#include "post.h"
 #include "voxel_api.h"
 
 namespace fpt {
@@ -936,6 +937,97 @@ hipError_t launch_voxel_accumulate(const FptRenderConfig *dcfg, const FptRenderC
     dim3 grid((hcfg.width + 15u) / 16u, (hcfg.height + 15u) / 16u, 1u);
     hipLaunchKernelGGL(voxel_accumulate_kernel, grid, dim3(256), 0, stream, dcfg, v,
                        static_cast<const float *>(store.focus), accum, start, count, dda_steps);
+    return hipGetLastError();
+}
+
+// voxelDiagnostic (Shaders.metal:7390-7442).  PATH_FINAL sums
+// min(max(spp, 1), 64) voxel paths; the rest look at the un-jittered camera
+// ray's first voxel hit.
+__device__ f3 voxel_diagnostic(UV xy, const FptRenderConfig &cfg, const FptDiagnosticConfig &dg, const VoxelView &v,
+                               float focus, Work &w) {
+    const uint32_t mode = dg.mode;
+    int unused = 0;
+    if (mode == FPT_DIAGNOSTIC_PATH_FINAL) {
+        const uint32_t samples = cfg.samples < 1u ? 1u : (cfg.samples > 64u ? 64u : cfg.samples);
+        f3 c = mk(0.f, 0.f, 0.f);
+        for (uint32_t s = 0; s < samples; ++s) c = c + render_voxel_path(xy, s, focus, cfg, v, w, unused);
+        return post_process(c / float(samples), cfg);
+    }
+    const float focal = 1.0f / tanf(cfg.camera_fov / 2.0f * kPi / 180.0f);
+    const f3 dir = rotate_camera_roll(normalize(mk(xy.x, xy.y, focal)), cfg.camera_yaw_pitch[0],
+                                      cfg.camera_yaw_pitch[1], cfg.camera_roll);
+    const VHit h = trace_voxel(ld3(cfg.camera_position), dir, cfg, v, NoVisit());
+    if (mode == FPT_DIAGNOSTIC_HIT_MASK) return h.hit ? mk(1.f, 1.f, 1.f) : mk(0.f, 0.f, 0.f);
+    if (mode == FPT_DIAGNOSTIC_DEPTH) {  // SM:6827-6832 on the entry distance
+        if (!h.hit) return mk(0.f, 0.f, 0.f);
+        const float md = dg.max_distance > 0.001f ? dg.max_distance : cfg.render[4];
+        const float k = 1.0f - clampf(h.t / fmaxf(md, 0.001f), 0.f, 1.f);
+        return mk(k, k, k);
+    }
+    if (mode == FPT_DIAGNOSTIC_SDF_PRIMARY_STEPS) {
+        const float scale = float((cfg.voxel_resolution > 1u ? cfg.voxel_resolution : 1u) * 3u);
+        const float k = clampf(float(h.steps) / fmaxf(scale, 1.0f), 0.f, 1.f);
+        return mk(k, k, k);
+    }
+    if (!h.hit) return mk(0.f, 0.f, 0.f);
+    f3 n = cfg.voxel_normal_mode == 1u ? normal_at(h.cell_center, cfg, w)
+                                        : (cfg.voxel_normal_mode == 2u ? normal_at(h.position, cfg, w) : h.normal);
+    if (dot(n, dir) > 0.0f) n = -n;
+    if (mode == FPT_DIAGNOSTIC_DIFFUSE_NORMAL) {
+        const f3 fwd = rotate_camera_roll(mk(0.f, 0.f, 1.f), cfg.camera_yaw_pitch[0], cfg.camera_yaw_pitch[1],
+                                          cfg.camera_roll);
+        const float k = 0.12f + 0.88f * fmaxf(dot(n, -fwd), 0.0f);
+        return mk(k, k, k);
+    }
+    f3 on = cfg.voxel_normal_mode == 2u ? h.normal : n;
+    if (dot(on, dir) > 0.0f) on = -on;
+    const Material mat = hit_material(h, cfg, v, w);
+    if (mode == FPT_DIAGNOSTIC_MATERIAL)
+        return mk(clampf(mat.rgb.x, 0.f, 1.f), clampf(mat.rgb.y, 0.f, 1.f), clampf(mat.rgb.z, 0.f, 1.f));
+    if (mode == FPT_DIAGNOSTIC_PATH_DIRECT) {
+        const f3 c = cfg.sun[0] == 1.0f ? voxel_sun(h.position, xy, 0.0f, mat, n, on, cfg, v, unused)
+                                        : mk(0.f, 0.f, 0.f);
+        return post_process(vmin(vmax0(c), 8.0f), cfg);
+    }
+    if (mode == FPT_DIAGNOSTIC_PATH_ENVIRONMENT) return post_process(vmin(vmax0(environment(dir, cfg)), 8.0f), cfg);
+    if (mode == FPT_DIAGNOSTIC_PATH_THROUGHPUT) {
+        const f3 next = normalize(vmix(reflect(dir, n), random_vector(n, xy, 0.0f), mat.roughness));
+        const float al = clampf(dot(next, n) * 0.5f + 0.5f, 0.f, 1.f);
+        const f3 c = mat.rgb * al;
+        return mk(clampf(c.x, 0.f, 1.f), clampf(c.y, 0.f, 1.f), clampf(c.z, 0.f, 1.f));
+    }
+    return mk(clampf(n.x * 0.5f + 0.5f, 0.f, 1.f), clampf(n.y * 0.5f + 0.5f, 0.f, 1.f),
+              clampf(n.z * 0.5f + 0.5f, 0.f, 1.f));
+}
+
+__global__ __launch_bounds__(256) void voxel_diagnostic_kernel(const FptRenderConfig *__restrict__ cfgp, VoxelView v,
+                                                               const float *__restrict__ focus_p,
+                                                               FptDiagnosticConfig dg, uchar4 *__restrict__ out) {
+    const FptRenderConfig &cfg = *cfgp;
+    const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u;
+    const uint32_t x = blockIdx.x * 16u + (wave & 1u) * 8u + (lane & 7u);
+    const uint32_t y = blockIdx.y * 16u + (wave >> 1) * 8u + (lane >> 3);
+    if (x >= cfg.width || y >= cfg.height) return;
+    Work w = {0, 0};
+    const f3 c = voxel_diagnostic(screen_uv(x, y, cfg.width, cfg.height), cfg, dg, v, *focus_p, w);
+    out[size_t(cfg.height - 1u - y) * cfg.width + x] =
+        make_uchar4((unsigned char)(clampf(c.x, 0.f, 1.f) * 255.0f), (unsigned char)(clampf(c.y, 0.f, 1.f) * 255.0f),
+                    (unsigned char)(clampf(c.z, 0.f, 1.f) * 255.0f), 255);
+}
+
+hipError_t launch_voxel_diagnostic(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, VoxelStore &store,
+                                   const FptDiagnosticConfig &dg, uchar4 *out, hipStream_t stream) {
+    VoxelView v = view_of(store);
+    if (!store.focus) {
+        hipError_t e = hipMalloc(&store.focus, sizeof(float));
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(voxel_focus_kernel, dim3(1), dim3(64), 0, stream, dcfg, v, static_cast<float *>(store.focus));
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    dim3 grid((hcfg.width + 15u) / 16u, (hcfg.height + 15u) / 16u, 1u);
+    hipLaunchKernelGGL(voxel_diagnostic_kernel, grid, dim3(256), 0, stream, dcfg, v,
+                       static_cast<const float *>(store.focus), dg, out);
     return hipGetLastError();
 }
 

@@ -30,6 +30,8 @@ void voxel_free(VoxelStore &store);
 hipError_t launch_voxel_accumulate(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, VoxelStore &store,
                                    float4 *accum, uint32_t start, uint32_t count, hipStream_t stream,
                                    unsigned long long *dda_steps);
+hipError_t launch_voxel_diagnostic(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, VoxelStore &store,
+                                   const FptDiagnosticConfig &dg, uchar4 *out, hipStream_t stream);
 hipError_t launch_voxel_trace_cells(const FptRenderConfig *dcfg, const VoxelStore &store, const float *rays, size_t n,
                                     uint32_t cap, int *cells, uint32_t *counts, float *hits, hipStream_t stream);
 

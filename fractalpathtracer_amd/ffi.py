@@ -192,6 +192,15 @@ class FptAsyncJitStats(C.Structure):
                 ("_pad0", u32)]
 
 
+# FptDiagnosticMode (metal_bridge.h:370-387).
+DIAGNOSTIC_MODES = {
+    "depth": 0, "normal": 1, "material": 2, "hit_mask": 3, "path_direct": 4, "path_environment": 5,
+    "path_throughput": 6, "path_final": 7, "diffuse_normal": 8, "mandel_color_index": 9,
+    "mandel_palette_position": 10, "sdf_primary_steps": 11, "sdf_shadow_steps": 12, "sdf_normal_evals": 13,
+    "sdf_bounces": 14, "sdf_bounce_contribution": 15,
+}
+
+
 class FptDiagnosticConfig(C.Structure):
     _fields_ = [("mode", u32), ("_pad0", u32), ("max_distance", f32), ("normal_mix", f32),
                 ("dispatch_origin", u32 * 2)]
@@ -252,6 +261,7 @@ EXPORTED_SYMBOLS = (
     "fpt_hip_autotune", "fpt_hip_device_alloc", "fpt_hip_device_free", "fpt_hip_copy", "fpt_hip_fill_f32",
     "fpt_hip_stream_create", "fpt_hip_stream_destroy", "fpt_hip_synchronize", "fpt_hip_time_renders",
     "fpt_hip_proposer_samples", "fpt_hip_proposer_eval", "fpt_hip_plan_set_proposer", "fpt_hip_plan_profile",
+    "fpt_hip_diagnostic_render",
 )
 
 _lib = None
@@ -345,6 +355,9 @@ def _declare(lib):
     lib.fpt_hip_plan_profile.argtypes = [C.c_void_p, C.c_void_p, P(FptSdfProfileStats), P(FptProposerStats), cp,
                                          C.c_size_t]
     lib.fpt_hip_plan_profile.restype = C.c_int
+    lib.fpt_hip_diagnostic_render.argtypes = [P(FptRenderConfig), P(FptDiagnosticConfig), P(C.c_uint8), C.c_size_t,
+                                              P(C.c_double), cp, C.c_size_t]
+    lib.fpt_hip_diagnostic_render.restype = C.c_int
     lib.fpt_hip_last_error.argtypes = []
     lib.fpt_hip_last_error.restype = C.c_char_p
 

@@ -858,6 +858,20 @@ int fpt_hip_time_renders(FptHipPlan *plan,
                          size_t error_len);
 
 /*
+ * Diagnostic image (the fpt_metal_diagnostic_render kernels,
+ * Shaders.metal:7282-7442) into host memory: rgba holds width*height*4 bytes
+ * in PNG row order (row 0 = top).  *elapsed_ms (optional) is the GPU time of
+ * the diagnostic kernel plus the voxel build for the voxel backend.
+ */
+int fpt_hip_diagnostic_render(const struct FptRenderConfig *config,
+                              const struct FptDiagnosticConfig *diagnostic,
+                              uint8_t *rgba,
+                              size_t rgba_len,
+                              double *elapsed_ms,
+                              char *error,
+                              size_t error_len);
+
+/*
  * Learned far-field step proposer.  No counterpart in the reference (its
  * march takes every step on the exact distance, Shaders.metal:3753-3783):
  * a per-scene MLP distance estimate used for march steps far from the

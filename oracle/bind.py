@@ -35,6 +35,8 @@ def lib():
         P = C.POINTER
         dp = P(C.c_double)
         L.orc_render.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_uint32, C.c_uint32, dp]
+        L.orc_present.argtypes = [C.c_void_p, C.c_int, P(C.c_float), P(C.c_uint8)]
+        L.orc_diagnostic.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_int, C.c_int, P(C.c_uint8)]
         L.orc_render_samples.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32,
                                          C.c_uint32, dp]
         L.orc_trace_primary.argtypes = [C.c_void_p, C.c_int, C.c_int, dp]
@@ -164,6 +166,26 @@ def render_samples(cfg, samples, rows=None, precision: int = 64, threads: int = 
                                   _dp(out))
     if rc != 0:
         raise RuntimeError("orc_render_samples failed")
+    return out
+
+
+def present(cfg, accum, precision: int = 64) -> np.ndarray:
+    """RGBA8 (H, W, 4) of a (H, W, 4) float accumulation buffer."""
+    a = np.ascontiguousarray(accum, np.float32)
+    out = np.zeros((cfg.height, cfg.width, 4), np.uint8)
+    if lib().orc_present(C.addressof(cfg), precision, a.ctypes.data_as(C.POINTER(C.c_float)),
+                         out.ctypes.data_as(C.POINTER(C.c_uint8))) != 0:
+        raise RuntimeError("orc_present failed")
+    return out
+
+
+def diagnostic(cfg, mode: int, bounce: int = 0, max_distance: float = 0.0, precision: int = 64,
+               threads: int = 0) -> np.ndarray:
+    """Diagnostic RGBA8 (H, W, 4) in PNG row order."""
+    out = np.zeros((cfg.height, cfg.width, 4), np.uint8)
+    if lib().orc_diagnostic(C.addressof(cfg), int(mode), int(bounce), float(max_distance), precision, threads,
+                            out.ctypes.data_as(C.POINTER(C.c_uint8))) != 0:
+        raise RuntimeError("orc_diagnostic failed")
     return out
 
 

@@ -1193,6 +1193,7 @@ void trace_primary(const FptRenderConfig &cfg, int threads, double *out) {
 }
 
 #include "fpt_oracle_voxel.inc"
+#include "fpt_oracle_diag.inc"
 
 }  // namespace
 
@@ -1292,6 +1293,47 @@ void orc_mandelbox_step(double s[5], double scale, double min_r2, double fixed_r
     double dr = s[4];
     mandelbox_kernel<double, double>(z, dr, scale, min_r2, fixed_r2);
     s[0] = z.x; s[1] = z.y; s[2] = z.z; s[3] = length(z); s[4] = dr;
+}
+
+int orc_present(const FptRenderConfig *cfg, int precision, const float *accum, uint8_t *rgba) {
+    if (!cfg || !accum || !rgba || cfg->width == 0 || cfg->height == 0) return 1;
+    if (precision == 64) present<double>(*cfg, accum, rgba);
+    else present<float>(*cfg, accum, rgba);
+    return 0;
+}
+
+int orc_diagnostic(const FptRenderConfig *cfg, uint32_t mode, uint32_t target_bounce, float max_distance,
+                   int precision, int threads, uint8_t *rgba) {
+    if (!cfg || !rgba || cfg->width == 0 || cfg->height == 0 || mode > FPT_DIAGNOSTIC_SDF_BOUNCE_CONTRIBUTION)
+        return 1;
+    const bool voxel = cfg->renderer_backend == FPT_RENDERER_VOXEL;
+    VoxTables t;
+    float focus = 0.0f;
+    if (voxel) {
+        if (!vx_build(*cfg, threads, t)) return 1;
+        focus = vx_focus(*cfg, t);
+    }
+    const uint32_t W = cfg->width, H = cfg->height;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 2) num_threads(threads > 0 ? threads : omp_get_max_threads())
+#endif
+    for (long y = 0; y < long(H); ++y) {
+        for (uint32_t x = 0; x < W; ++x) {
+            UV uv = screen_uv(x, uint32_t(y), *cfg);
+            uint8_t *o = rgba + (size_t(H - 1 - uint32_t(y)) * W + x) * 4;
+            if (voxel) {
+                V3<float> c = voxel_diagnostic(uv, *cfg, t, focus, mode, max_distance);
+                store_rgba8(o, c.x, c.y, c.z);
+            } else if (precision == 64) {
+                V3<double> c = sdf_diagnostic<double>(uv, *cfg, mode, target_bounce, max_distance);
+                store_rgba8(o, c.x, c.y, c.z);
+            } else {
+                V3<float> c = sdf_diagnostic<float>(uv, *cfg, mode, target_bounce, max_distance);
+                store_rgba8(o, c.x, c.y, c.z);
+            }
+        }
+    }
+    return 0;
 }
 
 uint32_t orc_hash13_bits(float x, float y, float z) { return hash13bits(x, y, z); }

@@ -72,6 +72,17 @@ void orc_mandelbulb_step(double state[5], double power);
 void orc_mandelbox_step(double state[5], double scale, double min_r2,
                         double fixed_r2);
 
+/* Present (post-process, chromatic aberration, highlight bloom, vertical
+ * flip; Shaders.metal:6778-6825, 7462-7507) of a float4 accumulation buffer
+ * (width*height*4 floats, row 0 at the top of the scene) into RGBA8. */
+int orc_present(const struct FptRenderConfig *cfg, int precision, const float *accum, uint8_t *rgba);
+
+/* Diagnostic image (sdfDiagnostic / voxelDiagnostic, Shaders.metal:
+ * 6827-6985, 7210-7442) into RGBA8 in PNG row order.  The voxel backend is
+ * FP32 whatever `precision` says. */
+int orc_diagnostic(const struct FptRenderConfig *cfg, uint32_t mode, uint32_t target_bounce,
+                   float max_distance, int precision, int threads, uint8_t *rgba);
+
 /* hash13 (Shaders.metal:541-555). */
 uint32_t orc_hash13_bits(float x, float y, float z);
 float orc_hash13(float x, float y, float z);
