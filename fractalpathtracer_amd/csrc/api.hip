@@ -165,6 +165,245 @@ bool write_png(const char *path, uint32_t w, uint32_t h, const std::vector<uint8
     return true;
 }
 
+struct Image {
+    uint32_t width = 0, height = 0;
+    std::vector<uint8_t> rgba;
+};
+
+uint32_t get_be32(const uint8_t *p) { return (uint32_t(p[0]) << 24) | (uint32_t(p[1]) << 16) | (uint32_t(p[2]) << 8) | p[3]; }
+
+// PNG decoder for 8-bit, non-interlaced grey / grey+alpha / RGB / RGBA
+// images (all five row filters), expanded to RGBA.
+bool read_png(const char *path, Image &img, char *error, size_t error_len) {
+    FILE *f = path ? std::fopen(path, "rb") : nullptr;
+    if (!f) {
+        set_error(error, error_len, "failed to read image %s", path ? path : "(null)");
+        return false;
+    }
+    std::vector<uint8_t> data;
+    uint8_t buf[65536];
+    size_t n;
+    while ((n = std::fread(buf, 1, sizeof(buf), f)) > 0) data.insert(data.end(), buf, buf + n);
+    std::fclose(f);
+    static const uint8_t sig[8] = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1a, '\n'};
+    if (data.size() < 8 || std::memcmp(data.data(), sig, 8) != 0) {
+        set_error(error, error_len, "%s is not a PNG image", path);
+        return false;
+    }
+    uint32_t w = 0, h = 0;
+    int channels = 0;
+    std::vector<uint8_t> z;
+    for (size_t pos = 8; pos + 12 <= data.size();) {
+        const uint32_t len = get_be32(&data[pos]);
+        if (pos + 12 + size_t(len) > data.size()) break;
+        const uint8_t *type = &data[pos + 4], *body = &data[pos + 8];
+        if (!std::memcmp(type, "IHDR", 4) && len >= 13) {
+            w = get_be32(body);
+            h = get_be32(body + 4);
+            const uint8_t depth = body[8], ctype = body[9], interlace = body[12];
+            channels = ctype == 0 ? 1 : ctype == 4 ? 2 : ctype == 2 ? 3 : ctype == 6 ? 4 : 0;
+            if (depth != 8 || channels == 0 || interlace != 0) {
+                set_error(error, error_len, "%s: only 8-bit non-interlaced grey/RGB(A) PNGs are supported", path);
+                return false;
+            }
+        } else if (!std::memcmp(type, "IDAT", 4)) {
+            z.insert(z.end(), body, body + len);
+        } else if (!std::memcmp(type, "IEND", 4)) {
+            break;
+        }
+        pos += 12 + size_t(len);
+    }
+    if (w == 0 || h == 0 || w > 65536 || h > 65536) {
+        set_error(error, error_len, "%s: missing or invalid PNG header", path);
+        return false;
+    }
+    const size_t stride = size_t(w) * channels;
+    std::vector<uint8_t> raw(size_t(h) * (stride + 1));
+    z_stream zs{};
+    zs.next_in = z.data();
+    zs.avail_in = uInt(z.size());
+    zs.next_out = raw.data();
+    zs.avail_out = uInt(raw.size());
+    int zr = inflateInit(&zs);
+    if (zr == Z_OK) zr = inflate(&zs, Z_FINISH);
+    inflateEnd(&zs);
+    if (zr != Z_STREAM_END || zs.total_out != raw.size()) {
+        set_error(error, error_len, "%s: corrupt PNG image data", path);
+        return false;
+    }
+    std::vector<uint8_t> px(size_t(h) * stride), zero(stride, 0);
+    for (uint32_t y = 0; y < h; ++y) {
+        const uint8_t ft = raw[size_t(y) * (stride + 1)];
+        const uint8_t *in = &raw[size_t(y) * (stride + 1) + 1];
+        uint8_t *cur = &px[size_t(y) * stride];
+        const uint8_t *prev = y ? &px[size_t(y - 1) * stride] : zero.data();
+        for (size_t x = 0; x < stride; ++x) {
+            const int a = x >= size_t(channels) ? cur[x - channels] : 0;
+            const int b = prev[x];
+            const int c = x >= size_t(channels) ? prev[x - channels] : 0;
+            int pred = 0;
+            if (ft == 1) pred = a;
+            else if (ft == 2) pred = b;
+            else if (ft == 3) pred = (a + b) / 2;
+            else if (ft == 4) {
+                const int p = a + b - c, pa = std::abs(p - a), pb = std::abs(p - b), pc = std::abs(p - c);
+                pred = (pa <= pb && pa <= pc) ? a : (pb <= pc ? b : c);
+            } else if (ft != 0) {
+                set_error(error, error_len, "%s: bad PNG row filter %u", path, unsigned(ft));
+                return false;
+            }
+            cur[x] = uint8_t(in[x] + pred);
+        }
+    }
+    img.width = w;
+    img.height = h;
+    img.rgba.assign(size_t(w) * h * 4, 255);
+    for (size_t i = 0; i < size_t(w) * h; ++i) {
+        const uint8_t *s = &px[i * channels];
+        uint8_t *d = &img.rgba[i * 4];
+        if (channels <= 2) {
+            d[0] = d[1] = d[2] = s[0];
+            if (channels == 2) d[3] = s[1];
+        } else {
+            d[0] = s[0]; d[1] = s[1]; d[2] = s[2];
+            if (channels == 4) d[3] = s[3];
+        }
+    }
+    return true;
+}
+
+// Luma (Rec. 709 weights on 8-bit values), metal_bridge.mm:2383-2386.
+double luma_of(const Image &im, size_t p) {
+    const uint8_t *c = &im.rgba[p * 4];
+    return 0.2126 * double(c[0]) + 0.7152 * double(c[1]) + 0.0722 * double(c[2]);
+}
+
+// SSIM of two luma sets with sample (n - 1) statistics and the 8-bit
+// constants C1 = (0.01 * 255)^2, C2 = (0.03 * 255)^2.
+double ssim_of(const double *a, const double *b, size_t n, bool empty_is_one) {
+    const double c1 = 6.5025, c2 = 58.5225;
+    double ma = 0, mb = 0;
+    for (size_t i = 0; i < n; ++i) {
+        ma += a[i];
+        mb += b[i];
+    }
+    ma /= double(n);
+    mb /= double(n);
+    double va = 0, vb = 0, cv = 0;
+    for (size_t i = 0; i < n; ++i) {
+        va += (a[i] - ma) * (a[i] - ma);
+        vb += (b[i] - mb) * (b[i] - mb);
+        cv += (a[i] - ma) * (b[i] - mb);
+    }
+    const double den = std::max(1.0, double(n) - 1.0);
+    va /= den;
+    vb /= den;
+    cv /= den;
+    const double num = (2.0 * ma * mb + c1) * (2.0 * cv + c2);
+    const double dnm = (ma * ma + mb * mb + c1) * (va + vb + c2);
+    return dnm > 0.0 ? num / dnm : (empty_is_one ? 1.0 : 0.0);
+}
+
+// Pixel-count-weighted mean SSIM over 8x8 luma blocks (metal_bridge.mm:
+// 2388-2436).
+double block_ssim(const Image &a, const Image &b) {
+    double sum = 0, weight = 0;
+    std::vector<double> la, lb;
+    for (uint32_t by = 0; by < a.height; by += 8)
+        for (uint32_t bx = 0; bx < a.width; bx += 8) {
+            la.clear();
+            lb.clear();
+            for (uint32_t y = by; y < std::min(a.height, by + 8u); ++y)
+                for (uint32_t x = bx; x < std::min(a.width, bx + 8u); ++x) {
+                    la.push_back(luma_of(a, size_t(y) * a.width + x));
+                    lb.push_back(luma_of(b, size_t(y) * a.width + x));
+                }
+            const double n = double(la.size());
+            sum += ssim_of(la.data(), lb.data(), la.size(), true) * n;
+            weight += n;
+        }
+    return weight > 0 ? std::clamp(sum / weight, -1.0, 1.0) : 0.0;
+}
+
+// SSIM of the 64x64 box-downsampled lumas (metal_bridge.mm:2438-2497).
+double low_frequency_ssim(const Image &a, const Image &b) {
+    const uint32_t T = 64;
+    std::vector<double> da(T * T), db(T * T);
+    for (uint32_t y = 0; y < T; ++y) {
+        const uint32_t y0 = uint32_t(uint64_t(y) * a.height / T);
+        const uint32_t y1 = std::max(uint32_t(uint64_t(y + 1) * a.height / T), y0 + 1u);
+        for (uint32_t x = 0; x < T; ++x) {
+            const uint32_t x0 = uint32_t(uint64_t(x) * a.width / T);
+            const uint32_t x1 = std::max(uint32_t(uint64_t(x + 1) * a.width / T), x0 + 1u);
+            double sa = 0, sb = 0, n = 0;
+            for (uint32_t sy = y0; sy < y1; ++sy)
+                for (uint32_t sx = x0; sx < x1; ++sx) {
+                    sa += luma_of(a, size_t(sy) * a.width + sx);
+                    sb += luma_of(b, size_t(sy) * a.width + sx);
+                    n += 1;
+                }
+            da[y * T + x] = sa / n;
+            db[y * T + x] = sb / n;
+        }
+    }
+    return std::clamp(ssim_of(da.data(), db.data(), da.size(), true), -1.0, 1.0);
+}
+
+struct CompareGate {
+    const char *name;
+    double mae, strict_mae, strict_ssim;
+};
+
+// Scene-class gates chosen from the candidate path (metal_bridge.mm:
+// 2358-2381).
+CompareGate gate_for(const char *path) {
+    const std::string p(path ? path : "");
+    auto has = [&](const char *k) { return p.find(k) != std::string::npos; };
+    const bool preview = has("preview-parity");
+    if (preview && has("Ball_Fractal")) return {"preview_ball_fractal", 50.0, 35.0, 0.50};
+    if (preview && has("Glass_Ball")) return {"preview_glass", 60.0, 42.0, 0.45};
+    if (preview) return {"preview", 90.0, 55.0, 0.45};
+    static const CompareGate named[] = {
+        {"cornell", 50.0, 30.0, 0.55},      {"glass", 50.0, 30.0, 0.30},          {"ball_fractal", 45.0, 28.0, 0.55},
+        {"cage_fractal", 70.0, 45.0, 0.45}, {"ifs_fractal", 75.0, 48.0, 0.45},    {"mandelbox_fractal", 80.0, 52.0, 0.42},
+        {"menger_sponge", 60.0, 40.0, 0.48}, {"tower_fractal", 75.0, 48.0, 0.45}, {"tree_fractal", 60.0, 38.0, 0.48}};
+    static const char *keys[] = {"Cornell_Box", "Glass_Ball", "Ball_Fractal", "Cage_Fractal", "IFS_Fractal",
+                                 "Mandelbox_Fractal", "Menger_Sponge", "Tower_Fractal", "Tree_Fractal"};
+    for (int i = 0; i < 9; ++i)
+        if (has(keys[i])) return named[i];
+    return {"generic", 90.0, 55.0, 0.45};
+}
+
+// Three panels (baseline | candidate | difference heat map), 360 px wide
+// each, 12 px gaps on a grey 32 background (metal_bridge.mm:2315-2356).
+std::vector<uint8_t> comparison_sheet(const Image &a, const Image &b, uint32_t &ow, uint32_t &oh) {
+    const uint32_t gap = 12, pw = 360;
+    const uint32_t ph = std::max<uint32_t>(1u, uint32_t(std::lround(double(a.height) * pw / double(a.width))));
+    ow = pw * 3 + gap * 2;
+    oh = ph;
+    std::vector<uint8_t> sheet(size_t(ow) * oh * 4, 32);
+    for (uint32_t y = 0; y < oh; ++y) {
+        const uint32_t sy = std::min<uint32_t>(a.height - 1, uint32_t(uint64_t(y) * a.height / ph));
+        for (uint32_t x = 0; x < pw; ++x) {
+            const uint32_t sx = std::min<uint32_t>(a.width - 1, uint32_t(uint64_t(x) * a.width / pw));
+            const size_t src = (size_t(sy) * a.width + sx) * 4;
+            uint8_t *pa = &sheet[(size_t(y) * ow + x) * 4];
+            uint8_t *pb = &sheet[(size_t(y) * ow + pw + gap + x) * 4];
+            uint8_t *pd = &sheet[(size_t(y) * ow + 2 * (pw + gap) + x) * 4];
+            std::memcpy(pa, &a.rgba[src], 4);
+            std::memcpy(pb, &b.rgba[src], 4);
+            int d = 0;
+            for (int ch = 0; ch < 3; ++ch) d += std::abs(int(a.rgba[src + ch]) - int(b.rgba[src + ch]));
+            const uint8_t heat = uint8_t(std::min(255, d / 3 * 4));
+            pd[0] = heat;
+            pd[1] = heat / 2;
+            pd[2] = uint8_t(255 - heat);
+            pd[3] = 255;
+        }
+    }
+    return sheet;
+}
+
 // metal_bridge.mm:2302-2313: distinct colours among the first 2048 found.
 size_t unique_colour_sample(const std::vector<uint8_t> &rgba) {
     std::vector<uint32_t> colours;
@@ -1104,9 +1343,66 @@ int fpt_hip_diagnostic_render(const FptRenderConfig *config, const FptDiagnostic
 
 int fpt_compare_images(const char *baseline_path, const char *candidate_path, const char *report_path, char *error,
                        size_t error_len) {
-    (void)baseline_path; (void)candidate_path; (void)report_path;
-    set_error(error, error_len, "image comparison is not implemented yet");
-    return 1;
+    if (!report_path || !report_path[0]) {
+        set_error(error, error_len, "missing comparison report path");
+        return 1;
+    }
+    Image a, b;
+    if (!read_png(baseline_path, a, error, error_len) || !read_png(candidate_path, b, error, error_len)) return 1;
+    if (a.width != b.width || a.height != b.height) {
+        set_error(error, error_len, "dimension mismatch: baseline %ux%u candidate %ux%u", a.width, a.height, b.width,
+                  b.height);
+        return 1;
+    }
+    // metal_bridge.mm:6077-6104: RGB channel statistics.
+    const size_t pixels = size_t(a.width) * a.height;
+    double abs_sum = 0, sq_sum = 0;
+    uint32_t max_abs = 0;
+    uint64_t changed = 0;
+    for (size_t p = 0; p < pixels; ++p) {
+        bool ch_changed = false;
+        for (int ch = 0; ch < 3; ++ch) {
+            const uint32_t d = uint32_t(std::abs(int(a.rgba[p * 4 + ch]) - int(b.rgba[p * 4 + ch])));
+            abs_sum += d;
+            sq_sum += double(d) * d;
+            max_abs = std::max(max_abs, d);
+            ch_changed |= d > 0;
+        }
+        changed += ch_changed;
+    }
+    const double samples = double(pixels * 3);
+    const double mae = abs_sum / samples, rmse = std::sqrt(sq_sum / samples);
+    const double changed_pct = pixels ? 100.0 * double(changed) / double(pixels) : 0.0;
+    const size_t ua = unique_colour_sample(a.rgba), ub = unique_colour_sample(b.rgba);
+    const CompareGate gate = gate_for(candidate_path);
+    const double ssim = block_ssim(a, b), lf_ssim = low_frequency_ssim(a, b);
+    const bool recognition = ub >= 16 && mae < gate.mae;
+    const bool strict = ub >= 64 && mae < gate.strict_mae && lf_ssim >= gate.strict_ssim;
+    std::string sheet_path(report_path);
+    const size_t slash = sheet_path.find_last_of('/'), dot = sheet_path.find_last_of('.');
+    if (dot != std::string::npos && (slash == std::string::npos || dot > slash)) sheet_path.resize(dot);
+    sheet_path += ".comparison.png";
+    uint32_t sw = 0, sh = 0;
+    const std::vector<uint8_t> sheet = comparison_sheet(a, b, sw, sh);
+    if (!write_png(sheet_path.c_str(), sw, sh, sheet, error, error_len)) return 1;
+    FILE *f = std::fopen(report_path, "w");
+    if (!f) {
+        set_error(error, error_len, "failed to open report %s", report_path);
+        return 1;
+    }
+    std::fprintf(f,
+                 "{\n  \"baseline\": \"%s\",\n  \"candidate\": \"%s\",\n  \"width\": %u,\n  \"height\": %u,\n"
+                 "  \"mean_absolute_error\": %.17g,\n  \"root_mean_square_error\": %.17g,\n"
+                 "  \"luminance_ssim\": %.17g,\n  \"low_frequency_luminance_ssim\": %.17g,\n"
+                 "  \"max_absolute_error\": %u,\n  \"changed_pixel_percent\": %.17g,\n  \"gate_class\": \"%s\",\n"
+                 "  \"gate_mae_threshold\": %g,\n  \"strict_mae_threshold\": %g,\n  \"strict_ssim_threshold\": %g,\n"
+                 "  \"comparison_sheet\": \"%s\",\n  \"unique_colours\": {\n    \"baseline\": %zu,\n"
+                 "    \"candidate\": %zu\n  },\n  \"recognition_gate\": %s,\n  \"strict_gate\": %s\n}\n",
+                 baseline_path, candidate_path, a.width, a.height, mae, rmse, ssim, lf_ssim, max_abs, changed_pct,
+                 gate.name, gate.mae, gate.strict_mae, gate.strict_ssim, sheet_path.c_str(), ua, ub,
+                 recognition ? "true" : "false", strict ? "true" : "false");
+    std::fclose(f);
+    return 0;
 }
 
 int fpt_test_async_stitch_context(const char *, const char *, const char *, const FptRenderConfig *,
