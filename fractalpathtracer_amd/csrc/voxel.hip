@@ -37,6 +37,7 @@
 
 #include "fpt_device.h"
 #include "post.h"
+#include "bounds.h"
 #include "voxel_api.h"
 
 namespace fpt {
@@ -321,7 +322,12 @@ __global__ __launch_bounds__(256) void voxel_stage_kernel(const FptRenderConfig 
     const f3 c = cell_center(x, y, z, ld3(cfg.voxel_bounds_min), cs);
     Work w = {0, 0};
     const float d = distance_sdf(c, cfg, w);
-    const bool o = classify_occupied(d, length(cs * 0.5f), cfg);
+    bool o = classify_occupied(d, length(cs * 0.5f), cfg);
+    const uint32_t cov = cfg.voxel_coverage_mode;
+    if (cov == FPT_VOXEL_COVERAGE_LIPSCHITZ || cov == FPT_VOXEL_COVERAGE_INTERVAL) {  // SM:5388-5398
+        const CellClass k = classify_cell(c, cs * 0.5f, d, cfg);
+        o = cov == FPT_VOXEL_COVERAGE_LIPSCHITZ ? k.lipschitz : k.interval;
+    }
     occ[i] = o ? 1u : 0u;
     if (dist) dist[i] = __float2half_rn(d);
     if (mat) mat[i] = o ? uint8_t(palette_id(material_param(material_orbit(c, cfg, w), cfg), material_is_constant(cfg))) : 0u;
@@ -717,7 +723,23 @@ __global__ __launch_bounds__(256) void voxel_trace_cells_kernel(const FptRenderC
 // DDA known-answer contract (SM:5156-5386, the checks that concern the grid
 // DDA, slab test, cell ownership and precision offsets).
 // ---------------------------------------------------------------------------
-__global__ void voxel_dda_contract_kernel(uint32_t *failures_out) {
+// Distances of the 3x3x3 grid over the box lie in its certified bounds
+// (programBoundsContainGridSamples, SM:4292-4308).
+__device__ bool bounds_contain_grid(f3 c, f3 h, const FptRenderConfig &cfg) {
+    const ProgramBounds b = program_bounds(c, h, cfg);
+    if (!b.certified || !isfinite(b.minimum) || !isfinite(b.maximum)) return false;
+    Work w = {0, 0};
+    for (int z = 0; z < 3; ++z)
+        for (int y = 0; y < 3; ++y)
+            for (int x = 0; x < 3; ++x) {
+                const f3 u = mk(float(x) - 1.0f, float(y) - 1.0f, float(z) - 1.0f);
+                const float d = distance_sdf(c + mk(u.x * h.x, u.y * h.y, u.z * h.z), cfg, w);
+                if (!isfinite(d) || d < b.minimum - 2.0e-5f || d > b.maximum + 2.0e-5f) return false;
+            }
+    return true;
+}
+
+__global__ void voxel_dda_contract_kernel(uint32_t *failures_out, const FptRenderConfig *__restrict__ pc) {
     uint32_t f = 0u;
     const f3 bmin = mk(0.f, 0.f, 0.f), bmax = mk(4.f, 4.f, 4.f), cs = mk(1.f, 1.f, 1.f);
     i3 cell = {0, 0, 0};
@@ -751,6 +773,42 @@ __global__ void voxel_dda_contract_kernel(uint32_t *failures_out) {
     if (!(po.x > 1.0f) || po.y != -1.0f || !(ne.x < 1.0f) || ne.y != -1.0f) f |= 1u << 27;
     f3 oo = precision_offset(mk(0.f, 0.f, 0.f), mk(1.f, -1.f, 0.f));
     if (!(oo.x > 0.0f) || !(oo.y < 0.0f) || oo.z != 0.0f) f |= 1u << 28;
+
+    // Bounds and cell classification (SM:5211-5298): pc[0..8] are the
+    // contract programs (sphere, box, plane, repeat, transformed CSG, ...).
+    if (CELL_OUTSIDE != 0u || CELL_INSIDE != 1u || CELL_SURFACE != 2u || CELL_UNKNOWN != 3u) f |= 1u << 10;
+    auto cls = [&](f3 c, f3 h, const FptRenderConfig &cfg) {
+        Work w = {0, 0};
+        return classify_cell(c, h, distance_sdf(c, cfg, w), cfg);
+    };
+    const f3 q = mk(0.25f, 0.25f, 0.25f), z0 = mk(0.f, 0.f, 0.f), t1 = mk(0.1f, 0.1f, 0.1f);
+    const ProgramBounds sb = program_bounds(z0, q, pc[0]);
+    const CellClass s_in = cls(z0, q, pc[0]), s_out = cls(mk(2.f, 0.f, 0.f), q, pc[0]);
+    const CellClass s_surf = cls(mk(1.f, 0.f, 0.f), t1, pc[0]);
+    if (!sb.certified || sb.lipschitz != 1.0f || !s_in.certified || s_in.cell_class != CELL_INSIDE ||
+        s_out.cell_class != CELL_OUTSIDE || s_surf.cell_class != CELL_SURFACE || !bounds_contain_grid(z0, q, pc[0]))
+        f |= 1u << 11;
+    if (!s_surf.lipschitz || !s_surf.interval || s_in.lipschitz || s_in.interval) f |= 1u << 20;
+    const CellClass miss = cls(mk(0.75f, 0.75f, 0.75f), q, pc[0]);
+    if (miss.legacy || !miss.lipschitz || !miss.interval || miss.cell_class != CELL_SURFACE) f |= 1u << 21;
+    if (!bounds_contain_grid(mk(0.35f, -0.15f, 0.2f), mk(0.4f, 0.3f, 0.2f), pc[1])) f |= 1u << 12;
+    if (!program_bounds(mk(0.2f, -0.1f, 0.3f), mk(0.4f, 0.2f, 0.5f), pc[2]).certified ||
+        !bounds_contain_grid(mk(0.2f, -0.1f, 0.3f), mk(0.4f, 0.2f, 0.5f), pc[2]))
+        f |= 1u << 13;
+    if (!program_bounds(z0, mk(0.3f, 0.3f, 0.3f), pc[3]).certified ||
+        program_bounds(mk(2.f, 0.f, 0.f), t1, pc[3]).certified || !bounds_contain_grid(z0, mk(0.3f, 0.3f, 0.3f), pc[3]))
+        f |= 1u << 14;
+    if (!bounds_contain_grid(mk(0.4f, -0.2f, 0.1f), mk(0.6f, 0.5f, 0.4f), pc[4])) f |= 1u << 15;
+    if (!bounds_contain_grid(mk(0.2f, 0.f, 0.f), mk(0.8f, 0.5f, 0.4f), pc[5])) f |= 1u << 16;
+    if (!bounds_contain_grid(mk(0.4f, 0.f, 0.f), mk(1.1f, 0.5f, 0.4f), pc[6])) f |= 1u << 17;
+    const CellClass ss = cls(mk(1.f, 0.f, 0.f), t1, pc[7]);
+    if (!program_bounds(z0, q, pc[7]).certified || ss.cell_class != CELL_SURFACE || !ss.certified || !ss.legacy ||
+        !ss.lipschitz || !ss.interval || !bounds_contain_grid(z0, q, pc[7]))
+        f |= 1u << 18;
+    if (s_out.legacy || !s_surf.legacy) f |= 1u << 19;
+    if (!program_bounds(mk(0.2f, -0.35f, 0.1f), mk(0.7f, 0.45f, 0.55f), pc[8]).certified ||
+        !bounds_contain_grid(mk(0.2f, -0.35f, 0.1f), mk(0.7f, 0.45f, 0.55f), pc[8]))
+        f |= 1u << 29;
     *failures_out = f;
 }
 
@@ -800,12 +858,6 @@ int voxel_build(const FptRenderConfig &cfg, const FptRenderConfig *dcfg, VoxelSt
         return 1;
     }
     const uint32_t cov = cfg.voxel_coverage_mode;
-    const bool is_prog = cfg.sdf_id == FPT_SDF_PROGRAM && cfg.sdf_program_count > 0u;
-    if ((cov == FPT_VOXEL_COVERAGE_LIPSCHITZ || cov == FPT_VOXEL_COVERAGE_INTERVAL) && is_prog) {
-        if (error && error_len)
-            std::snprintf(error, error_len, "%s", "interval/Lipschitz voxel coverage of typed programs is not built");
-        return 1;
-    }
     if (cov > FPT_VOXEL_COVERAGE_SHELL) {
         if (error && error_len) std::snprintf(error, error_len, "voxel coverage mode %u is not supported", cov);
         return 1;
@@ -1041,17 +1093,69 @@ hipError_t launch_voxel_trace_cells(const FptRenderConfig *dcfg, const VoxelStor
 
 }  // namespace fpt
 
+namespace {
+// The typed programs of the reference's bounds contract
+// (metal_bridge.mm:3748-3790): every config a zero-initialised typed-program
+// scene with surface band 0.5.
+void contract_programs(FptRenderConfig *pc) {
+    std::memset(pc, 0, 9 * sizeof(FptRenderConfig));
+    for (int i = 0; i < 9; ++i) {
+        pc[i].sdf_id = FPT_SDF_PROGRAM;
+        pc[i].voxel_surface_band = 0.5f;
+    }
+    auto add = [](FptRenderConfig &c, uint32_t op, uint32_t flags, float x, float y, float z, float w) {
+        FptSdfInstruction &in = c.sdf_program[c.sdf_program_count++];
+        in.opcode = op;
+        in.flags = flags;
+        in.data[0] = x; in.data[1] = y; in.data[2] = z; in.data[3] = w;
+    };
+    add(pc[0], FPT_SDF_OP_SPHERE, 0, 1.0f, 0, 0, 0);
+    add(pc[1], FPT_SDF_OP_BOX, 0, 0.8f, 0.6f, 0.4f, 0);
+    add(pc[2], FPT_SDF_OP_PLANE, 0, 1.0f, 2.0f, -0.5f, 0.2f);
+    add(pc[3], FPT_SDF_OP_REPEAT, 0, 4.0f, 4.0f, 4.0f, 0);
+    add(pc[3], FPT_SDF_OP_SPHERE, 0, 1.0f, 0, 0, 0);
+    add(pc[4], FPT_SDF_OP_ABS, 0, 0, 0, 0, 0);
+    add(pc[4], FPT_SDF_OP_ROTATE_X, 0, 0.31f, 0, 0, 0);
+    add(pc[4], FPT_SDF_OP_ROTATE_Y, 0, -0.47f, 0, 0, 0);
+    add(pc[4], FPT_SDF_OP_ROTATE_Z, 0, 0.19f, 0, 0, 0);
+    add(pc[4], FPT_SDF_OP_SCALE, 0, 1.25f, 0, 0, 0);
+    add(pc[4], FPT_SDF_OP_TRANSLATE, 0, 0.2f, -0.1f, 0.3f, 0);
+    add(pc[4], FPT_SDF_OP_BOX, 0, 1.0f, 0.8f, 0.6f, 0);
+    add(pc[4], FPT_SDF_OP_SPHERE, 2, 0.35f, 0, 0, 0);
+    add(pc[5], FPT_SDF_OP_SPHERE, 0, 1.0f, 0, 0, 0);
+    add(pc[5], FPT_SDF_OP_TRANSLATE, 0, 0.5f, 0, 0, 0);
+    add(pc[5], FPT_SDF_OP_BOX, 1, 0.7f, 0.7f, 0.7f, 0);
+    add(pc[6], FPT_SDF_OP_SPHERE, 0, 0.7f, 0, 0, 0);
+    add(pc[6], FPT_SDF_OP_TRANSLATE, 0, 0.9f, 0, 0, 0);
+    add(pc[6], FPT_SDF_OP_SPHERE, 0, 0.7f, 0, 0, 0);
+    add(pc[7], FPT_SDF_OP_SORT_DESC, 0, 0, 0, 0, 0);
+    add(pc[7], FPT_SDF_OP_SPHERE, 0, 1.0f, 0, 0, 0);
+    add(pc[8], FPT_SDF_OP_SORT_DESC, 0, 0, 0, 0, 0);
+    add(pc[8], FPT_SDF_OP_BOX, 0, 0.9f, 0.55f, 0.3f, 0);
+}
+}  // namespace
+
+// Runs the known-answer checks of the reference's contract (Shaders.metal:
+// 5156-5386) that this library implements: bits 0-21, 27-29.  Bits 22-26
+// (voxel leaf refinement), 30 (Cage-fractal bounds) and 31 (derivative
+// bounds) belong to features this library does not build.
 extern "C" int fpt_test_voxel_dda(const char *metallib_path, char *error, size_t error_len) {
     (void)metallib_path;
+    std::vector<FptRenderConfig> pc(9);
+    contract_programs(pc.data());
     uint32_t *d = nullptr;
+    FptRenderConfig *dpc = nullptr;
     hipError_t e = hipMalloc(&d, 4);
+    if (e == hipSuccess) e = hipMalloc(&dpc, pc.size() * sizeof(FptRenderConfig));
+    if (e == hipSuccess) e = hipMemcpy(dpc, pc.data(), pc.size() * sizeof(FptRenderConfig), hipMemcpyHostToDevice);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(fpt::voxel_dda_contract_kernel, dim3(1), dim3(1), 0, 0, d);
+        hipLaunchKernelGGL(fpt::voxel_dda_contract_kernel, dim3(1), dim3(1), 0, 0, d, dpc);
         e = hipGetLastError();
     }
     uint32_t f = 0xffffffffu;
     if (e == hipSuccess) e = hipMemcpy(&f, d, 4, hipMemcpyDeviceToHost);
     if (d) (void)hipFree(d);
+    if (dpc) (void)hipFree(dpc);
     if (e != hipSuccess) {
         if (error && error_len) std::snprintf(error, error_len, "voxel DDA contract could not run: %s", hipGetErrorString(e));
         return 1;

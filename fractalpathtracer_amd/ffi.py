@@ -59,6 +59,8 @@ VOXEL_STORAGE_SPARSE_BRICKS = 1
 VOXEL_STORAGE_SPARSE_BRICKS8 = 8
 
 VOXEL_COVERAGE_LEGACY = 0
+VOXEL_COVERAGE_LIPSCHITZ = 1
+VOXEL_COVERAGE_INTERVAL = 2
 VOXEL_COVERAGE_SHELL = 3
 
 u32 = C.c_uint32

@@ -35,6 +35,8 @@ def lib():
         P = C.POINTER
         dp = P(C.c_double)
         L.orc_render.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_uint32, C.c_uint32, dp]
+        L.orc_program_bounds.argtypes = [C.c_void_p, C.c_int, dp, dp, dp]
+        L.orc_classify_cell.argtypes = [C.c_void_p, C.c_int, dp, dp, dp]
         L.orc_present.argtypes = [C.c_void_p, C.c_int, P(C.c_float), P(C.c_uint8)]
         L.orc_diagnostic.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_int, C.c_int, P(C.c_uint8)]
         L.orc_render_samples.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32,
@@ -167,6 +169,23 @@ def render_samples(cfg, samples, rows=None, precision: int = 64, threads: int = 
     if rc != 0:
         raise RuntimeError("orc_render_samples failed")
     return out
+
+
+def program_bounds(cfg, center, half, precision: int = 64) -> dict:
+    c = np.ascontiguousarray(center, np.float64)
+    h = np.ascontiguousarray(half, np.float64)
+    out = np.zeros(4)
+    lib().orc_program_bounds(C.addressof(cfg), precision, _dp(c), _dp(h), _dp(out))
+    return {"minimum": out[0], "maximum": out[1], "lipschitz": out[2], "certified": bool(out[3])}
+
+
+def classify_cell(cfg, center, half, precision: int = 64) -> dict:
+    c = np.ascontiguousarray(center, np.float64)
+    h = np.ascontiguousarray(half, np.float64)
+    out = np.zeros(7)
+    lib().orc_classify_cell(C.addressof(cfg), precision, _dp(c), _dp(h), _dp(out))
+    return {"interval_min": out[0], "interval_max": out[1], "certified": bool(out[2]), "legacy": bool(out[3]),
+            "lipschitz": bool(out[4]), "interval": bool(out[5]), "cell_class": int(out[6])}
 
 
 def present(cfg, accum, precision: int = 64) -> np.ndarray:

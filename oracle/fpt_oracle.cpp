@@ -1192,6 +1192,7 @@ void trace_primary(const FptRenderConfig &cfg, int threads, double *out) {
     }
 }
 
+#include "fpt_oracle_bounds.inc"
 #include "fpt_oracle_voxel.inc"
 #include "fpt_oracle_diag.inc"
 
@@ -1293,6 +1294,38 @@ void orc_mandelbox_step(double s[5], double scale, double min_r2, double fixed_r
     double dr = s[4];
     mandelbox_kernel<double, double>(z, dr, scale, min_r2, fixed_r2);
     s[0] = z.x; s[1] = z.y; s[2] = z.z; s[3] = length(z); s[4] = dr;
+}
+
+int orc_program_bounds(const FptRenderConfig *cfg, int precision, const double center[3], const double half[3],
+                       double out[4]) {
+    if (!cfg || !center || !half || !out) return 1;
+    if (precision == 64) {
+        BoundsT<double> b = program_bounds<double>(V3<double>(center[0], center[1], center[2]),
+                                                   V3<double>(half[0], half[1], half[2]), *cfg);
+        out[0] = b.minimum; out[1] = b.maximum; out[2] = b.lipschitz; out[3] = b.certified ? 1.0 : 0.0;
+    } else {
+        BoundsT<float> b = program_bounds<float>(V3<float>(float(center[0]), float(center[1]), float(center[2])),
+                                                 V3<float>(float(half[0]), float(half[1]), float(half[2])), *cfg);
+        out[0] = b.minimum; out[1] = b.maximum; out[2] = b.lipschitz; out[3] = b.certified ? 1.0 : 0.0;
+    }
+    return 0;
+}
+
+int orc_classify_cell(const FptRenderConfig *cfg, int precision, const double center[3], const double half[3],
+                      double out[7]) {
+    if (!cfg || !center || !half || !out) return 1;
+    CellClassT r;
+    if (precision == 64) {
+        V3<double> c{center[0], center[1], center[2]};
+        r = classify_cell<double>(c, V3<double>(half[0], half[1], half[2]), distance_sdf<double>(c, *cfg), *cfg);
+    } else {
+        V3<float> c{float(center[0]), float(center[1]), float(center[2])};
+        r = classify_cell<float>(c, V3<float>(float(half[0]), float(half[1]), float(half[2])),
+                                 distance_sdf<float>(c, *cfg), *cfg);
+    }
+    out[0] = r.interval_min; out[1] = r.interval_max; out[2] = r.certified; out[3] = r.legacy;
+    out[4] = r.lipschitz; out[5] = r.interval; out[6] = double(r.cell_class);
+    return 0;
 }
 
 int orc_present(const FptRenderConfig *cfg, int precision, const float *accum, uint8_t *rgba) {

@@ -72,6 +72,17 @@ void orc_mandelbulb_step(double state[5], double power);
 void orc_mandelbox_step(double state[5], double scale, double min_r2,
                         double fixed_r2);
 
+/* Certified distance interval of the typed program over the box centre +-
+ * half (programBounds, Shaders.metal:1134-1273): out = {min, max, lipschitz,
+ * certified}. */
+int orc_program_bounds(const struct FptRenderConfig *cfg, int precision, const double center[3],
+                       const double half[3], double out[4]);
+
+/* classifyAccelerationCell (Shaders.metal:4196-4244): out = {interval_min,
+ * interval_max, certified, legacy, lipschitz, interval, class}. */
+int orc_classify_cell(const struct FptRenderConfig *cfg, int precision, const double center[3],
+                      const double half[3], double out[7]);
+
 /* Present (post-process, chromatic aberration, highlight bloom, vertical
  * flip; Shaders.metal:6778-6825, 7462-7507) of a float4 accumulation buffer
  * (width*height*4 floats, row 0 at the top of the scene) into RGBA8. */

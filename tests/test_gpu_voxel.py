@@ -45,6 +45,38 @@ def test_tables_are_bit_exact(storage, coverage):
         assert np.array_equal(g["palette"], c["palette"])
 
 
+def _program_cfg(storage, coverage, res=40):
+    """Box minus sphere through abs/scale/translate/sort (no transcendentals,
+    so the distance and its certified bounds are IEEE-exact on both sides)."""
+    c = scenes.voxel_mandelbox_scene(64, 48, 1, resolution=res)
+    scenes.set_program(c, [(ffi.OP_ABS, 0, []), (ffi.OP_SCALE, 0, [1.25]), (ffi.OP_TRANSLATE, 0, [0.2, -0.1, 0.3]),
+                           (ffi.OP_SORT_DESC, 0, []), (ffi.OP_BOX, 0, [1.0, 0.8, 0.6]), (ffi.OP_SPHERE, 2, [0.9])])
+    for i, v in enumerate([-2.0, -2.0, -2.0]):
+        c.voxel_bounds_min[i] = v
+    for i, v in enumerate([2.0, 2.0, 2.0]):
+        c.voxel_bounds_max[i] = v
+    c.voxel_storage = storage
+    c.voxel_coverage_mode = coverage
+    return c
+
+
+@pytest.mark.parametrize("storage", [ffi.VOXEL_STORAGE_DENSE, ffi.VOXEL_STORAGE_SPARSE_BRICKS,
+                                     ffi.VOXEL_STORAGE_SPARSE_BRICKS8])
+@pytest.mark.parametrize("coverage", [ffi.VOXEL_COVERAGE_LEGACY, ffi.VOXEL_COVERAGE_LIPSCHITZ,
+                                      ffi.VOXEL_COVERAGE_INTERVAL])
+def test_program_coverage_tables_are_bit_exact(storage, coverage):
+    cfg = _program_cfg(storage, coverage)
+    g = api.voxel_build(cfg)
+    c = bind.voxel_build(cfg)
+    print("coverage", coverage, "cells", g["active_cells"], "bricks", g["active_bricks"])
+    for k in ("active_bricks", "active_cells"):
+        assert g[k] == c[k], k
+    assert np.array_equal(g["page_table"], c["page_table"])
+    assert g["records"].tobytes() == c["records"].tobytes()
+    assert np.array_equal(g["material"], c["material"])
+    assert np.array_equal(g["distance"], c["distance"])
+
+
 def test_tables_bit_exact_at_256_cubed():
     cfg = _cfg(ffi.VOXEL_STORAGE_SPARSE_BRICKS8, 3, 256)
     g = api.voxel_build(cfg)
