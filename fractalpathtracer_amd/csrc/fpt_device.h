@@ -981,4 +981,96 @@ __device__ f3 environment(f3 view, const FptRenderConfig &cfg) {  // SM:5823-584
     return vmax0(env * cfg.world[4] - mk(0.5f, 0.5f, 0.5f) * contrast + mk(0.5f, 0.5f, 0.5f));
 }
 
+// ---------------------------------------------------------------------------
+// Mandelbulber native field: Kaleidoscopic IFS, formula 10
+// (Shaders.metal:786-949; CPU form src/mandelbulber/orbit.rs:699-750 with
+// distances 0, intensities 1, no per-plane rotation, Ifs finalizer
+// orbit.rs:662).  Parameters: set_values[0] world scale, [1] iterations,
+// [2] bailout, [3] IFS scale, [4] abs mask, [5] plane mask, [6..8] offset,
+// [9..11] rotation, [12 + 3k ..] plane k direction (k < 9); vset_values
+// [115..119, 130] the screen iteration budget, [120..128] the global
+// offset / rotation / repeat.
+// ---------------------------------------------------------------------------
+// Rz(r.x) Ry(r.y) Rx(r.z) (Mandelbulber SetRotation3), applied x first.
+__device__ __forceinline__ f3 mb_rotation3(f3 v, f3 r) {
+    float s, c;
+    sincosf(r.z, &s, &c);
+    v = mk(v.x, v.y * c - v.z * s, v.y * s + v.z * c);
+    sincosf(r.y, &s, &c);
+    v = mk(v.x * c + v.z * s, v.y, -v.x * s + v.z * c);
+    sincosf(r.x, &s, &c);
+    return mk(v.x * c - v.y * s, v.x * s + v.y * c, v.z);
+}
+// Rz(r.z) Ry(r.y) Rx(r.x) (SetRotation2).
+__device__ __forceinline__ f3 mb_rotation2(f3 v, f3 r) {
+    return mb_rotation3(v, mk(r.z, r.y, r.x));
+}
+__device__ __forceinline__ float mb_repeat(float v, float period) {
+    if (!(period > 0.0f)) return v;
+    return v - period * floorf((v + 0.5f * period) / period);
+}
+
+__device__ __forceinline__ f3 mb_global_point(f3 p, const FptRenderConfig &cfg) {  // SM:827-843
+    const float ws = fmaxf(cfg.set_values[0], 1.0f);
+    const float *v = cfg.vset_values;
+    f3 q = mk(p.x, p.z, p.y) / ws - mk(v[120], v[121], v[122]);
+    q = mb_rotation2(q, mk(v[123], v[124], v[125]));
+    q = mk(mb_repeat(q.x, v[126]), mb_repeat(q.y, v[127]), mb_repeat(q.z, v[128]));
+    return mk(q.x, q.z, q.y) * ws;
+}
+
+__device__ __forceinline__ int mb_iteration_budget(f3 world, const FptRenderConfig &cfg, int mult) {  // SM:863-884
+    const float *v = cfg.vset_values;
+    int configured = int(cfg.set_values[1]) * (mult > 1 ? mult : 1);
+    configured = configured < 1 ? 1 : (configured > 4096 ? 4096 : configured);
+    const float rate = v[130];
+    if (!(rate > 0.0f) || !isfinite(rate)) return configured;
+    float fp = v[115] > 0.5f ? length(ld3(cfg.camera_position) - world) * v[116] : v[117];
+    fp = clampf(fp, v[118], v[119]);
+    const float minimum = fmaxf(v[118], 1.0e-12f);
+    const float octaves = fmaxf(log2f(fmaxf(fp / minimum, 1.0f)), 0.0f);
+    const int reduction = int(floorf(octaves * rate));
+    int floor_budget = int(ceilf(float(configured) * 0.75f));
+    floor_budget = floor_budget > 1 ? floor_budget : 1;
+    return configured - reduction > floor_budget ? configured - reduction : floor_budget;
+}
+
+// {distance, radius, derivative, +-iterations (negative = escaped)}.
+__device__ float4 kifs_field_sample(f3 p, const FptRenderConfig &cfg, int mult) {  // SM:897-949
+    const float *sv = cfg.set_values;
+    const int budget = mb_iteration_budget(p, cfg, mult);
+    p = mb_global_point(p, cfg);
+    const float ws = fmaxf(sv[0], 1.0f);
+    const f3 sc = p / ws;
+    f3 z = mk(sc.x, sc.z, sc.y);
+    float der = 1.0f, r = length(z);
+    int done = 0;
+    bool escaped = false;
+    const float bailout = fmaxf(sv[2], 1.0f), k = fmaxf(sv[3], 1.000001f);
+    const uint32_t abs_mask = uint32_t(rintf(sv[4])), plane_mask = uint32_t(rintf(sv[5]));
+    const f3 off = mk(sv[6], sv[7], sv[8]), rot = mk(sv[9], sv[10], sv[11]);
+    for (int it = 0; it < budget; ++it) {
+        if (abs_mask & 1u) z.x = fabsf(z.x);
+        if (abs_mask & 2u) z.y = fabsf(z.y);
+        if (abs_mask & 4u) z.z = fabsf(z.z);
+        for (uint32_t pl = 0; pl < 9u; ++pl) {
+            if (!(plane_mask & (1u << pl))) continue;
+            const f3 dir = mk(sv[12 + 3 * pl], sv[13 + 3 * pl], sv[14 + 3 * pl]);
+            const float proj = dot(z, dir);
+            if (proj < 0.0f) z = z - dir * (2.0f * proj);
+        }
+        z = mb_rotation3(z - off, rot) + off;
+        z = z * k - off * (k - 1.0f);
+        der *= fabsf(k);
+        r = length(z);
+        done = it + 1;
+        if (r > bailout) {
+            escaped = true;
+            break;
+        }
+    }
+    const float d = (r - 2.0f) / fmaxf(fabsf(der), 1.0e-30f) * ws;
+    return make_float4(d, r, der, escaped ? -float(done) : float(done));
+}
+
 }  // namespace fpt

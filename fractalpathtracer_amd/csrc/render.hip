@@ -209,8 +209,10 @@ __global__ __launch_bounds__(256) void trace_primary_kernel(const FptRenderConfi
     hits[size_t(y) * cfg.width + x] = h;
 }
 
-// Distance field samples: {distance, radius, derivative, iterations} of the
-// first fractal leaf of the program (or the scene distance alone).
+// Distance field samples {distance, radius, derivative, iterations}: the
+// Mandelbulber native KIFS field for sdf_id MANDELBULBER, otherwise the
+// scene distance with the orbit statistics of the program's first fractal
+// leaf.
 __global__ __launch_bounds__(256) void field_sample_kernel(const FptRenderConfig *__restrict__ cfgp,
                                                            const float4 *__restrict__ pts, size_t n,
                                                            FptMandelbulberFieldSample *__restrict__ out) {
@@ -221,6 +223,15 @@ __global__ __launch_bounds__(256) void field_sample_kernel(const FptRenderConfig
     f3 p = mk(q.x, q.y, q.z);
     Work w = {0, 0};
     FptMandelbulberFieldSample s;
+    if (cfg.sdf_id == FPT_SDF_MANDELBULBER) {  // mandelbulber_field_sample_kernel (SM:1001-1008)
+        const float4 k = kifs_field_sample(p, cfg, 1);
+        s.distance = k.x;
+        s.radius = k.y;
+        s.derivative = k.z;
+        s.iterations = fabsf(k.w);
+        out[i] = s;
+        return;
+    }
     s.distance = map_sdf(p, cfg, w);
     s.radius = 0.f;
     s.derivative = 0.f;

@@ -35,6 +35,7 @@ def lib():
         P = C.POINTER
         dp = P(C.c_double)
         L.orc_render.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_uint32, C.c_uint32, dp]
+        L.orc_kifs_sample.argtypes = [C.c_void_p, C.c_int, dp, C.c_size_t, dp]
         L.orc_program_bounds.argtypes = [C.c_void_p, C.c_int, dp, dp, dp]
         L.orc_classify_cell.argtypes = [C.c_void_p, C.c_int, dp, dp, dp]
         L.orc_present.argtypes = [C.c_void_p, C.c_int, P(C.c_float), P(C.c_uint8)]
@@ -168,6 +169,16 @@ def render_samples(cfg, samples, rows=None, precision: int = 64, threads: int = 
                                   _dp(out))
     if rc != 0:
         raise RuntimeError("orc_render_samples failed")
+    return out
+
+
+def kifs_sample(cfg, points, precision: int = 64) -> np.ndarray:
+    """(n, 4) {distance, radius, derivative, +-iterations} of the native KIFS
+    field."""
+    p = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+    out = np.zeros((len(p), 4))
+    if lib().orc_kifs_sample(C.addressof(cfg), precision, _dp(p), len(p), _dp(out)) != 0:
+        raise RuntimeError("orc_kifs_sample failed")
     return out
 
 

@@ -1193,6 +1193,7 @@ void trace_primary(const FptRenderConfig &cfg, int threads, double *out) {
 }
 
 #include "fpt_oracle_bounds.inc"
+#include "fpt_oracle_kifs.inc"
 #include "fpt_oracle_voxel.inc"
 #include "fpt_oracle_diag.inc"
 
@@ -1325,6 +1326,21 @@ int orc_classify_cell(const FptRenderConfig *cfg, int precision, const double ce
     }
     out[0] = r.interval_min; out[1] = r.interval_max; out[2] = r.certified; out[3] = r.legacy;
     out[4] = r.lipschitz; out[5] = r.interval; out[6] = double(r.cell_class);
+    return 0;
+}
+
+int orc_kifs_sample(const FptRenderConfig *cfg, int precision, const double *points, size_t n, double *out) {
+    if (!cfg || (n && (!points || !out))) return 1;
+    for (size_t i = 0; i < n; ++i) {
+        const double *q = points + 3 * i;
+        if (precision == 64) {
+            kifs_field_sample<double>(V3<double>(q[0], q[1], q[2]), *cfg, 1, out + 4 * i);
+        } else {
+            float o[4];
+            kifs_field_sample<float>(V3<float>{float(q[0]), float(q[1]), float(q[2])}, *cfg, 1, o);
+            for (int k = 0; k < 4; ++k) out[4 * i + k] = o[k];
+        }
+    }
     return 0;
 }
 

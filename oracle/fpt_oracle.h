@@ -72,6 +72,12 @@ void orc_mandelbulb_step(double state[5], double power);
 void orc_mandelbox_step(double state[5], double scale, double min_r2,
                         double fixed_r2);
 
+/* Mandelbulber native Kaleidoscopic IFS field (Shaders.metal:827-949) at n
+ * points: out[4i..4i+3] = {distance, radius, derivative, +-iterations}
+ * (negative = escaped). */
+int orc_kifs_sample(const struct FptRenderConfig *cfg, int precision, const double *points, size_t n,
+                    double *out);
+
 /* Certified distance interval of the typed program over the box centre +-
  * half (programBounds, Shaders.metal:1134-1273): out = {min, max, lipschitz,
  * certified}. */
