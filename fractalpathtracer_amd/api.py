@@ -21,6 +21,7 @@ class RenderResult:
     build_ms: float
     profile: dict = field(default_factory=dict)
     voxel: dict = field(default_factory=dict)
+    stitch: dict = field(default_factory=dict)
 
 
 def render_full(cfg: FptRenderConfig, output_png: str | None = None, profile: bool = False) -> RenderResult:
@@ -35,19 +36,26 @@ def render_full(cfg: FptRenderConfig, output_png: str | None = None, profile: bo
     vcells = C.c_uint64(0)
     vrej = C.c_uint32(0)
     prof = ffi.FptSdfProfileStats()
+    cache_status = C.c_uint32(0)
+    vstats = ffi.FptStitchValidationStats()
+    pstats = ffi.FptStitchPipelineStats()
     err = ffi.error_buffer()
     rc = lib.fpt_metal_render(None, None, None, output_png.encode() if output_png else None, None, 0,
                               C.byref(cfg), C.byref(build_ms), C.byref(elapsed), C.byref(vbytes),
-                              C.byref(vbricks), C.byref(vcells), C.byref(vrej), None, None, None, None,
+                              C.byref(vbricks), C.byref(vcells), C.byref(vrej), None, C.byref(cache_status),
+                              C.byref(vstats), C.byref(pstats),
                               C.byref(prof) if profile else None,
                               out.ctypes.data_as(C.POINTER(C.c_float)), n, err, len(err))
     ffi.check(rc, err)
+    stitch = {"cache_status": cache_status.value,
+              "validation": {k: getattr(vstats, k) for k, _ in ffi.FptStitchValidationStats._fields_},
+              "pipeline": {k: getattr(pstats, k) for k, _ in ffi.FptStitchPipelineStats._fields_}}
     pdict = {}
     if profile:
         pdict = {k: getattr(prof, k) for k, _ in ffi.FptSdfProfileStats._fields_[:13]}
     return RenderResult(out.reshape(cfg.height, cfg.width, 4), elapsed.value, build_ms.value, pdict,
                         dict(memory_bytes=vbytes.value, active_bricks=vbricks.value,
-                             active_cells=vcells.value))
+                             active_cells=vcells.value), stitch)
 
 
 def render(cfg: FptRenderConfig) -> np.ndarray:
@@ -146,6 +154,17 @@ def test_voxel_dda() -> None:
     lib = ffi.load_library()
     err = ffi.error_buffer(512)
     ffi.check(lib.fpt_test_voxel_dda(None, err, len(err)), err)
+
+
+def topology_source(cfg: FptRenderConfig) -> str:
+    """Generated topology-specialised source of cfg's program."""
+    lib = ffi.load_library()
+    need = C.c_size_t(0)
+    err = ffi.error_buffer()
+    ffi.check(lib.fpt_hip_topology_source(C.byref(cfg), None, 0, C.byref(need), err, len(err)), err)
+    buf = C.create_string_buffer(need.value)
+    ffi.check(lib.fpt_hip_topology_source(C.byref(cfg), buf, need.value, None, err, len(err)), err)
+    return buf.value.decode()
 
 
 def proposer_samples(cfg: FptRenderConfig, count: int, seed: int = 0, box_half: float = 1.5,
@@ -376,6 +395,15 @@ class Plan:
         st = ffi.FptProposerSettings(box_half, switch_distance, resume_distance, step_scale)
         ffi.check(self.lib.fpt_hip_plan_set_proposer(self.handle, w.ctypes.data_as(C.POINTER(C.c_float)), w.size,
                                                      C.byref(st), err, len(err)), err)
+
+    def specialize(self, enable: bool = True) -> dict:
+        """Compile (or fetch from the cache), validate and attach the
+        topology-specialised kernel (fpt_hip_plan_specialize); the stats
+        are returned, and a failed validation raises FptError."""
+        st = ffi.FptSpecializationStats()
+        err = ffi.error_buffer()
+        ffi.check(self.lib.fpt_hip_plan_specialize(self.handle, int(enable), C.byref(st), err, len(err)), err)
+        return {k: getattr(st, k) for k, _ in ffi.FptSpecializationStats._fields_}
 
     def profile(self, buffer: "DeviceBuffer") -> tuple[dict, dict]:
         """Counting pass of the full frame: (profile counters, learned-step

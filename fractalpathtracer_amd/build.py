@@ -42,15 +42,51 @@ def sources():
     return sorted(CSRC.glob("*.hip"))
 
 
+# Device headers the runtime compiler needs for topology-specialised kernels,
+# under the include names the sources use.
+EMBEDDED_HEADERS = [
+    ("fpt_device.h", CSRC / "fpt_device.h"),
+    ("fpt_internal.h", CSRC / "fpt_internal.h"),
+    ("learned.h", CSRC / "learned.h"),
+    ("persistent_kernel.h", CSRC / "persistent_kernel.h"),
+    ("../../include/fpt_abi.h", PKG.parent / "include" / "fpt_abi.h"),
+]
+
+
+def write_embedded_headers() -> Path:
+    """_build/embedded_headers.h: the headers as string constants."""
+    out = ["// Generated by build.py: device headers embedded for the runtime compiler.", "#pragma once",
+           "#include <stddef.h>", "namespace fpt {"]
+    names, texts = [], []
+    for i, (name, path) in enumerate(EMBEDDED_HEADERS):
+        text = path.read_text()
+        # split into raw-literal chunks (compiler string-literal limits)
+        chunks = [text[j:j + 16000] for j in range(0, len(text), 16000)] or [""]
+        lit = "\n".join(f'R"FPTHDR({c})FPTHDR"' for c in chunks)
+        out.append(f"static const char kEmbeddedHeader{i}[] = {lit};")
+        names.append(f'"{name}"')
+        texts.append(f"kEmbeddedHeader{i}")
+    out.append(f"static const size_t kEmbeddedHeaderCount = {len(EMBEDDED_HEADERS)};")
+    out.append("static const char *kEmbeddedHeaderNames[] = {" + ", ".join(names) + "};")
+    out.append("static const char *kEmbeddedHeaderText[] = {" + ", ".join(texts) + "};")
+    out.append("}  // namespace fpt")
+    path = BUILD / "embedded_headers.h"
+    new = "\n".join(out) + "\n"
+    if not path.exists() or path.read_text() != new:
+        path.write_text(new)
+    return path
+
+
 def build(verbose: bool = False, jobs: int | None = None) -> Path:
     """Compile every csrc/*.hip for gfx950 and link libfpt_hip.so."""
     BUILD.mkdir(exist_ok=True)
+    write_embedded_headers()
     cc = hipcc()
     srcs = sources()
 
     def compile_one(src: Path) -> Path:
         obj = BUILD / (src.stem + ".o")
-        cmd = [cc, *COMMON, "-c", str(src), "-o", str(obj)]
+        cmd = [cc, *COMMON, "-I", str(BUILD), "-c", str(src), "-o", str(obj)]
         if verbose:
             print(" ".join(cmd), flush=True)
         r = subprocess.run(cmd, capture_output=True, text=True)
@@ -62,7 +98,7 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
     with ThreadPoolExecutor(max_workers=n) as ex:
         objs = list(ex.map(compile_one, srcs))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [cc, "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp), "-lz"]
+    cmd = [cc, "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp), "-lz", "-lhiprtc"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-8000:]}")

@@ -19,6 +19,7 @@
 #include "../../include/fpt_abi.h"
 #include "fpt_internal.h"
 #include "learned.h"
+#include "specialize.h"
 #include "voxel_api.h"
 
 namespace {
@@ -434,6 +435,9 @@ struct FptHipPlan {
     uint32_t event_batch = 24;   // waiting lanes before per-hit work runs
     uint32_t *counter = nullptr; // persistent pixel counter
     fpt::ProposerHost proposer;  // learned stepping (fpt_hip_plan_set_proposer)
+    // Topology-specialised persistent kernel (fpt_hip_plan_specialize).
+    std::shared_ptr<fpt::SpecializedModule> spec;
+    fpt::SpecializeReport spec_report;
 };
 
 namespace {
@@ -452,8 +456,61 @@ hipError_t launch_sdf(FptHipPlan *plan, float4 *acc, uint32_t start, uint32_t ch
         if (e != hipSuccess) return e;
     }
     uint32_t wg = plan->workgroups ? plan->workgroups : default_workgroups();
+    if (plan->spec && !plan->proposer.enabled && !prof) {
+        // Same kernel, built for the program's topology (counting passes use
+        // the interpreter: the work counts are the same).
+        hipError_t e = fpt::launch_zero_counter(plan->counter, s);
+        if (e != hipSuccess) return e;
+        return fpt::launch_specialized(*plan->spec, plan->dcfg, acc, start, chunk, plan->counter,
+                                       fpt::persistent_slots(c.width, c.height), wg, int(plan->event_batch), s);
+    }
     return fpt::launch_sdf_persistent(plan->dcfg, c, acc, start, chunk, plan->counter, wg, int(plan->event_batch),
                                       prof, plan->proposer.enabled ? &plan->proposer.dev : nullptr, s);
+}
+
+// Attach a validated topology-specialised kernel to `plan`
+// (2^20 validation points; fails without attaching when any point fails).
+int plan_specialize(FptHipPlan *plan, char *error, size_t error_len) {
+    std::shared_ptr<fpt::SpecializedModule> mod;
+    fpt::SpecializeReport rep;
+    std::string failure;
+    if (fpt::specialize(plan->host, mod, rep, failure)) {
+        plan->spec_report = rep;
+        set_error(error, error_len, "%s", failure.c_str());
+        return 1;
+    }
+    Stream st;
+    HIP_TRY(hipStreamCreate(&st.s), "hipStreamCreate");
+    HIP_TRY(fpt::validate_specialized(*mod, plan->dcfg, 1u << 20, rep, st.s), "topology validation");
+    plan->spec_report = rep;
+    if (rep.distance_failures || rep.gradient_failures) {
+        set_error(error, error_len,
+                  "topology-specialised program failed validation: %llu distance and %llu gradient failures of %llu "
+                  "(max errors %g, %g)",
+                  (unsigned long long)rep.distance_failures, (unsigned long long)rep.gradient_failures,
+                  (unsigned long long)rep.sample_count, double(rep.max_distance_error),
+                  double(rep.max_gradient_error));
+        return 1;
+    }
+    plan->spec = mod;
+    return 0;
+}
+
+void fill_specialization_stats(const FptHipPlan *plan, FptSpecializationStats *st) {
+    const fpt::SpecializeReport &r = plan->spec_report;
+    std::memset(st, 0, sizeof(*st));
+    st->cache_status = r.cache_status;
+    st->instruction_count = std::min<uint32_t>(plan->host.sdf_program_count, FPT_SDF_PROGRAM_MAX_OPS);
+    st->source_bytes = r.source_bytes;
+    st->code_bytes = r.code_bytes;
+    st->sample_count = r.sample_count;
+    st->distance_failures = r.distance_failures;
+    st->gradient_failures = r.gradient_failures;
+    st->max_distance_error = r.max_distance_error;
+    st->max_gradient_error = r.max_gradient_error;
+    st->compile_ms = r.compile_ms;
+    st->link_ms = r.link_ms;
+    st->validate_ms = r.validate_ms;
 }
 // Samples per SDF launch.  The one-path-per-lane kernel follows the
 // reference's accumulation policy (chunk_for); the persistent kernel keeps
@@ -547,6 +604,7 @@ void fpt_hip_plan_destroy(FptHipPlan *plan) {
     if (plan->counter) (void)hipFree(plan->counter);
     fpt::voxel_free(plan->voxels);
     fpt::proposer_release(plan->proposer);
+    plan->spec.reset();
     delete plan;
 }
 
@@ -614,6 +672,50 @@ int fpt_hip_plan_profile(FptHipPlan *plan, float *device_accum, FptSdfProfileSta
         learned->proposals = hp.proposals;
         learned->far_steps = hp.far_steps;
         learned->backtracks = hp.backtracks;
+    }
+    return 0;
+}
+
+int fpt_hip_plan_specialize(FptHipPlan *plan, uint32_t enable, FptSpecializationStats *stats, char *error,
+                            size_t error_len) {
+    if (!plan) {
+        set_error(error, error_len, "missing plan");
+        return 1;
+    }
+    if (!enable) {
+        plan->spec.reset();
+        plan->spec_report = fpt::SpecializeReport();
+        if (stats) std::memset(stats, 0, sizeof(*stats));
+        return 0;
+    }
+    if (plan->host.renderer_backend != FPT_RENDERER_SDF) {
+        set_error(error, error_len, "runtime-source compilation requires the direct SDF renderer");
+        return 1;
+    }
+    const int rc = plan_specialize(plan, error, error_len);
+    if (stats) fill_specialization_stats(plan, stats);
+    return rc;
+}
+
+int fpt_hip_topology_source(const FptRenderConfig *config, char *out, size_t out_len, size_t *needed, char *error,
+                            size_t error_len) {
+    if (!config) {
+        set_error(error, error_len, "missing render config");
+        return 1;
+    }
+    std::string failure;
+    const std::string src = fpt::topology_source(*config, failure);
+    if (src.empty()) {
+        set_error(error, error_len, "%s", failure.c_str());
+        return 1;
+    }
+    if (needed) *needed = src.size() + 1;
+    if (out && out_len) {
+        if (out_len < src.size() + 1) {
+            set_error(error, error_len, "source buffer is too small (%zu < %zu)", out_len, src.size() + 1);
+            return 1;
+        }
+        std::memcpy(out, src.c_str(), src.size() + 1);
     }
     return 0;
 }
@@ -1006,6 +1108,49 @@ int fpt_metal_render(const char *metallib_path, const char *stitch_metallib_path
         ~PlanGuard() { fpt_hip_plan_destroy(p); }
     } guard{plan};
     if (build_ms) *build_ms = plan->build_ms;
+    if (config->sdf_topology_specialization != 0u) {  // metal_bridge.mm:4469-4500: runtime-source render
+        if (config->renderer_backend != FPT_RENDERER_SDF) {
+            set_error(error, error_len, "runtime-source compilation requires the direct SDF renderer");
+            return 1;
+        }
+        const int rc = plan_specialize(plan, error, error_len);
+        const fpt::SpecializeReport &r = plan->spec_report;
+        if (stitch_cache_status) *stitch_cache_status = r.cache_status;
+        if (stitch_validation_stats) {
+            stitch_validation_stats->sample_count = r.sample_count;
+            stitch_validation_stats->distance_failures = r.distance_failures;
+            stitch_validation_stats->gradient_failures = r.gradient_failures;
+            stitch_validation_stats->max_distance_error = r.max_distance_error;
+            stitch_validation_stats->max_gradient_error = r.max_gradient_error;
+        }
+        if (stitch_pipeline_stats) {
+            FptStitchPipelineStats &ps = *stitch_pipeline_stats;
+            const uint32_t n = std::min<uint32_t>(config->sdf_program_count, FPT_SDF_PROGRAM_MAX_OPS);
+            ps.instruction_count = n;
+            uint32_t prev = 0xffffffffu;
+            for (uint32_t i = 0; i < n; ++i) {
+                const FptSdfInstruction &op = config->sdf_program[i];
+                const bool prim = op.opcode == FPT_SDF_OP_SPHERE || op.opcode == FPT_SDF_OP_BOX ||
+                                  op.opcode == FPT_SDF_OP_PLANE || op.opcode == FPT_SDF_OP_MANDELBULB ||
+                                  op.opcode == FPT_SDF_OP_MANDELBOX || op.opcode == FPT_SDF_OP_MENGER;
+                if (prim) {
+                    ps.primitive_instruction_count++;
+                    if (op.opcode != prev) ps.primitive_type_runs++;
+                    prev = op.opcode;
+                    if (i) (op.flags == 1u ? ps.intersection_count : op.flags == 2u ? ps.subtraction_count
+                                                                                      : ps.union_count)++;
+                } else if (op.opcode != FPT_SDF_OP_ORBIT_ADD) {
+                    ps.transform_instruction_count++;
+                }
+            }
+            ps.thread_execution_width = 64u;
+            ps.max_total_threads_per_threadgroup = 256u;
+            ps.runtime_source_bytes = r.source_bytes;
+            ps.runtime_library_compile_ms = r.compile_ms;
+            ps.runtime_pipeline_link_ms = r.link_ms;
+        }
+        if (rc) return 1;
+    }
     if (config->renderer_backend == FPT_RENDERER_VOXEL) {
         if (voxel_memory_bytes) *voxel_memory_bytes = plan->voxels.memory_bytes;
         if (voxel_active_bricks) *voxel_active_bricks = plan->voxels.active_bricks;

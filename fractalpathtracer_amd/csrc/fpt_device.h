@@ -19,8 +19,10 @@
 //    forward-mode derivative (Dual) of exactly the function the march uses.
 #pragma once
 
+#ifndef __HIPCC_RTC__  // the runtime compiler provides these itself
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
 #include "../../include/fpt_abi.h"
 
@@ -566,124 +568,156 @@ struct Work {  // per-thread work counters (SM:7650-7704 categories)
     int orbit_iterations;
 };
 
+// Typed-program evaluation, one opcode at a time.  eval_op<OP>/surf_op<OP>
+// are the per-instruction steps of deProgramDistance / deShadingProgram
+// (SM:1882-2036) and programSurfaceInterpreted (SM:3266-3369); the
+// interpreter below dispatches to them with a switch, and a topology-
+// specialised program (runtime-compiled, see specialize.hip) calls the same
+// steps with the opcodes fixed at compile time, so both evaluate the same
+// float operations in the same order.
+struct EvalState {
+    f3 p;
+    float ds, dist, orbit;
+    bool has;
+};
+
+template <uint32_t OP>
+__device__ __forceinline__ constexpr bool op_is_transform() {
+    return OP == FPT_SDF_OP_ABS || OP == FPT_SDF_OP_TRANSLATE || OP == FPT_SDF_OP_SCALE ||
+           OP == FPT_SDF_OP_ROTATE_X || OP == FPT_SDF_OP_ROTATE_Y || OP == FPT_SDF_OP_ROTATE_Z ||
+           OP == FPT_SDF_OP_REPEAT || OP == FPT_SDF_OP_SORT_DESC || OP == FPT_SDF_OP_BOX_FOLD;
+}
+template <uint32_t OP>
+__device__ __forceinline__ constexpr bool op_is_distance() {
+    return OP == FPT_SDF_OP_SPHERE || OP == FPT_SDF_OP_BOX || OP == FPT_SDF_OP_PLANE ||
+           OP == FPT_SDF_OP_MANDELBULB || OP == FPT_SDF_OP_MANDELBOX || OP == FPT_SDF_OP_MENGER;
+}
+
+template <uint32_t OP, bool ORBIT>
+__device__ __forceinline__ void eval_op(const FptSdfInstruction &op, EvalState &s, int *orbit_counter) {
+    const float *d = op.data;
+    if constexpr (op_is_transform<OP>()) {
+        FptSdfInstruction t = op;
+        t.opcode = OP;
+        (void)apply_transform(t, s.p, s.ds);
+    } else if constexpr (OP == FPT_SDF_OP_ORBIT_ADD) {
+        if (ORBIT) s.orbit += length(s.p * ld3(d)) * d[3];
+    } else if constexpr (op_is_distance<OP>()) {
+        const float div = fmaxf(s.ds, 1.0e-6f);
+        float cand;
+        if constexpr (OP == FPT_SDF_OP_SPHERE) {
+            cand = length(s.p) - d[0];
+        } else if constexpr (OP == FPT_SDF_OP_BOX) {
+            f3 q = vabs(s.p) - vabs(ld3(d));
+            cand = fminf(fmaxf(q.x, fmaxf(q.y, q.z)), 0.0f) + length(vmax0(q));
+        } else if constexpr (OP == FPT_SDF_OP_PLANE) {
+            cand = dot(s.p, normalize(ld3(d))) + d[3];
+        } else {
+            Leaf<float> lf = eval_leaf<float>(OP, v3<float>{s.p.x, s.p.y, s.p.z}, d, orbit_counter);
+            cand = lf.d;
+            if (ORBIT) s.orbit += lf.trap;
+        }
+        cand = cand / div;
+        s.dist = s.has ? combine(s.dist, cand, op.flags) : cand;
+        s.has = true;
+        if (ORBIT && (OP == FPT_SDF_OP_SPHERE || OP == FPT_SDF_OP_BOX)) s.orbit += fabsf(cand) * d[3];
+    }
+}
+
+template <bool ORBIT>
+__device__ __forceinline__ void eval_dispatch(const FptSdfInstruction &op, EvalState &s, int *oc) {
+    switch (op.opcode) {
+        case FPT_SDF_OP_ABS: eval_op<FPT_SDF_OP_ABS, ORBIT>(op, s, oc); break;
+        case FPT_SDF_OP_TRANSLATE: eval_op<FPT_SDF_OP_TRANSLATE, ORBIT>(op, s, oc); break;
+        case FPT_SDF_OP_SCALE: eval_op<FPT_SDF_OP_SCALE, ORBIT>(op, s, oc); break;
+        case FPT_SDF_OP_ROTATE_X: eval_op<FPT_SDF_OP_ROTATE_X, ORBIT>(op, s, oc); break;
+        case FPT_SDF_OP_ROTATE_Y: eval_op<FPT_SDF_OP_ROTATE_Y, ORBIT>(op, s, oc); break;
+        case FPT_SDF_OP_ROTATE_Z: eval_op<FPT_SDF_OP_ROTATE_Z, ORBIT>(op, s, oc); break;
+        case FPT_SDF_OP_REPEAT: eval_op<FPT_SDF_OP_REPEAT, ORBIT>(op, s, oc); break;
+        case FPT_SDF_OP_SORT_DESC: eval_op<FPT_SDF_OP_SORT_DESC, ORBIT>(op, s, oc); break;
+        case FPT_SDF_OP_BOX_FOLD: eval_op<FPT_SDF_OP_BOX_FOLD, ORBIT>(op, s, oc); break;
+        case FPT_SDF_OP_SPHERE: eval_op<FPT_SDF_OP_SPHERE, ORBIT>(op, s, oc); break;
+        case FPT_SDF_OP_BOX: eval_op<FPT_SDF_OP_BOX, ORBIT>(op, s, oc); break;
+        case FPT_SDF_OP_PLANE: eval_op<FPT_SDF_OP_PLANE, ORBIT>(op, s, oc); break;
+        case FPT_SDF_OP_MANDELBULB: eval_op<FPT_SDF_OP_MANDELBULB, ORBIT>(op, s, oc); break;
+        case FPT_SDF_OP_MANDELBOX: eval_op<FPT_SDF_OP_MANDELBOX, ORBIT>(op, s, oc); break;
+        case FPT_SDF_OP_MENGER: eval_op<FPT_SDF_OP_MENGER, ORBIT>(op, s, oc); break;
+        case FPT_SDF_OP_ORBIT_ADD: eval_op<FPT_SDF_OP_ORBIT_ADD, ORBIT>(op, s, oc); break;
+        default: break;
+    }
+}
+
 // deProgramDistance / deShadingProgram: distance (and orbit when asked).
 template <bool ORBIT>
 __device__ float program_eval(const FptSdfInstruction *ops, uint32_t count, f3 p, float *orbit_out,
                               int *orbit_counter) {
-    float ds = 1.0f;
-    float dist = kInf;
-    float orbit = 0.0f;
-    bool has = false;
-    for (uint32_t i = 0; i < count; ++i) {
-        const FptSdfInstruction &op = ops[i];
-        if (apply_transform(op, p, ds)) continue;
-        const float *d = op.data;
-        const float div = fmaxf(ds, 1.0e-6f);
-        float cand;
-        bool leaf = true;
-        switch (op.opcode) {
-            case FPT_SDF_OP_SPHERE: cand = length(p) - d[0]; break;
-            case FPT_SDF_OP_BOX: {
-                f3 q = vabs(p) - vabs(ld3(d));
-                cand = fminf(fmaxf(q.x, fmaxf(q.y, q.z)), 0.0f) + length(vmax0(q));
-                break;
-            }
-            case FPT_SDF_OP_PLANE: cand = dot(p, normalize(ld3(d))) + d[3]; break;
-            case FPT_SDF_OP_MANDELBULB:
-            case FPT_SDF_OP_MANDELBOX:
-            case FPT_SDF_OP_MENGER: {
-                Leaf<float> lf = eval_leaf<float>(op.opcode, v3<float>{p.x, p.y, p.z}, d, orbit_counter);
-                cand = lf.d;
-                if (ORBIT) orbit += lf.trap;
-                break;
-            }
-            case FPT_SDF_OP_ORBIT_ADD:
-                if (ORBIT) orbit += length(p * ld3(d)) * d[3];
-                leaf = false;
-                break;
-            default: leaf = false; break;
-        }
-        if (!leaf) continue;
-        cand = cand / div;
-        dist = has ? combine(dist, cand, op.flags) : cand;
-        has = true;
-        if (ORBIT && (op.opcode == FPT_SDF_OP_SPHERE || op.opcode == FPT_SDF_OP_BOX))
-            orbit += fabsf(cand) * d[3];
-    }
-    if (ORBIT) *orbit_out = orbit;
-    return dist;
+    EvalState s = {p, 1.0f, kInf, 0.0f, false};
+    for (uint32_t i = 0; i < count; ++i) eval_dispatch<ORBIT>(ops[i], s, orbit_counter);
+    if (ORBIT) *orbit_out = s.orbit;
+    return s.dist;
 }
 
-// programSurfaceInterpreted (SM:3266-3369) through fractal leaves.
-__device__ float program_surface(const FptSdfInstruction *ops, uint32_t count, f3 p, f3 &grad) {
-    f3 jx = mk(1.f, 0.f, 0.f), jy = mk(0.f, 1.f, 0.f), jz = mk(0.f, 0.f, 1.f);
-    float ds = 1.0f;
-    float dist = kInf;
-    grad = mk(0.f, 0.f, 0.f);
-    bool has = false;
-    for (uint32_t i = 0; i < count; ++i) {
-        const FptSdfInstruction &op = ops[i];
-        const float *d = op.data;
-        switch (op.opcode) {
-            case FPT_SDF_OP_ABS:
-                jx = jx * signf(p.x); jy = jy * signf(p.y); jz = jz * signf(p.z);
-                p = vabs(p);
-                continue;
-            case FPT_SDF_OP_TRANSLATE: p = p - ld3(d); continue;
-            case FPT_SDF_OP_SCALE: {
-                float s = fabsf(d[0]) > 1.0e-6f ? d[0] : 1.0f;
-                p = p * s; jx = jx * s; jy = jy * s; jz = jz * s;
-                ds *= fabsf(s);
-                continue;
-            }
-            case FPT_SDF_OP_ROTATE_X: {
-                float s, c; sincosf(d[0], &s, &c);
-                float py = p.y; f3 oj = jy;
-                p.y = c * py - s * p.z; p.z = s * py + c * p.z;
-                jy = oj * c - jz * s; jz = oj * s + jz * c;
-                continue;
-            }
-            case FPT_SDF_OP_ROTATE_Y: {
-                float s, c; sincosf(d[0], &s, &c);
-                float px = p.x; f3 oj = jx;
-                p.x = c * px - s * p.z; p.z = s * px + c * p.z;
-                jx = oj * c - jz * s; jz = oj * s + jz * c;
-                continue;
-            }
-            case FPT_SDF_OP_ROTATE_Z: {
-                float s, c; sincosf(d[0], &s, &c);
-                float px = p.x; f3 oj = jx;
-                p.x = c * px - s * p.y; p.y = s * px + c * p.y;
-                jx = oj * c - jy * s; jy = oj * s + jy * c;
-                continue;
-            }
-            case FPT_SDF_OP_REPEAT: { float u = 1.f; apply_transform(op, p, u); continue; }
-            case FPT_SDF_OP_SORT_DESC: {
-                float t; f3 tj;
-                if (p.x < p.z) { t = p.x; p.x = p.z; p.z = t; tj = jx; jx = jz; jz = tj; }
-                if (p.y < p.z) { t = p.y; p.y = p.z; p.z = t; tj = jy; jy = jz; jz = tj; }
-                if (p.x < p.y) { t = p.x; p.x = p.y; p.y = t; tj = jx; jx = jy; jy = tj; }
-                continue;
-            }
-            case FPT_SDF_OP_BOX_FOLD: {
-                float l = d[0] != 0.0f ? d[0] : 1.0f;
-                float fx = fabsf(p.x) > l ? -1.f : 1.f, fy = fabsf(p.y) > l ? -1.f : 1.f,
-                      fz = fabsf(p.z) > l ? -1.f : 1.f;
-                float u = 1.f;
-                apply_transform(op, p, u);
-                jx = jx * fx; jy = jy * fy; jz = jz * fz;
-                continue;
-            }
-            default: break;
-        }
-        const float scale = fmaxf(ds, 1.0e-6f);
+struct SurfState {
+    f3 p, jx, jy, jz, grad;
+    float ds, dist;
+    bool has;
+};
+
+template <uint32_t OP>
+__device__ __forceinline__ void surf_op(const FptSdfInstruction &op, SurfState &s) {
+    const float *d = op.data;
+    f3 &p = s.p, &jx = s.jx, &jy = s.jy, &jz = s.jz;
+    if constexpr (OP == FPT_SDF_OP_ABS) {
+        jx = jx * signf(p.x); jy = jy * signf(p.y); jz = jz * signf(p.z);
+        p = vabs(p);
+    } else if constexpr (OP == FPT_SDF_OP_TRANSLATE) {
+        p = p - ld3(d);
+    } else if constexpr (OP == FPT_SDF_OP_SCALE) {
+        float sc = fabsf(d[0]) > 1.0e-6f ? d[0] : 1.0f;
+        p = p * sc; jx = jx * sc; jy = jy * sc; jz = jz * sc;
+        s.ds *= fabsf(sc);
+    } else if constexpr (OP == FPT_SDF_OP_ROTATE_X) {
+        float sn, c; sincosf(d[0], &sn, &c);
+        float py = p.y; f3 oj = jy;
+        p.y = c * py - sn * p.z; p.z = sn * py + c * p.z;
+        jy = oj * c - jz * sn; jz = oj * sn + jz * c;
+    } else if constexpr (OP == FPT_SDF_OP_ROTATE_Y) {
+        float sn, c; sincosf(d[0], &sn, &c);
+        float px = p.x; f3 oj = jx;
+        p.x = c * px - sn * p.z; p.z = sn * px + c * p.z;
+        jx = oj * c - jz * sn; jz = oj * sn + jz * c;
+    } else if constexpr (OP == FPT_SDF_OP_ROTATE_Z) {
+        float sn, c; sincosf(d[0], &sn, &c);
+        float px = p.x; f3 oj = jx;
+        p.x = c * px - sn * p.y; p.y = sn * px + c * p.y;
+        jx = oj * c - jy * sn; jy = oj * sn + jy * c;
+    } else if constexpr (OP == FPT_SDF_OP_REPEAT) {
+        FptSdfInstruction t = op;
+        t.opcode = OP;
+        float u = 1.f;
+        (void)apply_transform(t, p, u);
+    } else if constexpr (OP == FPT_SDF_OP_SORT_DESC) {
+        float t; f3 tj;
+        if (p.x < p.z) { t = p.x; p.x = p.z; p.z = t; tj = jx; jx = jz; jz = tj; }
+        if (p.y < p.z) { t = p.y; p.y = p.z; p.z = t; tj = jy; jy = jz; jz = tj; }
+        if (p.x < p.y) { t = p.x; p.x = p.y; p.y = t; tj = jx; jx = jy; jy = tj; }
+    } else if constexpr (OP == FPT_SDF_OP_BOX_FOLD) {
+        float l = d[0] != 0.0f ? d[0] : 1.0f;
+        float fx = fabsf(p.x) > l ? -1.f : 1.f, fy = fabsf(p.y) > l ? -1.f : 1.f, fz = fabsf(p.z) > l ? -1.f : 1.f;
+        FptSdfInstruction t = op;
+        t.opcode = OP;
+        float u = 1.f;
+        (void)apply_transform(t, p, u);
+        jx = jx * fx; jy = jy * fy; jz = jz * fz;
+    } else if constexpr (op_is_distance<OP>()) {
+        const float scale = fmaxf(s.ds, 1.0e-6f);
         float cand;
         f3 lg;
-        if (op.opcode == FPT_SDF_OP_SPHERE) {
+        if constexpr (OP == FPT_SDF_OP_SPHERE) {
             float radius = length(p);
             cand = (radius - d[0]) / scale;
             lg = radius > 1.0e-8f ? p / radius : mk(0.f, 1.f, 0.f);
-        } else if (op.opcode == FPT_SDF_OP_BOX) {
+        } else if constexpr (OP == FPT_SDF_OP_BOX) {
             f3 q = vabs(p) - vabs(ld3(d));
             f3 o = vmax0(q);
             float ol = length(o);
@@ -692,28 +726,65 @@ __device__ float program_surface(const FptSdfInstruction *ops, uint32_t count, f
             else if (q.x >= q.y && q.x >= q.z) lg = mk(signf(p.x), 0.f, 0.f);
             else if (q.y >= q.z) lg = mk(0.f, signf(p.y), 0.f);
             else lg = mk(0.f, 0.f, signf(p.z));
-        } else if (op.opcode == FPT_SDF_OP_PLANE) {
+        } else if constexpr (OP == FPT_SDF_OP_PLANE) {
             lg = normalize(ld3(d));
             cand = (dot(p, lg) + d[3]) / scale;
-        } else if (is_leaf(op.opcode)) {
+        } else {
             v3<Df> dp = {{p.x, 1.f, 0.f, 0.f}, {p.y, 0.f, 1.f, 0.f}, {p.z, 0.f, 0.f, 1.f}};
-            Leaf<Df> lf = eval_leaf<Df>(op.opcode, dp, d, nullptr);
+            Leaf<Df> lf = eval_leaf<Df>(OP, dp, d, nullptr);
             cand = lf.d.v / scale;
             lg = mk(lf.d.a, lf.d.b, lf.d.c);
-        } else {
-            continue;
         }
         f3 cg = (jx * lg.x + jy * lg.y + jz * lg.z) / scale;
-        bool choose = !has || (op.flags == 1u ? cand > dist : (op.flags == 2u ? -cand > dist : cand < dist));
+        bool choose = !s.has || (op.flags == 1u ? cand > s.dist : (op.flags == 2u ? -cand > s.dist : cand < s.dist));
         if (choose) {
-            bool neg = op.flags == 2u && has;
-            dist = neg ? -cand : cand;
-            grad = neg ? -cg : cg;
+            bool neg = op.flags == 2u && s.has;
+            s.dist = neg ? -cand : cand;
+            s.grad = neg ? -cg : cg;
         }
-        has = true;
+        s.has = true;
     }
-    return dist;
 }
+
+__device__ __forceinline__ void surf_dispatch(const FptSdfInstruction &op, SurfState &s) {
+    switch (op.opcode) {
+        case FPT_SDF_OP_ABS: surf_op<FPT_SDF_OP_ABS>(op, s); break;
+        case FPT_SDF_OP_TRANSLATE: surf_op<FPT_SDF_OP_TRANSLATE>(op, s); break;
+        case FPT_SDF_OP_SCALE: surf_op<FPT_SDF_OP_SCALE>(op, s); break;
+        case FPT_SDF_OP_ROTATE_X: surf_op<FPT_SDF_OP_ROTATE_X>(op, s); break;
+        case FPT_SDF_OP_ROTATE_Y: surf_op<FPT_SDF_OP_ROTATE_Y>(op, s); break;
+        case FPT_SDF_OP_ROTATE_Z: surf_op<FPT_SDF_OP_ROTATE_Z>(op, s); break;
+        case FPT_SDF_OP_REPEAT: surf_op<FPT_SDF_OP_REPEAT>(op, s); break;
+        case FPT_SDF_OP_SORT_DESC: surf_op<FPT_SDF_OP_SORT_DESC>(op, s); break;
+        case FPT_SDF_OP_BOX_FOLD: surf_op<FPT_SDF_OP_BOX_FOLD>(op, s); break;
+        case FPT_SDF_OP_SPHERE: surf_op<FPT_SDF_OP_SPHERE>(op, s); break;
+        case FPT_SDF_OP_BOX: surf_op<FPT_SDF_OP_BOX>(op, s); break;
+        case FPT_SDF_OP_PLANE: surf_op<FPT_SDF_OP_PLANE>(op, s); break;
+        case FPT_SDF_OP_MANDELBULB: surf_op<FPT_SDF_OP_MANDELBULB>(op, s); break;
+        case FPT_SDF_OP_MANDELBOX: surf_op<FPT_SDF_OP_MANDELBOX>(op, s); break;
+        case FPT_SDF_OP_MENGER: surf_op<FPT_SDF_OP_MENGER>(op, s); break;
+        default: break;
+    }
+}
+
+__device__ __forceinline__ SurfState surf_begin(f3 p) {
+    return {p, mk(1.f, 0.f, 0.f), mk(0.f, 1.f, 0.f), mk(0.f, 0.f, 1.f), mk(0.f, 0.f, 0.f), 1.0f, kInf, false};
+}
+
+// programSurfaceInterpreted (SM:3266-3369) through fractal leaves.
+__device__ float program_surface(const FptSdfInstruction *ops, uint32_t count, f3 p, f3 &grad) {
+    SurfState s = surf_begin(p);
+    for (uint32_t i = 0; i < count; ++i) surf_dispatch(ops[i], s);
+    grad = s.grad;
+    return s.dist;
+}
+
+#ifdef FPT_TOPOLOGY_SPECIALIZED
+// Defined by the runtime-generated source (straight-line eval_op / surf_op
+// calls for the program's opcode sequence).
+__device__ float topology_distance(f3 p, const FptRenderConfig &cfg, int *orbit_counter);
+__device__ float topology_surface(f3 p, const FptRenderConfig &cfg, f3 &grad);
+#endif
 
 // ---------------------------------------------------------------------------
 // Built-in fractal scenes (SM:730-784).
@@ -787,6 +858,9 @@ __device__ __forceinline__ uint32_t prog_count(const FptRenderConfig &cfg) {
 
 __device__ __forceinline__ float distance_sdf(f3 p, const FptRenderConfig &cfg, Work &w) {
     w.distance_evals++;
+#ifdef FPT_TOPOLOGY_SPECIALIZED
+    if (is_program(cfg)) return topology_distance(p, cfg, &w.orbit_iterations);
+#endif
     if (is_program(cfg)) return program_eval<false>(cfg.sdf_program, prog_count(cfg), p, nullptr, &w.orbit_iterations);
     if (cfg.sdf_id == FPT_SDF_MANDELBOX_FRACTAL) return de_mandelbox_builtin(p, cfg, nullptr, &w.orbit_iterations);
     if (cfg.sdf_id == FPT_SDF_MENGER_SPONGE) return de_menger_builtin(p, cfg, &w.orbit_iterations);
@@ -926,7 +1000,11 @@ __device__ f3 normal_at(f3 p, const FptRenderConfig &cfg, Work &w) {
     const float e = fmaxf(cfg.render[2], 0.0002f);
     if ((cfg.sdf_normal_mode == 0u || cfg.sdf_normal_mode == 2u) && is_program(cfg)) {
         f3 g;
+#ifdef FPT_TOPOLOGY_SPECIALIZED
+        topology_surface(p, cfg, g);
+#else
         program_surface(cfg.sdf_program, prog_count(cfg), p, g);
+#endif
         if (dot(g, g) >= 1.0e-12f && finite3(g)) return normalize(g);
     }
     if (cfg.sdf_normal_mode == 1u) {

@@ -1,8 +1,10 @@
 // Internal interfaces between the HIP kernels and the C-ABI layer.
 #pragma once
 
+#ifndef __HIPCC_RTC__  // the runtime compiler provides these itself
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
 #include "../../include/fpt_abi.h"
 
@@ -41,6 +43,9 @@ struct BatchScene {
     unsigned long long accum_offset;
 };
 
+#ifndef __HIPCC_RTC__  // host launchers
+hipError_t launch_zero_counter(uint32_t *counter, hipStream_t stream);
+uint32_t persistent_slots(uint32_t width, uint32_t height);
 hipError_t launch_sdf_batch(const FptRenderConfig *dcfgs, const BatchScene *dscenes, uint32_t nscenes,
                             uint32_t total_slots, float4 *accum, uint32_t *counter, uint32_t workgroups,
                             int event_batch, uint32_t spp_cap, DeviceProfile *prof, hipStream_t stream);
@@ -61,5 +66,7 @@ hipError_t launch_sdf_diagnostic(const FptRenderConfig *dcfg, const FptRenderCon
                                  const FptDiagnosticConfig &dg, uchar4 *out, hipStream_t stream);
 hipError_t launch_present(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, const float4 *accum,
                           uchar4 *out, hipStream_t stream);
+
+#endif
 
 }  // namespace fpt

@@ -30,8 +30,10 @@
 // xor-shuffles for the final dot product.
 #pragma once
 
+#ifndef __HIPCC_RTC__  // the runtime compiler provides these itself
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
 #include "fpt_device.h"
 
@@ -150,6 +152,7 @@ __device__ __forceinline__ bool proposer_in_box(const ProposerDevice &P, f3 p) {
     return fabsf(p.x) < P.box_half && fabsf(p.y) < P.box_half && fabsf(p.z) < P.box_half;
 }
 
+#ifndef __HIPCC_RTC__
 // Host side (learned.hip).
 struct ProposerHost {
     void *blob = nullptr;  // device: frags | bias | w_out
@@ -166,5 +169,7 @@ hipError_t launch_proposer_eval(const ProposerDevice &P, const float4 *pts, size
 hipError_t launch_proposer_samples(const FptRenderConfig *dcfg, float box_half, float band, uint32_t count,
                                    uint64_t seed, uint32_t max_attempts, float4 *out, uint32_t *failures,
                                    hipStream_t stream);
+
+#endif
 
 }  // namespace fpt

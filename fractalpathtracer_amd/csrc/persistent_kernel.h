@@ -1,0 +1,463 @@
+// Persistent, path-regenerating SDF path tracer for gfx950, for one scene or
+// a batch of scenes in one launch.
+//
+// The straightforward kernel (render.hip) runs one path per lane from camera
+// to termination: a 64-lane wave then waits for its slowest ray at every
+// march, and the per-hit work (analytic normal = forward-mode DE, material,
+// sun setup) runs whenever any one lane reaches a hit.  Here every lane is a
+// small state machine over the same path (Shaders.metal:5926-6026):
+//
+//   MARCH  one distance evaluation + advance per loop iteration
+//          (Shaders.metal:3753-3783), for camera, bounce and shadow rays;
+//   EVENT  the march has ended; the lane waits until enough lanes of its
+//          wave are waiting (`event_batch`), then all of them run the hit /
+//          shadow / scatter logic together, so its cost is paid once per
+//          batch instead of once per lane;
+//   IDLE   no pixel left.
+//
+// When a path ends the lane starts the pixel's next sample; when a pixel's
+// samples are done it writes the running mean and takes the next work slot
+// from a global counter (one atomic per wave).  Slots are pixels in 8x8 tile
+// order, scene after scene for a batch, so the lanes of a wave work on
+// neighbouring pixels of one scene except at scene boundaries.  Samples of a
+// pixel are processed in order by one lane, so the running mean
+// mix(c, s, 1/(i+1)) and every RNG key are exactly those of the
+// straightforward kernel and the images are bit-identical.
+//
+// MULTI = false: one scene, its config read through a kernel-argument pointer
+// (wave-uniform scalar loads).  MULTI = true: every lane carries the pointer
+// to its scene's config (vector loads, cached), scenes may differ in size,
+// spp and program.
+//
+// LEARNED = true (single scene): march steps inside the proposer's training
+// box may use the learned distance proposal q (learned.h) instead of the exact
+// distance.  A march is in "far" mode from the camera, and again whenever an
+// exact step saw |d| > resume_distance; in far mode, while q >
+// switch_distance, the step is step_scale * q (no exact evaluation).  The
+// first proposal at or below switch_distance hands the march to the exact
+// distance; if that exact distance is negative the far step overshot into the
+// surface and the march backs up to the point before it.  Far steps have their
+// own budget (the march's step limit); when it runs out the march stays
+// exact.  Hits, normals, shading and termination tests are the exact ones.
+#pragma once
+
+#ifndef __HIPCC_RTC__
+#include <hip/hip_runtime.h>
+#endif
+
+#include "fpt_device.h"
+#include "fpt_internal.h"
+#include "learned.h"
+
+namespace fpt {
+
+enum : int { PH_IDLE = 0, PH_MARCH = 1, PH_EVENT = 2 };
+
+struct PersistentCounters {
+    int primary_steps, secondary_steps, shadow_steps, normal_evals, bounces;
+};
+
+struct LearnedCounters {
+    int proposer_calls, proposals, far_steps, backtracks;
+};
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+
+// Wave-level fetch of work slots: one atomic per wave, consecutive indices to
+// the requesting lanes in lane order.  Must be reached by every lane.
+__device__ __forceinline__ uint32_t fetch_slot(bool want, uint32_t *counter) {
+    const unsigned long long m = __ballot(want);
+    uint32_t base = 0;
+    if (m) {
+        const int leader = __ffsll((long long)m) - 1;
+        if (int(lane_id()) == leader) base = atomicAdd(counter, uint32_t(__popcll(m)));
+        base = __shfl(base, leader, 64);
+    }
+    const unsigned long long below = m & ((1ull << lane_id()) - 1ull);
+    return base + uint32_t(__popcll(below));
+}
+
+// Slot index within a scene -> (x, y): 8x8 tiles, tiles row-major.
+__device__ __forceinline__ void pixel_of(uint32_t k, uint32_t w, uint32_t &x, uint32_t &y) {
+    const uint32_t tiles_x = (w + 7u) >> 3;
+    const uint32_t tile = k >> 6, in = k & 63u;
+    x = (tile % tiles_x) * 8u + (in & 7u);
+    y = (tile / tiles_x) * 8u + (in >> 3);
+}
+
+__host__ __device__ __forceinline__ uint32_t slot_count(uint32_t w, uint32_t h) {
+    return ((w + 7u) / 8u) * ((h + 7u) / 8u) * 64u;
+}
+
+// Minimum resident waves per SIMD the register allocation must allow (exact
+// and learned variants); the compiler caps VGPRs at 512 / waves.
+#ifndef FPT_PERSISTENT_WAVES
+#define FPT_PERSISTENT_WAVES 2
+#endif
+#ifndef FPT_LEARNED_WAVES
+#define FPT_LEARNED_WAVES 1
+#endif
+
+template <bool PROFILE, bool MULTI, bool LEARNED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LEARNED ? FPT_LEARNED_WAVES
+                                                                             : FPT_PERSISTENT_WAVES)))
+void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
+                                                             const BatchScene *__restrict__ scenes, uint32_t nscenes,
+                                                             float4 *__restrict__ accum, uint32_t start,
+                                                             uint32_t count, uint32_t *__restrict__ counter,
+                                                             uint32_t total_slots, int event_batch,
+                                                             DeviceProfile *prof, ProposerDevice prop) {
+    const FptRenderConfig *cp = cfgs;  // the lane's scene (fixed when !MULTI)
+    float4 *out = accum;               // the lane's scene accumulation buffer
+
+    Work w = {0, 0};
+    PersistentCounters pc = {0, 0, 0, 0, 0};
+    LearnedCounters lc = {0, 0, 0, 0};
+    int pixels_done = 0;
+
+    // Per-scene constants, refreshed when a lane takes a pixel.
+    uint32_t W = 0, H = 0, s_end = 0, s_begin = 0;
+    f3 cam = mk(0.f, 0.f, 0.f);
+    float focal = 1.f, aa = 0.f, maxd = 0.f, maxd2 = 0.f, far2 = 0.f, min_dist = 0.f, adapt = 0.f;
+    bool check_t = false, sun_on = false;
+    int bounces = 0, ni_first = 0, ni_next = 0;
+    f3 sun_dir0 = mk(0.f, 0.f, 1.f);
+    auto load_scene = [&]() {
+        const FptRenderConfig &cfg = *cp;
+        const uint32_t total = cfg.samples < 1u ? 1u : (cfg.samples > 512u ? 512u : cfg.samples);
+        s_begin = MULTI ? 0u : start;
+        // MULTI: `count` caps the samples per pixel (0 = all; autotuning proxies).
+        s_end = MULTI ? (count && count < total ? count : total) : (start + count < total ? start + count : total);
+        cam = ld3(cfg.camera_position);
+        focal = 1.0f / tanf(cfg.camera_fov / 2.0f * kPi / 180.0f);
+        aa = 0.3f / fmaxf(float(W), float(H));
+        maxd = cfg.render[4];
+        maxd2 = maxd * maxd;
+        const float far = maxd * 0.99f;
+        far2 = far * far;
+        min_dist = cfg.render[3];
+        adapt = cfg.render[5];
+        check_t = cfg.sdf_id == FPT_SDF_PROGRAM && cfg.program_material[5] > 0.0f;
+        bounces = int(cfg.render[0]) < 8 ? int(cfg.render[0]) : 8;
+        if (cfg.sdf_bounce_cap > 0u && int(cfg.sdf_bounce_cap) < bounces) bounces = int(cfg.sdf_bounce_cap);
+        ni_first = int(cfg.render[1]);
+        ni_next = int(cfg.render[1] / (cfg.render[5] * 2.0f + 1.0f));
+        sun_on = cfg.sun[0] == 1.0f;
+        sun_dir0 = rotate_camera(mk(0.f, 0.f, 1.f), cfg.sun[1] * kPi / 180.0f, cfg.sun[2] * kPi / 180.0f);
+    };
+    if (!MULTI) {
+        W = cfgs->width;
+        H = cfgs->height;
+        load_scene();
+    }
+
+    // Lane state.
+    int phase = PH_IDLE;
+    uint32_t pix = 0, s = 0;
+    UV uv = {0.f, 0.f};
+    f3 acc = mk(0.f, 0.f, 0.f);
+    int bounce = 0, local_ni = 0;
+    float side = 1.0f, sky_mask = 0.0f;
+    f3 light = mk(0.f, 0.f, 0.f), color = mk(1.f, 1.f, 1.f), gcol = mk(0.f, 0.f, 0.f), dr = mk(0.f, 0.f, 0.f);
+    f3 hit_p = mk(0.f, 0.f, 0.f), hit_n = mk(0.f, 0.f, 0.f);
+    Material mat;
+    mat.rgb = mk(1.f, 1.f, 1.f);
+    mat.roughness = 1.f; mat.specular = 0.f; mat.translucency = 0.f; mat.ior = 1.5f; mat.emission = 0.f;
+    f3 p = mk(0.f, 0.f, 0.f), mdir = mk(0.f, 0.f, 1.f), mstart = mk(0.f, 0.f, 0.f);
+    int it = 0, max_iter = 0;
+    bool shadow = false;
+    // Learned stepping state (LEARNED only).
+    bool lfar = false, lastfar = false;
+    int fit = 0;
+    f3 psafe = mk(0.f, 0.f, 0.f);
+    bool retry = false;  // slot was tile padding: take another
+
+    // Start the camera path of sample `s` of the current pixel.
+    auto start_path = [&]() {
+        const FptRenderConfig &cfg = *cp;
+        const float frame = float(s);
+        float jx, jy;
+        random_point(aa, uv, frame, jx, jy);
+        dr = rotate_camera_roll(normalize(mk(uv.x + jx, uv.y + jy, focal)), cfg.camera_yaw_pitch[0],
+                                cfg.camera_yaw_pitch[1], cfg.camera_roll);
+        const float gt = clampf((dr.y + 1.0f) / 2.0f, 0.f, 1.f);
+        gcol = vmix(ld3(&cfg.background_gradient[3]), ld3(&cfg.background_gradient[0]), gt);
+        bounce = 0;
+        local_ni = ni_first;
+        side = 1.0f;
+        sky_mask = 0.0f;
+        light = mk(0.f, 0.f, 0.f);
+        color = mk(1.f, 1.f, 1.f);
+        p = cam;
+        mstart = cam;
+        mdir = dr;
+        it = 0;
+        max_iter = local_ni < 360 ? local_ni : 360;
+        shadow = false;
+        lfar = LEARNED;
+        lastfar = false;
+        fit = 0;
+        phase = PH_MARCH;
+    };
+
+    // Take a work slot (or go idle).  Wave-uniform call.
+    auto take_slot = [&](bool want) {
+        const uint32_t k = fetch_slot(want, counter);
+        retry = false;
+        if (!want) return;
+        if (k >= total_slots) {
+            phase = PH_IDLE;
+            return;
+        }
+        uint32_t local = k;
+        if (MULTI) {
+            // Binary search of the scene owning slot k.
+            uint32_t lo = 0, hi = nscenes - 1u;
+            while (lo < hi) {
+                uint32_t mid = (lo + hi + 1u) >> 1;
+                if (scenes[mid].slot_begin <= k) lo = mid;
+                else hi = mid - 1u;
+            }
+            const BatchScene sc = scenes[lo];
+            cp = cfgs + lo;
+            out = accum + sc.accum_offset;
+            W = sc.width;
+            H = sc.height;
+            local = k - sc.slot_begin;
+            load_scene();
+        }
+        uint32_t x, y;
+        pixel_of(local, W, x, y);
+        if (x >= W || y >= H) {
+            retry = true;
+            return;
+        }
+        pix = y * W + x;
+        uv = screen_uv(x, y, W, H);
+        s = s_begin;
+        if (s_begin != 0u) {
+            float4 prev = out[pix];
+            acc = mk(prev.x, prev.y, prev.z);
+        } else {
+            acc = mk(0.f, 0.f, 0.f);
+        }
+        start_path();
+    };
+
+    take_slot(true);
+    while (__any(retry)) take_slot(retry);
+
+    while (true) {
+        float q = 0.f;
+        bool use_q = false;
+        if (LEARNED) {
+            use_q = phase == PH_MARCH && lfar && proposer_in_box(prop, p);
+            if (__any(use_q)) {  // wave-uniform: the MLP needs every lane
+                q = proposer_eval(prop, p);
+                if (PROFILE) {
+                    lc.proposer_calls += lane_id() == 0u;
+                    lc.proposals += use_q;
+                }
+            }
+        }
+        if (phase == PH_MARCH) {
+            const FptRenderConfig &cfg = *cp;
+            if (LEARNED && use_q && q > prop.switch_distance) {
+                // Far step on the proposal.
+                psafe = p;
+                p = p + mdir * (q * prop.step_scale);
+                lastfar = true;
+                if (++fit >= max_iter) lfar = false;
+                if (PROFILE) lc.far_steps++;
+            } else {
+                if (LEARNED && use_q) lfar = false;  // within switch_distance: exact from here
+                const float d = map_sdf(p, cfg, w);
+                ++it;
+                if (LEARNED && lastfar && use_q && d < 0.0f) {
+                    // The last far step crossed the surface: back up.
+                    p = psafe;
+                    lastfar = false;
+                    if (PROFILE) lc.backtracks++;
+                    if (it >= max_iter) phase = PH_EVENT;
+                } else {
+                    lastfar = false;
+                    bool stop = !isfinite(d);
+                    if (!stop) {
+                        const float o = fabsf(d) * 0.99f;
+                        p = p + mdir * o;
+                        const f3 df = mstart - p;  // lod_falloff 0.0002 (SM:3771-3776)
+                        float lod = mixf(min_dist, mixf(min_dist, 0.1f, dot(df, df) * 0.0002f), adapt);
+                        if (check_t && user_material(p, cfg, w).translucency > 0.0f) lod = 0.0001f;
+                        stop = o < lod || maxd < o || it >= max_iter;
+                        if (LEARNED && fabsf(d) > prop.resume_distance && fit < max_iter) lfar = true;
+                    }
+                    if (stop) phase = PH_EVENT;
+                }
+            }
+        }
+        const unsigned long long ev = __ballot(phase == PH_EVENT);
+        const unsigned long long mar = __ballot(phase == PH_MARCH);
+        if (!ev && !mar) break;
+        if (!(__popcll(ev) >= event_batch || (ev && !mar))) continue;
+
+        bool need_slot = false;
+        if (phase == PH_EVENT) {
+            const FptRenderConfig &cfg = *cp;
+            const float frame = float(s);
+            bool path_done = false;
+            bool do_scatter = false;
+            if (shadow) {
+                // End of the sun shadow march (SM:5856-5860).
+                pc.shadow_steps += it;
+                if (length(hit_p - p) > maxd * 0.99f) {
+                    const float k = cfg.sun[3] * fmaxf(dot(hit_n, mdir), 0.0f) * mat.roughness * (1.0f - mat.translucency);
+                    light = light + ld3(cfg.sun_color) * k;
+                }
+                do_scatter = true;
+            } else {
+                // End of a camera / bounce march (SM:5971-5989).
+                if (bounce == 0) pc.primary_steps += it; else pc.secondary_steps += it;
+                pc.bounces++;
+                const f3 tv = p - cam;
+                const float travel2 = dot(tv, tv);
+                if (bounce == 0 && travel2 > maxd2) sky_mask = 1.0f;
+                local_ni = ni_next;
+                if (travel2 > far2) {
+                    light = light + environment(dr, cfg);
+                    path_done = true;
+                } else {
+                    hit_p = p;
+                    mat = user_material(p, cfg, w);
+                    if (mat.emission > 0.001f) light = light + mat.rgb * mat.emission;
+                    hit_n = normal_at(p, cfg, w);
+                    pc.normal_evals++;
+                    if (sun_on) {
+                        // Shadow ray (SM:5846-5856).
+                        float h1 = hash13(uv.x, uv.y, madd_rn(frame, 5.0f, 1.0f));
+                        float h2 = hash13(uv.x, uv.y, madd_rn(frame, 3.0f, 5.0f));
+                        float sn, cs;
+                        sincosf(h1 * 2.0f * kPi, &sn, &cs);
+                        float m = sqrtf(h2) * cfg.sun[4];
+                        mdir = rotate_camera(sun_dir0, cs * m, sn * m);
+                        p = hit_p + hit_n * 0.001f;
+                        mstart = p;
+                        it = 0;
+                        max_iter = ni_first < 360 ? ni_first : 360;
+                        shadow = true;
+                        lfar = false;
+                        lastfar = false;
+                        fit = 0;
+                        phase = PH_MARCH;
+                    } else {
+                        do_scatter = true;
+                    }
+                }
+            }
+            if (do_scatter) {
+                // Scatter (SM:5990-6021).
+                shadow = false;
+                f3 n = hit_n;
+                const float fi = float(bounce);
+                float r1 = hash13(uv.x, uv.y, madd_rn(frame, 1.37f, fi));
+                float r2 = hash13(uv.x, uv.y, madd_rn(frame, 7.91f, fi));
+                if (r1 > mat.translucency) {
+                    f3 metal = reflect(dr, n);
+                    f3 diffuse = random_vector(n, uv, madd_rn(frame, 13.37f, fi));
+                    float q = (mat.ior - 1.0f) / (mat.ior + 1.0f);
+                    float f0 = q * q;
+                    float om = 1.0f - clampf(dot(n, -dr), 0.f, 1.f);
+                    float om2 = om * om;
+                    float fres = f0 + (1.0f - f0) * (om2 * om2 * om);
+                    dr = normalize(vmix(metal, diffuse, mat.roughness));
+                    if (r2 < fres * mat.specular) dr = metal;
+                    else color = color * mat.rgb;
+                } else {
+                    float eta = side == 1.0f ? 1.0f / mat.ior : mat.ior;
+                    if (side != 1.0f) n = -n;
+                    float q = (mat.ior - 1.0f) / (mat.ior + 1.0f);
+                    float f0 = q * q;
+                    float om = 1.0f - clampf(dot(n, -dr), 0.f, 1.f);
+                    float om2 = om * om;
+                    float fres = f0 + (1.0f - f0) * (om2 * om2 * om);
+                    f3 refr = refract(dr, n, eta);
+                    f3 refl = reflect(dr, n);
+                    if (dot(refr, refr) < 0.000001f || !isfinite(refr.x) || r2 < fres) {
+                        dr = normalize(refl);
+                    } else {
+                        dr = normalize(refr);
+                        side = -side;
+                        color = color * vmix(mk(1.f, 1.f, 1.f), mat.rgb, 0.35f) * (1.0f - fres * 0.5f);
+                    }
+                }
+                f3 rp = hit_p + n * (0.001f * signf(dot(dr, n)));
+                bool stop = false;
+                if (cfg.sdf_russian_roulette != 0u && bounce + 1 < bounces && float(bounce + 1) >= cfg.sdf_rr_start) {
+                    float surv = clampf(fmaxf(color.x, fmaxf(color.y, color.z)), clampf(cfg.sdf_rr_min_prob, 0.01f, 1.0f),
+                                        1.0f);
+                    float rr = hash13(uv.x, uv.y, madd_rn(frame, 19.19f, __fmul_rn(fi, 3.17f)));
+                    if (rr > surv) stop = true;
+                    else color = color / surv;
+                }
+                ++bounce;
+                if (stop || bounce >= bounces) {
+                    path_done = true;
+                } else {
+                    p = rp;
+                    mstart = rp;
+                    mdir = dr;
+                    it = 0;
+                    max_iter = local_ni < 360 ? local_ni : 360;
+                    lfar = false;
+                    lastfar = false;
+                    fit = 0;
+                    phase = PH_MARCH;
+                }
+            }
+            if (path_done) {
+                f3 col = light * color;
+                if (cfg.world[6] == 1.0f) col = vmix(col, gcol, sky_mask);
+                col = vmin(col, 8.0f);
+                acc = vmix(acc, col, 1.0f / float(s + 1u));
+                ++s;
+                if (s < s_end) {
+                    start_path();
+                } else {
+                    out[pix] = make_float4(acc.x, acc.y, acc.z, 1.0f);
+                    ++pixels_done;
+                    need_slot = true;
+                }
+            }
+        }
+        // Wave-uniform hand-out for lanes whose pixel is finished.
+        take_slot(need_slot);
+        while (__any(retry)) take_slot(retry);
+    }
+
+    if (PROFILE) {
+        auto wadd = [&](unsigned long long *dst, unsigned long long v) {
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+            if (lane_id() == 0u && v) atomicAdd(dst, v);
+        };
+        wadd(&prof->distance_evals, (unsigned long long)w.distance_evals);
+        wadd(&prof->orbit_iterations, (unsigned long long)w.orbit_iterations);
+        wadd(&prof->primary_steps, (unsigned long long)pc.primary_steps);
+        wadd(&prof->secondary_steps, (unsigned long long)pc.secondary_steps);
+        wadd(&prof->shadow_steps, (unsigned long long)pc.shadow_steps);
+        wadd(&prof->normal_evals, (unsigned long long)pc.normal_evals);
+        wadd(&prof->bounces, (unsigned long long)pc.bounces);
+        wadd(&prof->pixels, (unsigned long long)pixels_done);
+        if (LEARNED) {
+            wadd(&prof->proposer_calls, (unsigned long long)lc.proposer_calls);
+            wadd(&prof->proposals, (unsigned long long)lc.proposals);
+            wadd(&prof->far_steps, (unsigned long long)lc.far_steps);
+            wadd(&prof->backtracks, (unsigned long long)lc.backtracks);
+        }
+    }
+}
+
+// Counter reset as a kernel (not a memset node), so a captured graph replays it.
+__global__ void zero_counter_kernel(uint32_t *c) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0u;
+}
+
+}  // namespace fpt

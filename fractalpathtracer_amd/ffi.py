@@ -239,13 +239,20 @@ class FptProposerStats(C.Structure):
 PROPOSER_PARAM_COUNT = 20801
 
 
+class FptSpecializationStats(C.Structure):
+    _fields_ = [("cache_status", u32), ("instruction_count", u32), ("source_bytes", C.c_uint64),
+                ("code_bytes", C.c_uint64), ("sample_count", C.c_uint64), ("distance_failures", C.c_uint64),
+                ("gradient_failures", C.c_uint64), ("max_distance_error", f32), ("max_gradient_error", f32),
+                ("compile_ms", C.c_double), ("link_ms", C.c_double), ("validate_ms", C.c_double)]
+
+
 # Expected sizes (src/ffi.rs:494-507 plus the new FptRayHit).
 EXPECTED_SIZES = {
     FptSdfInstruction: 32, FptPrimitiveInstance: 80, FptAffineTransform: 64,
     FptIndexedPrimitive: 32, FptTypedSoAProgram: 5136, FptStitchPipelineStats: 72,
     FptRenderConfig: 30448, FptSdfProfileStats: 288, FptDiagnosticConfig: 24,
     FptMandelbulberFieldSample: 16, FptAsyncJitStats: 40, FptRayHit: 40, FptVoxelBuildInfo: 40,
-    FptTuneResult: 24, FptProposerSettings: 16, FptProposerStats: 32,
+    FptTuneResult: 24, FptProposerSettings: 16, FptProposerStats: 32, FptSpecializationStats: 80,
 }
 
 PKG_DIR = Path(__file__).resolve().parent
@@ -263,7 +270,7 @@ EXPORTED_SYMBOLS = (
     "fpt_hip_autotune", "fpt_hip_device_alloc", "fpt_hip_device_free", "fpt_hip_copy", "fpt_hip_fill_f32",
     "fpt_hip_stream_create", "fpt_hip_stream_destroy", "fpt_hip_synchronize", "fpt_hip_time_renders",
     "fpt_hip_proposer_samples", "fpt_hip_proposer_eval", "fpt_hip_plan_set_proposer", "fpt_hip_plan_profile",
-    "fpt_hip_diagnostic_render",
+    "fpt_hip_diagnostic_render", "fpt_hip_plan_specialize", "fpt_hip_topology_source",
 )
 
 _lib = None
@@ -360,6 +367,10 @@ def _declare(lib):
     lib.fpt_hip_diagnostic_render.argtypes = [P(FptRenderConfig), P(FptDiagnosticConfig), P(C.c_uint8), C.c_size_t,
                                               P(C.c_double), cp, C.c_size_t]
     lib.fpt_hip_diagnostic_render.restype = C.c_int
+    lib.fpt_hip_plan_specialize.argtypes = [C.c_void_p, u32, P(FptSpecializationStats), cp, C.c_size_t]
+    lib.fpt_hip_plan_specialize.restype = C.c_int
+    lib.fpt_hip_topology_source.argtypes = [P(FptRenderConfig), cp, C.c_size_t, P(C.c_size_t), cp, C.c_size_t]
+    lib.fpt_hip_topology_source.restype = C.c_int
     lib.fpt_hip_last_error.argtypes = []
     lib.fpt_hip_last_error.restype = C.c_char_p
 

@@ -28,8 +28,17 @@
 #ifndef FPT_ABI_H
 #define FPT_ABI_H
 
+#ifndef __HIPCC_RTC__
 #include <stddef.h>
 #include <stdint.h>
+#else /* device-only build by the HIP runtime compiler: no C library headers */
+typedef unsigned char uint8_t;
+typedef unsigned short uint16_t;
+typedef unsigned int uint32_t;
+typedef unsigned long long uint64_t;
+typedef int int32_t;
+typedef long long int64_t;
+#endif
 
 #ifdef __cplusplus
 extern "C" {
@@ -870,6 +879,53 @@ int fpt_hip_diagnostic_render(const struct FptRenderConfig *config,
                               double *elapsed_ms,
                               char *error,
                               size_t error_len);
+
+/*
+ * Topology specialisation (the reference's runtime-source path,
+ * metal_bridge.mm:592-1300 and 4469-4500; validation Shaders.metal:
+ * 5055-5103).  A program's opcode sequence is compiled at run time with
+ * hipRTC into a straight-line distance / analytic-surface body inside the
+ * persistent path tracer; numeric parameters and CSG flags stay in the config,
+ * so one compiled module serves every program of that topology.  Modules are
+ * cached in memory and on disk ($FPT_HIPRTC_CACHE, default ~/.cache/fpt_hip).
+ * A module is attached only after 2^20 validation points agree with the
+ * interpreter: |d_s - d_i| <= 2e-5 (1 + |d_i|), gradient error <= 1e-4.
+ * fpt_metal_render does the same when config->sdf_topology_specialization is
+ * non-zero and reports through its stitch_* out-parameters.
+ */
+struct FptSpecializationStats {
+    uint32_t cache_status;       /* 1 compiled, 2 loaded from a cache */
+    uint32_t instruction_count;
+    uint64_t source_bytes;
+    uint64_t code_bytes;
+    uint64_t sample_count;
+    uint64_t distance_failures;
+    uint64_t gradient_failures;
+    float max_distance_error;
+    float max_gradient_error;
+    double compile_ms;           /* hipRTC compile, or cache read */
+    double link_ms;              /* module load */
+    double validate_ms;
+};
+FPT_ABI_ASSERT(sizeof(struct FptSpecializationStats) == 80, "FptSpecializationStats");
+
+/* enable 1: specialise (compile or fetch), validate and attach; enable 0:
+ * detach (the plan renders with the interpreter again).  Learned stepping
+ * and counting passes keep the interpreter kernel. */
+int fpt_hip_plan_specialize(FptHipPlan *plan,
+                            uint32_t enable,
+                            struct FptSpecializationStats *stats,
+                            char *error,
+                            size_t error_len);
+
+/* The generated source for `config`'s topology (NUL-terminated; *needed =
+ * bytes including the NUL; out may be NULL to query the size). */
+int fpt_hip_topology_source(const struct FptRenderConfig *config,
+                            char *out,
+                            size_t out_len,
+                            size_t *needed,
+                            char *error,
+                            size_t error_len);
 
 /*
  * Learned far-field step proposer.  No counterpart in the reference (its

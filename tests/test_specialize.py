@@ -1,0 +1,131 @@
+"""Topology specialisation (runtime-compiled straight-line programs,
+metal_bridge.mm:592-1300, 4469-4500; validation Shaders.metal:5055-5103).
+
+CPU: the generated source has one straight-line step per instruction and
+rejects what the interpreter cannot run.  GPU: the specialised kernel is
+validated on 2^20 points, renders bit-identically to the interpreter (the
+same per-instruction steps in the same order), is served from the memory
+and disk caches, and reports through fpt_metal_render's stitch outputs."""
+import os
+import re
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from fractalpathtracer_amd import api, ffi, scenes
+
+ROOT = Path(__file__).resolve().parents[1]
+GOLDEN = ROOT / "tests" / "golden"
+
+
+def _csg():
+    return scenes.apply_scene_json(scenes.default_config(), GOLDEN / "scenes" / "Exact_CSG.json")
+
+
+def test_generated_source_is_straight_line():
+    c = scenes.typed_program_scene(32, 32, 1)
+    src = api.topology_source(c)
+    n = c.sdf_program_count
+    steps = re.findall(r"eval_op<(\d+)u, false>\(ops\[(\d+)\]", src)
+    assert [int(i) for _, i in steps] == list(range(n))
+    assert [int(o) for o, _ in steps] == [c.sdf_program[i].opcode for i in range(n)]
+    surf = re.findall(r"surf_op<(\d+)u>\(ops\[(\d+)\]", src)
+    assert len(surf) == n and "#define FPT_TOPOLOGY_SPECIALIZED 1" in src
+    assert "fpt_topology_validate" in src and "for (" not in src.split("topology_distance")[1].split("}")[0]
+
+
+def test_generated_source_errors():
+    c = scenes.mandelbulb_scene(8, 8, 1)
+    c.sdf_id = ffi.SDF_MANDELBOX_FRACTAL
+    with pytest.raises(ffi.FptError, match="typed geometry program"):
+        api.topology_source(c)
+    c = _csg()
+    c.sdf_program[0].opcode = 77
+    with pytest.raises(ffi.FptError, match="unsupported opcode 77"):
+        api.topology_source(c)
+
+
+# --------------------------------------------------------------------------
+# GPU.
+# --------------------------------------------------------------------------
+
+def _render(cfg, specialize):
+    with api.Plan(cfg) as plan:
+        stats = plan.specialize() if specialize else None
+        buf = api.DeviceBuffer(cfg.width * cfg.height * 4)
+        plan.render_into(buf.address)
+        return buf.numpy().reshape(cfg.height, cfg.width, 4), stats
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scene", ["typed", "csg", "fold"])
+def test_specialized_render_is_bit_identical(scene):
+    cfg = {"typed": lambda: scenes.typed_program_scene(96, 64, 3),
+           "csg": lambda: _csg(),
+           "fold": lambda: scenes.apply_scene_json(scenes.default_config(),
+                                                   GOLDEN / "scenes" / "Typed_Program_Fold.json")}[scene]()
+    cfg.width, cfg.height, cfg.samples = 96, 64, 3
+    ref, _ = _render(cfg, False)
+    img, st = _render(cfg, True)
+    print(scene, st)
+    assert st["sample_count"] == 1 << 20 and st["distance_failures"] == 0 and st["gradient_failures"] == 0
+    assert st["cache_status"] in (1, 2) and st["code_bytes"] > 0
+    assert np.array_equal(img, ref)
+
+
+@pytest.mark.gpu
+def test_module_cache_serves_same_topology_with_new_parameters():
+    a = scenes.typed_program_scene(32, 32, 1)
+    b = scenes.typed_program_scene(32, 32, 1)
+    b.sdf_program[1].data[0] = -1.7       # new numbers, same opcodes
+    with api.Plan(a) as pa:
+        pa.specialize()
+    with api.Plan(b) as pb:
+        st = pb.specialize()
+    assert st["cache_status"] == 2 and st["compile_ms"] < 1000 and st["distance_failures"] == 0
+
+
+@pytest.mark.gpu
+def test_disk_cache_in_a_fresh_process(tmp_path):
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "from fractalpathtracer_amd import api, scenes\n"
+            "c = scenes.typed_program_scene(16, 16, 1)\n"
+            "with api.Plan(c) as p:\n"
+            "    print(p.specialize()['cache_status'])\n" % str(ROOT))
+    env = dict(os.environ, FPT_HIPRTC_CACHE=str(tmp_path))
+    first = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=300)
+    second = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=300)
+    assert first.returncode == 0 and second.returncode == 0, first.stderr[-1500:] + second.stderr[-1500:]
+    assert first.stdout.split()[-1] == "1" and second.stdout.split()[-1] == "2"
+    assert len(list(tmp_path.glob("topology-*.co"))) == 1
+
+
+@pytest.mark.gpu
+def test_metal_render_honours_topology_request_and_reports_stitch_stats():
+    cfg = scenes.typed_program_scene(64, 48, 2)
+    plain = api.render_full(cfg)
+    cfg.sdf_topology_specialization = 1
+    res = api.render_full(cfg)
+    st = res.stitch
+    assert st["cache_status"] in (1, 2)
+    assert st["validation"]["sample_count"] == 1 << 20
+    assert st["validation"]["distance_failures"] == 0 and st["validation"]["gradient_failures"] == 0
+    p = st["pipeline"]
+    assert p["instruction_count"] == cfg.sdf_program_count and p["primitive_instruction_count"] == 2
+    assert p["runtime_source_bytes"] > 0 and p["thread_execution_width"] == 64
+    assert np.array_equal(res.linear, plain.linear)
+
+
+@pytest.mark.gpu
+def test_specialization_errors():
+    vox = scenes.voxel_mandelbox_scene(16, 16, 1, resolution=32)
+    scenes.set_program(vox, [(ffi.OP_SPHERE, 0, [1.0])])
+    with api.Plan(vox) as p:
+        with pytest.raises(ffi.FptError, match="direct SDF renderer"):
+            p.specialize()
+    with api.Plan(scenes.voxel_mandelbox_scene(16, 16, 1, resolution=32)) as p:
+        with pytest.raises(ffi.FptError):
+            p.specialize()
