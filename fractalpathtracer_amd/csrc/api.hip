@@ -456,7 +456,7 @@ hipError_t launch_sdf(FptHipPlan *plan, float4 *acc, uint32_t start, uint32_t ch
         if (e != hipSuccess) return e;
     }
     uint32_t wg = plan->workgroups ? plan->workgroups : default_workgroups();
-    if (plan->spec && !plan->proposer.enabled && !prof) {
+    if (plan->spec && !plan->proposer.enabled && !prof && !fpt::single_bulb_program(c)) {
         // Same kernel, built for the program's topology (counting passes use
         // the interpreter: the work counts are the same).
         hipError_t e = fpt::launch_zero_counter(plan->counter, s);

@@ -360,6 +360,14 @@ __device__ __forceinline__ void bulb_step(v3<S> &z, S r, S &dr, float power, int
     }
 }
 
+// Logarithmic finalizer of the Mandelbulb orbit (ORB:654-660).
+template <class S>
+__device__ __forceinline__ S bulb_finish(S r, S dr) {
+    if (!(val(dr) > 0.0f)) return r;
+    if (val(r) > 1.0f) return (r * s_log(r) / dr) * 0.5f;
+    return sconst<S>(0.0f);
+}
+
 template <class S, int P>
 __device__ __forceinline__ Leaf<S> bulb_orbit(v3<S> p, float power, int ip, int iters, float bailout,
                                               int *orbit_counter) {
@@ -384,9 +392,7 @@ __device__ __forceinline__ Leaf<S> bulb_orbit(v3<S> p, float power, int ip, int 
     }
     if (orbit_counter) *orbit_counter += done;
     Leaf<S> out;
-    if (!(val(dr) > 0.0f)) out.d = r;
-    else if (val(r) > 1.0f) out.d = (r * s_log(r) / dr) * 0.5f;
-    else out.d = sconst<S>(0.0f);
+    out.d = bulb_finish(r, dr);
     out.radius = val(r);
     out.derivative = val(dr);
     out.trap = trap;

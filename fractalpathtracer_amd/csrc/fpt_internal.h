@@ -46,6 +46,9 @@ struct BatchScene {
 #ifndef __HIPCC_RTC__  // host launchers
 hipError_t launch_zero_counter(uint32_t *counter, hipStream_t stream);
 uint32_t persistent_slots(uint32_t width, uint32_t height);
+// The scene's program is one Mandelbulb leaf: the persistent kernel's
+// interleaved-orbit variant applies.
+bool single_bulb_program(const FptRenderConfig &c);
 hipError_t launch_sdf_batch(const FptRenderConfig *dcfgs, const BatchScene *dscenes, uint32_t nscenes,
                             uint32_t total_slots, float4 *accum, uint32_t *counter, uint32_t workgroups,
                             int event_batch, uint32_t spp_cap, DeviceProfile *prof, hipStream_t stream);

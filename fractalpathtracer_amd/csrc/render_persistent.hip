@@ -13,6 +13,11 @@ hipError_t launch_zero_counter(uint32_t *counter, hipStream_t stream) {
 
 uint32_t persistent_slots(uint32_t width, uint32_t height) { return slot_count(width, height); }
 
+bool single_bulb_program(const FptRenderConfig &c) {
+    return FPT_INTERLEAVED_BULB && c.sdf_id == FPT_SDF_PROGRAM && c.sdf_program_count == 1u &&
+           c.sdf_program[0].opcode == FPT_SDF_OP_MANDELBULB;
+}
+
 hipError_t launch_sdf_persistent(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, float4 *accum,
                                  uint32_t start, uint32_t count, uint32_t *counter, uint32_t workgroups,
                                  int event_batch, DeviceProfile *prof, const ProposerDevice *prop,
@@ -28,6 +33,13 @@ hipError_t launch_sdf_persistent(const FptRenderConfig *dcfg, const FptRenderCon
         else
             hipLaunchKernelGGL((sdf_persistent_kernel<false, false, true>), grid, block, 0, stream, dcfg, nullptr, 1u,
                                accum, start, count, counter, slots, event_batch, nullptr, *prop);
+    } else if (single_bulb_program(hcfg)) {
+        if (prof)
+            hipLaunchKernelGGL((sdf_persistent_kernel<true, false, false, true>), grid, block, 0, stream, dcfg, nullptr,
+                               1u, accum, start, count, counter, slots, event_batch, prof, none);
+        else
+            hipLaunchKernelGGL((sdf_persistent_kernel<false, false, false, true>), grid, block, 0, stream, dcfg,
+                               nullptr, 1u, accum, start, count, counter, slots, event_batch, nullptr, none);
     } else if (prof) {
         hipLaunchKernelGGL((sdf_persistent_kernel<true, false, false>), grid, block, 0, stream, dcfg, nullptr, 1u,
                            accum, start, count, counter, slots, event_batch, prof, none);

@@ -193,3 +193,24 @@ def test_png_output_and_blank_guard(tmp_path):
     blank.camera_yaw_pitch[0] = 3.0
     with pytest.raises(ffi.FptError, match="blank or single-colour"):
         api.render_full(blank, output_png=str(tmp_path / "b.png"))
+
+
+@pytest.mark.parametrize("power,iterations,bailout", [(8.0, 12, 2.0), (3.0, 9, 2.0), (7.5, 12, 2.0)])
+def test_single_mandelbulb_leaf_matches_the_generic_program_march(power, iterations, bailout):
+    """A program that is one Mandelbulb leaf runs the persistent kernel's
+    interleaved-orbit instance (one orbit iteration per trip, finalisation
+    batched over finished lanes); prefixing a zero translation makes the
+    same field a two-op program, which runs the generic program march.  The
+    images and the work counters must be identical: integer power 8 and 3
+    take the polynomial orbit step, 7.5 the trigonometric one."""
+    cfg = scenes.mandelbulb_scene(72, 56, 3, bounces=2, power=power, iterations=iterations, bailout=bailout)
+    generic = scenes.clone(cfg)
+    scenes.set_program(generic, [(ffi.OP_TRANSLATE, 0, [0.0, 0.0, 0.0, 0.0]),
+                                 (ffi.OP_MANDELBULB, 0, [power, float(iterations), bailout, 0.0])])
+    assert np.array_equal(api.render(cfg), api.render(generic))
+    a = api.render_full(cfg, profile=True)
+    b = api.render_full(generic, profile=True)
+    assert np.array_equal(a.linear, b.linear)
+    assert np.isfinite(a.linear).all() and a.linear[..., :3].max() > 0
+    for key in ("distance_evals", "march_orbit_iterations", "primary_steps", "shadow_steps", "normal_evals"):
+        assert a.profile[key] == b.profile[key], key
