@@ -545,6 +545,19 @@ int fpt_hip_plan_create(const FptRenderConfig *config, FptHipPlan **plan, char *
         fpt_hip_plan_destroy(p);
         return 1;
     }
+    if (config->renderer_backend == FPT_RENDERER_SDF && config->camera_dof > 0.0f && !(config->focus_distance > 0.0f)) {
+        // Focus prepass (metal_bridge.mm:4827-4838, 5455-5481): the estimate
+        // replaces the unset focus distance in the device config and in the
+        // plan's host copy.
+        e = fpt::launch_focus_prepass(p->dcfg, 1u, nullptr);
+        if (e == hipSuccess)
+            e = hipMemcpy(&p->host.focus_distance, &p->dcfg->focus_distance, sizeof(float), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            set_error(error, error_len, "focus-distance prepass failed: %s", hipGetErrorString(e));
+            fpt_hip_plan_destroy(p);
+            return 1;
+        }
+    }
     if (config->renderer_backend == FPT_RENDERER_VOXEL) {
         auto t0 = std::chrono::steady_clock::now();
         if (fpt::voxel_build(*config, p->dcfg, p->voxels, nullptr, error, error_len)) {
@@ -866,6 +879,15 @@ int fpt_hip_batch_create(const FptRenderConfig *configs, uint32_t count, FptHipB
     if (e == hipSuccess) e = hipMemcpy(b->dcfgs, configs, sizeof(FptRenderConfig) * count, hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(b->dscenes, b->scenes.data(), sizeof(fpt::BatchScene) * count, hipMemcpyHostToDevice);
+    // Focus prepass of every scene with depth of field and no focus distance
+    // (device configs only: the host copies keep the caller's values).
+    bool need_focus = false;
+    for (uint32_t i = 0; i < count; ++i)
+        need_focus = need_focus || (configs[i].camera_dof > 0.0f && !(configs[i].focus_distance > 0.0f));
+    if (e == hipSuccess && need_focus) {
+        e = fpt::launch_focus_prepass(b->dcfgs, count, nullptr);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+    }
     if (e != hipSuccess) {
         set_error(error, error_len, "batch upload failed: %s", hipGetErrorString(e));
         fpt_hip_batch_destroy(b);

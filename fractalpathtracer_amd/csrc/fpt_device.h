@@ -995,6 +995,34 @@ __device__ f3 march(f3 dr, f3 rp, int ni, float min_dist, float lod_falloff, con
     return rp;
 }
 
+// estimateFocusDistance (SM:5869-5878): the centre ray's hit distance, or 5
+// when it misses, is degenerate or passes 0.99 max_distance.
+__device__ float focus_estimate(const FptRenderConfig &cfg, Work &w) {
+    const float focal = 1.0f / tanf(cfg.camera_fov / 2.0f * kPi / 180.0f);
+    const f3 dr = rotate_camera_roll(normalize(mk(0.f, 0.f, focal)), cfg.camera_yaw_pitch[0], cfg.camera_yaw_pitch[1],
+                                     cfg.camera_roll);
+    const f3 cam = ld3(cfg.camera_position);
+    int steps = 0;
+    const f3 hit = march(dr, cam, 120, 0.001f, 0.001f, cfg, w, steps);
+    const float d = length(hit - cam);
+    if (!isfinite(d) || d <= 0.001f || d > cfg.render[4] * 0.99f) return 5.0f;
+    return d;
+}
+
+// Thin-lens depth of field of the camera ray (SM:5950-5956): the ray keeps
+// its point at the focus distance and starts from a lens offset of radius
+// camera_dof.  Every render entry point runs the focus prepass before its
+// kernels (focus_prepass_kernel), so a config with depth of field always
+// carries a positive focus distance here.
+__device__ __forceinline__ void apply_dof(UV xy, float frame, const FptRenderConfig &cfg, f3 &rp, f3 &dr) {
+    if (!(cfg.camera_dof > 0.0f)) return;
+    const f3 fp = rp + dr * cfg.focus_distance;
+    float lx, ly;
+    random_point(cfg.camera_dof, xy, frame, lx, ly);
+    rp = rp + rotate_camera_roll(mk(lx, ly, 0.f), cfg.camera_yaw_pitch[0], cfg.camera_yaw_pitch[1], cfg.camera_roll);
+    dr = normalize(fp - rp);
+}
+
 __device__ __forceinline__ f3 normalize_fd(f3 g) {  // SM:3863-3869
     float m = fmaxf(fabsf(g.x), fmaxf(fabsf(g.y), fabsf(g.z)));
     if (!(m > 0.0f) || !isfinite(m)) return mk(0.f, 1.f, 0.f);

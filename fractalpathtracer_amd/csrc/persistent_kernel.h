@@ -140,7 +140,7 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
     bool check_t = false, sun_on = false;
     int bounces = 0, ni_first = 0, ni_next = 0;
     f3 sun_dir0 = mk(0.f, 0.f, 1.f);
-    auto load_scene = [&]() {
+    auto load_scene = [&]() __attribute__((always_inline)) {
         const FptRenderConfig &cfg = *cp;
         const uint32_t total = cfg.samples < 1u ? 1u : (cfg.samples > 512u ? 512u : cfg.samples);
         s_begin = MULTI ? 0u : start;
@@ -207,13 +207,15 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
     bool retry = false;  // slot was tile padding: take another
 
     // Start the camera path of sample `s` of the current pixel.
-    auto start_path = [&]() {
+    auto start_path = [&]() __attribute__((always_inline)) {
         const FptRenderConfig &cfg = *cp;
         const float frame = float(s);
         float jx, jy;
         random_point(aa, uv, frame, jx, jy);
         dr = rotate_camera_roll(normalize(mk(uv.x + jx, uv.y + jy, focal)), cfg.camera_yaw_pitch[0],
                                 cfg.camera_yaw_pitch[1], cfg.camera_roll);
+        p = cam;
+        apply_dof(uv, frame, cfg, p, dr);
         const float gt = clampf((dr.y + 1.0f) / 2.0f, 0.f, 1.f);
         gcol = vmix(ld3(&cfg.background_gradient[3]), ld3(&cfg.background_gradient[0]), gt);
         bounce = 0;
@@ -222,8 +224,7 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
         sky_mask = 0.0f;
         light = mk(0.f, 0.f, 0.f);
         color = mk(1.f, 1.f, 1.f);
-        p = cam;
-        mstart = cam;
+        mstart = p;
         mdir = dr;
         it = 0;
         max_iter = local_ni < 360 ? local_ni : 360;
@@ -235,7 +236,7 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
     };
 
     // Take a work slot (or go idle).  Wave-uniform call.
-    auto take_slot = [&](bool want) {
+    auto take_slot = [&](bool want) __attribute__((always_inline)) {
         const uint32_t k = fetch_slot(want, counter);
         retry = false;
         if (!want) return;

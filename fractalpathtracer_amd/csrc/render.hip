@@ -41,7 +41,7 @@ struct PathCounts {
     int primary_steps, secondary_steps, shadow_steps, normal_evals, bounces;
 };
 
-// renderPath (SM:5926-6026); depth of field is not part of this library.
+// renderPath (SM:5926-6026).
 __device__ f3 render_path(UV xy, uint32_t sample_idx, const FptRenderConfig &cfg, Work &w, PathCounts &pc) {
     const float frame = float(sample_idx);
     const f3 cam = ld3(cfg.camera_position);
@@ -52,6 +52,7 @@ __device__ f3 render_path(UV xy, uint32_t sample_idx, const FptRenderConfig &cfg
     const float focal = 1.0f / tanf(cfg.camera_fov / 2.0f * kPi / 180.0f);
     f3 dr = rotate_camera_roll(normalize(mk(xy.x + jx, xy.y + jy, focal)), cfg.camera_yaw_pitch[0],
                                cfg.camera_yaw_pitch[1], cfg.camera_roll);
+    apply_dof(xy, frame, cfg, rp, dr);
     int local_ni = int(cfg.render[1]);
     float side = 1.0f;
     f3 light = mk(0.f, 0.f, 0.f), color = mk(1.f, 1.f, 1.f);
@@ -323,6 +324,7 @@ __device__ f3 path_cost_diagnostic(UV xy, const FptRenderConfig &cfg, const FptD
         const float frame = float(si);
         f3 rp = cam;
         f3 dr = jittered_camera_ray(xy, frame, cfg);
+        apply_dof(xy, frame, cfg, rp, dr);  // SM:6933-6939, 7222-7228
         int local_ni = int(cfg.render[1]);
         for (int i = 0; i < bounces; ++i) {
             int steps = 0;
@@ -393,6 +395,7 @@ __device__ f3 bounce_contribution_diagnostic(UV xy, const FptRenderConfig &cfg, 
         const float frame = float(si);
         f3 rp = cam;
         f3 dr = jittered_camera_ray(xy, frame, cfg);
+        apply_dof(xy, frame, cfg, rp, dr);  // SM:6933-6939, 7222-7228
         int local_ni = int(cfg.render[1]);
         f3 thr = mk(1.f, 1.f, 1.f);
         for (int i = 0; i < bounces; ++i) {
@@ -547,6 +550,27 @@ hipError_t launch_sdf_accumulate(const FptRenderConfig *dcfg, const FptRenderCon
 hipError_t launch_trace_primary(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, FptRayHit *hits,
                                 hipStream_t stream) {
     hipLaunchKernelGGL(trace_primary_kernel, tile_grid(hcfg.width, hcfg.height), dim3(256), 0, stream, dcfg, hits);
+    return hipGetLastError();
+}
+
+// estimate_focus_distance_kernel (SM:5880-5884), as the host's focus
+// prepass uses it (metal_bridge.mm:4827-4838, 5455-5481): the estimate is
+// written into the device config, so every later kernel of the render reads
+// it as the configured focus distance.  One thread per config; configs
+// without depth of field or with a focus distance set are left as they are.
+__global__ void focus_prepass_kernel(FptRenderConfig *cfgs, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    FptRenderConfig &cfg = cfgs[i];
+    if (!(cfg.camera_dof > 0.0f) || cfg.focus_distance > 0.0f) return;
+    Work w = {0, 0};
+    const float f = focus_estimate(cfg, w);
+    cfg.focus_distance = f;
+}
+
+hipError_t launch_focus_prepass(FptRenderConfig *dcfgs, uint32_t n, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(focus_prepass_kernel, dim3((n + 63u) / 64u), dim3(64), 0, stream, dcfgs, n);
     return hipGetLastError();
 }
 

@@ -43,6 +43,8 @@ def lib():
         L.orc_render_samples.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32,
                                          C.c_uint32, dp]
         L.orc_trace_primary.argtypes = [C.c_void_p, C.c_int, C.c_int, dp]
+        L.orc_focus_estimate.argtypes = [C.c_void_p, C.c_int]
+        L.orc_focus_estimate.restype = C.c_double
         L.orc_map_distance.argtypes = [C.c_void_p, C.c_int, dp, C.c_size_t, dp]
         L.orc_normal.argtypes = [C.c_void_p, C.c_int, dp, C.c_size_t, dp]
         L.orc_program_surface.argtypes = [C.c_void_p, C.c_int, dp, C.c_size_t, dp]
@@ -150,6 +152,11 @@ def counters() -> dict:
     keys = ("distance_evals", "orbit_iterations", "primary_steps", "secondary_steps",
             "shadow_steps", "normal_evals", "paths", "far_steps", "far_orbit_iterations")
     return dict(zip(keys, list(a)))
+
+
+def focus_estimate(cfg, precision: int = 64) -> float:
+    """estimateFocusDistance of the config's camera (FP64 or the FP32 twin)."""
+    return float(lib().orc_focus_estimate(C.byref(cfg), precision))
 
 
 def trace_primary(cfg, precision: int = 64, threads: int = 0) -> np.ndarray:

@@ -923,6 +923,43 @@ V3<T> march(V3<T> dr, V3<T> rp, int ni, T min_dist, T lod_falloff, const FptRend
     return rp;
 }
 
+// estimateFocusDistance (SM:5869-5878): hit distance of the un-jittered
+// centre ray (120 steps, threshold and LOD falloff 0.001), or 5 when it is
+// not finite, within 0.001 of the camera or beyond 0.99 max_distance.
+template <class T>
+T focus_estimate(const FptRenderConfig &cfg) {
+    const V3<T> dr = camera_dir<T>(T(0), T(0), cfg);
+    const V3<T> cam = load3<T>(cfg.camera_position);
+    const V3<T> hit = march(dr, cam, 120, T(0.001f), T(0.001f), cfg, nullptr);
+    const T d = length(hit - cam);
+    if (!std::isfinite(d) || d <= T(0.001f) || d > T(cfg.render[4]) * T(0.99f)) return T(5);
+    return d;
+}
+
+// Thin-lens camera (SM:5950-5956): keep the ray's point at the focus
+// distance, start it from a lens offset of radius camera_dof.
+template <class T>
+void apply_dof(UV xy, float frame, const FptRenderConfig &cfg, V3<T> &rp, V3<T> &dr) {
+    if (!(cfg.camera_dof > 0.0f)) return;
+    const T focus = cfg.focus_distance > 0.0f ? T(cfg.focus_distance) : focus_estimate<T>(cfg);
+    const V3<T> fp = rp + dr * focus;
+    T lx, ly;
+    random_point<T>(T(cfg.camera_dof), xy, frame, lx, ly);
+    rp = rp + rotate_camera_roll(V3<T>(lx, ly, T(0)), T(cfg.camera_yaw_pitch[0]), T(cfg.camera_yaw_pitch[1]),
+                                 T(cfg.camera_roll));
+    dr = normalize(fp - rp);
+}
+
+// The render entry points' focus prepass (metal_bridge.mm:4827-4838,
+// 5455-5481): a config with depth of field and no focus distance gets the
+// estimate, rounded to the config's float field.
+template <class T>
+FptRenderConfig with_focus(const FptRenderConfig &cfg) {
+    FptRenderConfig c = cfg;
+    if (c.camera_dof > 0.0f && !(c.focus_distance > 0.0f)) c.focus_distance = float(focus_estimate<T>(c));
+    return c;
+}
+
 template <class T>
 V3<T> normalize_fd(V3<T> g) {  // SM:3863-3869
     T m = m_max(std::fabs(g.x), m_max(std::fabs(g.y), std::fabs(g.z)));
@@ -1035,7 +1072,7 @@ V3<T> sun_contribution(V3<T> rp, UV xy, float seed, const Material<T> &mat, V3<T
     return V3<T>();
 }
 
-// renderPath (SM:5926-6026); DOF excluded (configs use dof = 0).
+// renderPath (SM:5926-6026).
 template <class T>
 V3<T> render_path(UV xy, uint32_t sample_idx, const FptRenderConfig &cfg) {
     float frame = float(sample_idx);
@@ -1046,6 +1083,7 @@ V3<T> render_path(UV xy, uint32_t sample_idx, const FptRenderConfig &cfg) {
     T jx, jy;
     random_point<T>(aa, xy, frame, jx, jy);
     V3<T> dr = camera_dir<T>(T(xy.x) + jx, T(xy.y) + jy, cfg);
+    apply_dof<T>(xy, frame, cfg, rp, dr);
     int local_ni = int(cfg.render[1]);
     T side = T(1);
     V3<T> light, color(T(1), T(1), T(1));
@@ -1206,8 +1244,8 @@ int orc_render(const FptRenderConfig *cfg, int precision, int threads, uint32_t 
     if (!cfg || !accum || cfg->width == 0 || cfg->height == 0) return 1;
     if (row_end > cfg->height || row_begin >= row_end) return 1;
     const uint32_t samples = std::min<uint32_t>(std::max<uint32_t>(cfg->samples, 1u), 512u);
-    if (precision == 64) render_rows<double>(*cfg, threads, row_begin, row_end, 0u, samples, accum);
-    else render_rows<float>(*cfg, threads, row_begin, row_end, 0u, samples, accum);
+    if (precision == 64) render_rows<double>(with_focus<double>(*cfg), threads, row_begin, row_end, 0u, samples, accum);
+    else render_rows<float>(with_focus<float>(*cfg), threads, row_begin, row_end, 0u, samples, accum);
     return 0;
 }
 
@@ -1215,9 +1253,16 @@ int orc_render_samples(const FptRenderConfig *cfg, int precision, int threads, u
                        uint32_t row_end, uint32_t sample_begin, uint32_t sample_end, double *accum) {
     if (!cfg || !accum || cfg->width == 0 || cfg->height == 0) return 1;
     if (row_end > cfg->height || row_begin >= row_end || sample_begin >= sample_end) return 1;
-    if (precision == 64) render_rows<double>(*cfg, threads, row_begin, row_end, sample_begin, sample_end, accum);
-    else render_rows<float>(*cfg, threads, row_begin, row_end, sample_begin, sample_end, accum);
+    if (precision == 64)
+        render_rows<double>(with_focus<double>(*cfg), threads, row_begin, row_end, sample_begin, sample_end, accum);
+    else
+        render_rows<float>(with_focus<float>(*cfg), threads, row_begin, row_end, sample_begin, sample_end, accum);
     return 0;
+}
+
+double orc_focus_estimate(const FptRenderConfig *cfg, int precision) {
+    if (!cfg) return -1.0;
+    return precision == 64 ? focus_estimate<double>(*cfg) : double(focus_estimate<float>(*cfg));
 }
 
 int orc_trace_primary(const FptRenderConfig *cfg, int precision, int threads, double *out) {
@@ -1362,6 +1407,7 @@ int orc_diagnostic(const FptRenderConfig *cfg, uint32_t mode, uint32_t target_bo
         if (!vx_build(*cfg, threads, t)) return 1;
         focus = vx_focus(*cfg, t);
     }
+    const FptRenderConfig sdf_cfg = voxel ? *cfg : (precision == 64 ? with_focus<double>(*cfg) : with_focus<float>(*cfg));
     const uint32_t W = cfg->width, H = cfg->height;
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 2) num_threads(threads > 0 ? threads : omp_get_max_threads())
@@ -1374,10 +1420,10 @@ int orc_diagnostic(const FptRenderConfig *cfg, uint32_t mode, uint32_t target_bo
                 V3<float> c = voxel_diagnostic(uv, *cfg, t, focus, mode, max_distance);
                 store_rgba8(o, c.x, c.y, c.z);
             } else if (precision == 64) {
-                V3<double> c = sdf_diagnostic<double>(uv, *cfg, mode, target_bounce, max_distance);
+                V3<double> c = sdf_diagnostic<double>(uv, sdf_cfg, mode, target_bounce, max_distance);
                 store_rgba8(o, c.x, c.y, c.z);
             } else {
-                V3<float> c = sdf_diagnostic<float>(uv, *cfg, mode, target_bounce, max_distance);
+                V3<float> c = sdf_diagnostic<float>(uv, sdf_cfg, mode, target_bounce, max_distance);
                 store_rgba8(o, c.x, c.y, c.z);
             }
         }
