@@ -156,6 +156,16 @@ def test_voxel_dda() -> None:
     ffi.check(lib.fpt_test_voxel_dda(None, err, len(err)), err)
 
 
+def hydrate_hdri(cfg: FptRenderConfig) -> FptRenderConfig:
+    """A copy of `cfg` with its HDRI LUT loaded from hdri_path, as every
+    render entry point does (fpt_hip_hydrate_hdri; host only)."""
+    lib = ffi.load_library()
+    out = FptRenderConfig.from_buffer_copy(cfg)
+    err = ffi.error_buffer()
+    ffi.check(lib.fpt_hip_hydrate_hdri(C.byref(out), err, len(err)), err)
+    return out
+
+
 def topology_source(cfg: FptRenderConfig) -> str:
     """Generated topology-specialised source of cfg's program."""
     lib = ffi.load_library()

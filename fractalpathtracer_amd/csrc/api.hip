@@ -536,6 +536,11 @@ int fpt_hip_plan_create(const FptRenderConfig *config, FptHipPlan **plan, char *
     *plan = nullptr;
     if (validate_config(config, error, error_len)) return 1;
     if (check_device(error, error_len)) return 1;
+    // The HDRI LUT always comes from the scene's HDRI file (metal_bridge.mm:
+    // 4330-4332): the plan works on the hydrated copy.
+    FptRenderConfig hydrated = *config;
+    if (!fpt::hydrate_hdri(hydrated, error, error_len)) return 1;
+    config = &hydrated;
     FptHipPlan *p = new FptHipPlan();
     p->host = *config;
     hipError_t e = hipMalloc(&p->dcfg, sizeof(FptRenderConfig));
@@ -569,6 +574,14 @@ int fpt_hip_plan_create(const FptRenderConfig *config, FptHipPlan **plan, char *
     }
     *plan = p;
     return 0;
+}
+
+int fpt_hip_hydrate_hdri(FptRenderConfig *config, char *error, size_t error_len) {
+    if (!config) {
+        set_error(error, error_len, "missing render config");
+        return 1;
+    }
+    return fpt::hydrate_hdri(*config, error, error_len) ? 0 : 1;
 }
 
 int fpt_hip_plan_render(FptHipPlan *plan, float *device_accum, void *stream, char *error, size_t error_len) {
@@ -854,6 +867,12 @@ int fpt_hip_batch_create(const FptRenderConfig *configs, uint32_t count, FptHipB
     if (check_device(error, error_len)) return 1;
     FptHipBatch *b = new FptHipBatch();
     b->host.assign(configs, configs + count);
+    for (uint32_t i = 0; i < count; ++i) {
+        if (!fpt::hydrate_hdri(b->host[i], error, error_len)) {
+            fpt_hip_batch_destroy(b);
+            return 1;
+        }
+    }
     uint64_t slot = 0, off = 0;
     for (uint32_t i = 0; i < count; ++i) {
         fpt::BatchScene sc;
@@ -876,7 +895,8 @@ int fpt_hip_batch_create(const FptRenderConfig *configs, uint32_t count, FptHipB
     hipError_t e = hipMalloc(&b->dcfgs, sizeof(FptRenderConfig) * count);
     if (e == hipSuccess) e = hipMalloc(&b->dscenes, sizeof(fpt::BatchScene) * count);
     if (e == hipSuccess) e = hipMalloc(&b->counter, sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMemcpy(b->dcfgs, configs, sizeof(FptRenderConfig) * count, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(b->dcfgs, b->host.data(), sizeof(FptRenderConfig) * count, hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(b->dscenes, b->scenes.data(), sizeof(fpt::BatchScene) * count, hipMemcpyHostToDevice);
     // Focus prepass of every scene with depth of field and no focus distance

@@ -1074,7 +1074,39 @@ __device__ f3 sky(f3 dr, f3 sun) {  // SM:5764-5787
     return col * 0.8f + mk(disk, disk, disk) * 100.0f;
 }
 
-__device__ f3 environment(f3 view, const FptRenderConfig &cfg) {  // SM:5823-5844 (no HDRI)
+// One channel of an HDRI LUT texel (SM:5796-5801): FP16, row-major, RGB.
+__device__ __forceinline__ float hdri_texel(const FptRenderConfig &cfg, uint32_t x, uint32_t y, uint32_t c, uint32_t w,
+                                            uint32_t h) {
+    const uint32_t i = ((y < h - 1u ? y : h - 1u) * w + (x < w - 1u ? x : w - 1u)) * 3u + c;
+    return float(__builtin_bit_cast(_Float16, cfg.hdri_lut[i]));
+}
+
+// sampleHdriLut (SM:5803-5821): bilinear lookup of the equirectangular LUT
+// (u = azimuth atan2(z, x), wrapping; v = polar angle from +y, clamped).
+// fract follows the shading language's definition, which never returns 1.
+__device__ f3 sample_hdri(f3 direction, const FptRenderConfig &cfg) {
+    const uint32_t w = cfg.hdri_width < 1u ? 1u : (cfg.hdri_width > 32u ? 32u : cfg.hdri_width);
+    const uint32_t h = cfg.hdri_height < 1u ? 1u : (cfg.hdri_height > 16u ? 16u : cfg.hdri_height);
+    auto fr = [](float x) { return fminf(x - floorf(x), 0x1.fffffep-1f); };
+    const f3 d = normalize(direction);
+    const float u = fr(atan2f(d.z, d.x) / (2.0f * kPi) + 0.5f);
+    const float v = clampf(acosf(clampf(d.y, -1.0f, 1.0f)) / kPi, 0.0f, 1.0f);
+    const float x = u * float(w) - 0.5f;
+    const float y = v * float(h - 1u);
+    const int x0i = int(floorf(x));
+    const uint32_t x0 = uint32_t((x0i % int(w) + int(w)) % int(w));
+    const uint32_t x1 = (x0 + 1u) % w;
+    const int y0i = int(floorf(y));
+    const uint32_t y0 = uint32_t(y0i < 0 ? 0 : (y0i > int(h - 1u) ? int(h - 1u) : y0i));
+    const uint32_t y1 = y0 + 1u < h - 1u ? y0 + 1u : h - 1u;
+    const float fx = fr(x), fy = fr(y);
+    auto texel = [&](uint32_t tx, uint32_t ty) {
+        return mk(hdri_texel(cfg, tx, ty, 0u, w, h), hdri_texel(cfg, tx, ty, 1u, w, h), hdri_texel(cfg, tx, ty, 2u, w, h));
+    };
+    return vmix(vmix(texel(x0, y0), texel(x1, y0), fx), vmix(texel(x0, y1), texel(x1, y1), fx), fy);
+}
+
+__device__ f3 environment(f3 view, const FptRenderConfig &cfg) {  // SM:5823-5844
     if (cfg.world[4] < 0.00001f) return mk(0.f, 0.f, 0.f);
     f3 ld = rotate_camera(mk(0.f, 0.f, 1.f), cfg.world[2] * kPi / 180.0f, cfg.world[3] * kPi / 180.0f);
     f3 env;
@@ -1084,6 +1116,8 @@ __device__ f3 environment(f3 view, const FptRenderConfig &cfg) {  // SM:5823-584
         env = mk(l, l, l);
     } else if (cfg.world[0] == 1.0f) {
         env = sky(view, ld);
+    } else if (cfg.world[0] == 2.0f && cfg.hdri_enabled != 0u && cfg.hdri_width > 0u) {
+        env = sample_hdri(rotate_camera(view, cfg.world[2] * kPi / 180.0f, 0.0f), cfg);
     } else if (cfg.world[0] == 3.0f) {
         env = ld3(cfg.world_one_color);
     } else {

@@ -55,6 +55,11 @@ hipError_t launch_sdf_batch(const FptRenderConfig *dcfgs, const BatchScene *dsce
 
 hipError_t launch_sdf_accumulate(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, float4 *accum,
                                  uint32_t start, uint32_t count, DeviceProfile *prof, hipStream_t stream);
+// HDRI hydration of a host config (hdri.hip): clears the LUT, then, for
+// world mode 2 with HDRI enabled, fills it from the Radiance file at
+// hdri_path.  Returns false with a message on failure.
+bool hydrate_hdri(FptRenderConfig &cfg, char *error, size_t error_len);
+uint16_t half_bits_round_up(float v);
 // Writes the focus estimate into each of the n device configs that has
 // depth of field and no focus distance.
 hipError_t launch_focus_prepass(FptRenderConfig *dcfgs, uint32_t n, hipStream_t stream);

@@ -117,8 +117,11 @@ __host__ __device__ __forceinline__ uint32_t slot_count(uint32_t w, uint32_t h) 
 #endif
 
 template <bool PROFILE, bool MULTI, bool LEARNED, bool BULB = false>
+// flatten: every device function the kernel reaches is inlined, so no call
+// (with its stack traffic for live state) ever sits on the march or the
+// per-hit path, however the inliner's size budget would decide.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LEARNED ? FPT_LEARNED_WAVES
-                                                                             : FPT_PERSISTENT_WAVES)))
+                                                                             : FPT_PERSISTENT_WAVES), flatten))
 void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
                                                              const BatchScene *__restrict__ scenes, uint32_t nscenes,
                                                              float4 *__restrict__ accum, uint32_t start,

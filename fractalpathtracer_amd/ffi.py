@@ -270,7 +270,7 @@ EXPORTED_SYMBOLS = (
     "fpt_hip_autotune", "fpt_hip_device_alloc", "fpt_hip_device_free", "fpt_hip_copy", "fpt_hip_fill_f32",
     "fpt_hip_stream_create", "fpt_hip_stream_destroy", "fpt_hip_synchronize", "fpt_hip_time_renders",
     "fpt_hip_proposer_samples", "fpt_hip_proposer_eval", "fpt_hip_plan_set_proposer", "fpt_hip_plan_profile",
-    "fpt_hip_diagnostic_render", "fpt_hip_plan_specialize", "fpt_hip_topology_source",
+    "fpt_hip_diagnostic_render", "fpt_hip_plan_specialize", "fpt_hip_topology_source", "fpt_hip_hydrate_hdri",
 )
 
 _lib = None
@@ -303,6 +303,8 @@ def _declare(lib):
     lib.fpt_hip_trace_primary.argtypes = [P(FptRenderConfig), P(FptRayHit), C.c_size_t, cp,
                                           C.c_size_t]
     lib.fpt_hip_trace_primary.restype = C.c_int
+    lib.fpt_hip_hydrate_hdri.argtypes = [P(FptRenderConfig), cp, C.c_size_t]
+    lib.fpt_hip_hydrate_hdri.restype = C.c_int
     lib.fpt_hip_plan_create.argtypes = [P(FptRenderConfig), P(C.c_void_p), cp, C.c_size_t]
     lib.fpt_hip_plan_create.restype = C.c_int
     lib.fpt_hip_plan_render.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, cp, C.c_size_t]

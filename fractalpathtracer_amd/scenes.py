@@ -82,9 +82,22 @@ def set_program(c: FptRenderConfig, ops) -> None:
             _set(dst.data, d[:4])
 
 
+def set_hdri_path(c: FptRenderConfig, path) -> None:
+    """world.hdri (src/scene.rs:873-879): the path as NUL-terminated bytes,
+    truncated to 255; HDRI is enabled iff the path is not empty.  The render
+    entry points load the file into the LUT when world mode is 2."""
+    data = str(path).encode()[: len(c.hdri_path) - 1]
+    C.memset(C.addressof(c) + type(c).hdri_path.offset, 0, len(c.hdri_path))
+    for i, b in enumerate(data):
+        c.hdri_path[i] = b
+    c.hdri_enabled = 1 if data else 0
+
+
 def apply_scene_json(c: FptRenderConfig, scene) -> FptRenderConfig:
     """Apply a scene JSON object (or path) to `c` (src/scene.rs:1619-1976 subset)."""
+    base = Path(".")
     if not isinstance(scene, dict):
+        base = Path(scene).parent
         scene = json.loads(Path(scene).read_text())
     for k in ("width", "height", "samples"):
         if k in scene:
@@ -114,6 +127,9 @@ def apply_scene_json(c: FptRenderConfig, scene) -> FptRenderConfig:
     w = scene.get("world", {})
     if "one_color" in w:
         _set(c.world_one_color, w["one_color"])
+    if isinstance(w.get("hdri"), str):  # relative to the scene file (src/scene.rs:864-871, 1958-1965)
+        h = Path(w["hdri"])
+        set_hdri_path(c, h if h.is_absolute() else base / h)
     obj("sun", c.sun, ["enabled", "rotation", "elevation", "power", "softness"])
     if "color" in scene.get("sun", {}):
         _set(c.sun_color, scene["sun"]["color"])

@@ -681,6 +681,17 @@ int fpt_hip_program_surface(const struct FptRenderConfig *config,
                             size_t error_len);
 
 /*
+ * HDRI hydration, in place, of a host config: clears hdri_width, hdri_height
+ * and hdri_lut, then, when world[0] == 2 and hdri_enabled, reads the Radiance
+ * RGBE file named by hdri_path and box-downsamples it into the 32 x 16 FP16
+ * LUT.  Every render entry point (and plan or batch creation) does this on its
+ * own copy of the config, as the reference's render calls do
+ * (metal_bridge.mm:2262-2300, 4330-4332, 5879-5881); this entry point exposes
+ * the same step to callers that want the table itself.  Host only.
+ */
+int fpt_hip_hydrate_hdri(struct FptRenderConfig *config, char *error, size_t error_len);
+
+/*
  * Device-resident render plan: the config (and any derived device tables:
  * voxel pool, step-proposer weights) uploaded once, so repeated renders
  * touch only HBM.  fpt_hip_plan_render enqueues the render on `stream`

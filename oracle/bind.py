@@ -43,6 +43,7 @@ def lib():
         L.orc_render_samples.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32,
                                          C.c_uint32, dp]
         L.orc_trace_primary.argtypes = [C.c_void_p, C.c_int, C.c_int, dp]
+        L.orc_environment.argtypes = [C.c_void_p, C.c_int, dp, C.c_size_t, dp]
         L.orc_focus_estimate.argtypes = [C.c_void_p, C.c_int]
         L.orc_focus_estimate.restype = C.c_double
         L.orc_map_distance.argtypes = [C.c_void_p, C.c_int, dp, C.c_size_t, dp]
@@ -152,6 +153,15 @@ def counters() -> dict:
     keys = ("distance_evals", "orbit_iterations", "primary_steps", "secondary_steps",
             "shadow_steps", "normal_evals", "paths", "far_steps", "far_orbit_iterations")
     return dict(zip(keys, list(a)))
+
+
+def environment(cfg, dirs, precision: int = 64) -> np.ndarray:
+    """environment() of the config for (n, 3) view directions."""
+    d = np.ascontiguousarray(np.asarray(dirs, np.float64).reshape(-1, 3))
+    out = np.zeros_like(d)
+    if lib().orc_environment(C.byref(cfg), precision, _dp(d), len(d), _dp(out)) != 0:
+        raise RuntimeError("orc_environment failed")
+    return out
 
 
 def focus_estimate(cfg, precision: int = 64) -> float:

@@ -1024,8 +1024,47 @@ V3<T> background_gradient(V3<T> dir, const FptRenderConfig &cfg) {  // SM:5789-5
     return vmix(load3<T>(&cfg.background_gradient[3]), load3<T>(&cfg.background_gradient[0]), t);
 }
 
+// IEEE half bits to float (exact).
+inline float half_to_float(uint16_t h) {
+    const int e = (h >> 10) & 31, m = h & 1023;
+    const float mag = e == 0 ? std::ldexp(float(m), -24)
+                             : (e == 31 ? (m ? std::numeric_limits<float>::quiet_NaN()
+                                             : std::numeric_limits<float>::infinity())
+                                        : std::ldexp(float(1024 + m), e - 25));
+    return (h & 0x8000u) ? -mag : mag;
+}
+
+// Shading-language fract: x - floor(x), never 1.
 template <class T>
-V3<T> environment(V3<T> view, const FptRenderConfig &cfg) {  // SM:5823-5844 (HDRI excluded)
+T sl_fract(T x) {
+    return m_min(x - std::floor(x), T(0x1.fffffep-1f));
+}
+
+template <class T>
+V3<T> sample_hdri(V3<T> direction, const FptRenderConfig &cfg) {  // SM:5796-5821
+    const uint32_t w = std::min(std::max(cfg.hdri_width, 1u), 32u);
+    const uint32_t h = std::min(std::max(cfg.hdri_height, 1u), 16u);
+    auto texel = [&](uint32_t x, uint32_t y) {
+        const uint32_t i = (std::min(y, h - 1u) * w + std::min(x, w - 1u)) * 3u;
+        return V3<T>(T(half_to_float(cfg.hdri_lut[i])), T(half_to_float(cfg.hdri_lut[i + 1])),
+                     T(half_to_float(cfg.hdri_lut[i + 2])));
+    };
+    const V3<T> d = normalize(direction);
+    const T u = sl_fract(std::atan2(d.z, d.x) / (T(2) * T(kPi)) + T(0.5));
+    const T v = m_clamp(std::acos(m_clamp(d.y, T(-1), T(1))) / T(kPi), T(0), T(1));
+    const T x = u * T(w) - T(0.5);
+    const T y = v * T(h - 1u);
+    const int x0i = int(std::floor(x));
+    const uint32_t x0 = uint32_t((x0i % int(w) + int(w)) % int(w));
+    const uint32_t x1 = (x0 + 1u) % w;
+    const uint32_t y0 = uint32_t(std::min(std::max(int(std::floor(y)), 0), int(h - 1u)));
+    const uint32_t y1 = std::min(y0 + 1u, h - 1u);
+    const T fx = sl_fract(x), fy = sl_fract(y);
+    return vmix(vmix(texel(x0, y0), texel(x1, y0), fx), vmix(texel(x0, y1), texel(x1, y1), fx), fy);
+}
+
+template <class T>
+V3<T> environment(V3<T> view, const FptRenderConfig &cfg) {  // SM:5823-5844
     if (T(cfg.world[4]) < T(0.00001f)) return V3<T>();
     V3<T> ld = rotate_camera(V3<T>(T(0), T(0), T(1)), T(cfg.world[2]) * T(kPi) / T(180),
                              T(cfg.world[3]) * T(kPi) / T(180));
@@ -1036,6 +1075,8 @@ V3<T> environment(V3<T> view, const FptRenderConfig &cfg) {  // SM:5823-5844 (HD
         env = V3<T>(l, l, l);
     } else if (cfg.world[0] == 1.0f) {
         env = sky(view, ld);
+    } else if (cfg.world[0] == 2.0f && cfg.hdri_enabled != 0u && cfg.hdri_width > 0u) {
+        env = sample_hdri(rotate_camera(view, T(cfg.world[2]) * T(kPi) / T(180), T(0)), cfg);
     } else if (cfg.world[0] == 3.0f) {
         env = load3<T>(cfg.world_one_color);
     } else {
@@ -1257,6 +1298,21 @@ int orc_render_samples(const FptRenderConfig *cfg, int precision, int threads, u
         render_rows<double>(with_focus<double>(*cfg), threads, row_begin, row_end, sample_begin, sample_end, accum);
     else
         render_rows<float>(with_focus<float>(*cfg), threads, row_begin, row_end, sample_begin, sample_end, accum);
+    return 0;
+}
+
+int orc_environment(const FptRenderConfig *cfg, int precision, const double *dirs, size_t n, double *out) {
+    if (!cfg || (n && (!dirs || !out))) return 1;
+    for (size_t i = 0; i < n; ++i) {
+        const double *d = dirs + 3 * i;
+        if (precision == 64) {
+            const V3<double> e = environment(V3<double>(d[0], d[1], d[2]), *cfg);
+            out[3 * i] = e.x; out[3 * i + 1] = e.y; out[3 * i + 2] = e.z;
+        } else {
+            const V3<float> e = environment(V3<float>(float(d[0]), float(d[1]), float(d[2])), *cfg);
+            out[3 * i] = e.x; out[3 * i + 1] = e.y; out[3 * i + 2] = e.z;
+        }
+    }
     return 0;
 }
 

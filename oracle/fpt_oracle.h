@@ -42,6 +42,8 @@ int orc_render_samples(const struct FptRenderConfig *cfg, int precision, int thr
 /* Un-jittered primary ray per pixel: march + normal (Shaders.metal:
  * 3753-3783, 3871-3971).  out: 9 doubles per pixel {t, px, py, pz, nx, ny,
  * nz, hit, steps}. */
+/* environment (Shaders.metal:5823-5844) for n view directions (xyz). */
+int orc_environment(const struct FptRenderConfig *cfg, int precision, const double *dirs, size_t n, double *out);
 /* estimateFocusDistance (Shaders.metal:5869-5878) of the config's camera. */
 double orc_focus_estimate(const struct FptRenderConfig *cfg, int precision);
 int orc_trace_primary(const struct FptRenderConfig *cfg, int precision,
