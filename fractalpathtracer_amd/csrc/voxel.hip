@@ -155,9 +155,10 @@ struct VoxelView {
 
 struct VHit {
     bool hit;
-    float t;
-    float exit_t;
-    f3 position;
+    float t;        // cell entry distance
+    float exit_t;   // cell exit distance
+    float surface;  // entry distance, or the leaf refinement's surface distance
+    f3 position;    // origin + dir * surface
     f3 cell_center;
     f3 normal;
     uint32_t payload;  // packed colour (4^3/dense) or material id (8^3)
@@ -173,6 +174,7 @@ __device__ VHit trace_voxel(f3 origin, f3 dir, const FptRenderConfig &cfg, const
     r.hit = false;
     r.t = kInf;
     r.exit_t = kInf;
+    r.surface = kInf;
     r.steps = 0;
     r.payload = 0;
     r.index = 0;
@@ -251,6 +253,7 @@ __device__ VHit trace_voxel(f3 origin, f3 dir, const FptRenderConfig &cfg, const
             r.hit = true;
             r.t = entry_t;
             r.exit_t = fminf(next_t.x, fminf(next_t.y, next_t.z));
+            r.surface = entry_t;
             r.position = origin + dir * entry_t;
             r.cell_center = mk(bmin.x + (float(cell.x) + 0.5f) * cs.x, bmin.y + (float(cell.y) + 0.5f) * cs.y,
                                bmin.z + (float(cell.z) + 0.5f) * cs.z);
@@ -268,6 +271,162 @@ __device__ VHit trace_voxel(f3 origin, f3 dir, const FptRenderConfig &cfg, const
 struct NoVisit {
     __device__ void operator()(i3) const {}
 };
+
+// ---------------------------------------------------------------------------
+// Leaf refinement of an occupied cell (SM:4312-4490): a surface point inside
+// the cell's ray interval [t0, t1], found with the scene's distance.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool sign_differs(float a, float b) { return signbit(a) != signbit(b); }
+
+// Secant/bisection on a signed crossing: a 4-interval scan brackets the
+// first sign change, then 8 iterations alternate a guarded secant step (kept
+// 10% inside the bracket) with bisection; |d| <= 1e-6 ends early.  Without a
+// crossing the hit is left unrefined.
+__device__ bool refine_secant_bisection(f3 o, f3 d, float t0, float t1, const FptRenderConfig &cfg, Work &w,
+                                        float &out) {
+    if (!isfinite(t0) || !isfinite(t1) || !(t1 > t0)) return false;
+    const float tol = 1.0e-6f;
+    float pt = t0, pd = map_sdf(o + d * pt, cfg, w);
+    if (!isfinite(pd)) return false;
+    if (fabsf(pd) <= tol) {
+        out = pt;
+        return true;
+    }
+    float lo_t = t0, hi_t = t1, lo_d = pd, hi_d = pd;
+    bool bracket = false;
+    for (uint32_t k = 1u; k <= 4u; ++k) {
+        const float ct = mixf(t0, t1, float(k) / 4.0f);
+        const float cd = map_sdf(o + d * ct, cfg, w);
+        if (!isfinite(cd)) return false;
+        if (fabsf(cd) <= tol) {
+            out = ct;
+            return true;
+        }
+        if (sign_differs(pd, cd)) {
+            lo_t = pt; lo_d = pd; hi_t = ct; hi_d = cd;
+            bracket = true;
+            break;
+        }
+        pt = ct;
+        pd = cd;
+    }
+    if (!bracket) return false;
+    for (uint32_t k = 0u; k < 8u; ++k) {
+        float ct = 0.5f * (lo_t + hi_t);
+        if ((k & 1u) == 0u) {
+            const float den = hi_d - lo_d;
+            if (isfinite(den) && fabsf(den) > 1.0e-12f) {
+                const float st = (lo_t * hi_d - hi_t * lo_d) / den;
+                const float guard = (hi_t - lo_t) * 0.1f;
+                if (st > lo_t + guard && st < hi_t - guard) ct = st;
+            }
+        }
+        const float cd = map_sdf(o + d * ct, cfg, w);
+        if (!isfinite(cd)) return false;
+        if (fabsf(cd) <= tol) {
+            out = ct;
+            return true;
+        }
+        if (sign_differs(lo_d, cd)) {
+            hi_t = ct; hi_d = cd;
+        } else {
+            lo_t = ct; lo_d = cd;
+        }
+    }
+    out = clampf(fabsf(lo_d) <= fabsf(hi_d) ? lo_t : hi_t, t0, t1);
+    return true;
+}
+
+// Sphere tracing inside the cell for typed programs with certified bounds:
+// steps of 0.99 |d| / max(L, 1) (L the cell's Lipschitz bound), at least half
+// the tolerance max(1e-6, 1e-4 (t1 - t0)), at most 12, never past the exit
+// (where |d| within the tolerance still counts as the surface).
+__device__ bool refine_restricted_trace(f3 o, f3 d, float t0, float t1, f3 center, f3 half,
+                                        const FptRenderConfig &cfg, Work &w, float &out) {
+    if (cfg.sdf_id != FPT_SDF_PROGRAM || !isfinite(t0) || !isfinite(t1) || !(t1 > t0)) return false;
+    const ProgramBounds b = program_bounds(center, half, cfg);
+    if (!b.certified || !isfinite(b.lipschitz)) return false;
+    float t = t0;
+    const float tol = fmaxf(1.0e-6f, (t1 - t0) * 1.0e-4f);
+    const float inv_l = 1.0f / fmaxf(b.lipschitz, 1.0f);
+    for (uint32_t k = 0u; k < 12u; ++k) {
+        const float dist = map_sdf(o + d * t, cfg, w);
+        if (!isfinite(dist)) return false;
+        if (fabsf(dist) <= tol) {
+            out = clampf(t, t0, t1);
+            return true;
+        }
+        const float nt = t + fmaxf(fabsf(dist) * inv_l * 0.99f, tol * 0.5f);
+        if (!(nt < t1)) {
+            const float de = map_sdf(o + d * t1, cfg, w);
+            if (isfinite(de) && fabsf(de) <= tol) {
+                out = t1;
+                return true;
+            }
+            return false;
+        }
+        t = nt;
+    }
+    return false;
+}
+
+// Built-in fractals (unsigned estimators): the smallest |d| of 13 evenly
+// spaced samples, then 3 ternary reductions around it; accepted only within
+// max(eps, 1e-5, (t1 - t0) / 64) of the surface.
+__device__ bool refine_fixed_de(f3 o, f3 d, float t0, float t1, const FptRenderConfig &cfg, Work &w, float &out) {
+    const bool fractal = cfg.sdf_id >= FPT_SDF_BALL_FRACTAL && cfg.sdf_id <= FPT_SDF_TREE_FRACTAL;
+    if (!fractal || !isfinite(t0) || !isfinite(t1) || !(t1 > t0)) return false;
+    const float span = t1 - t0, step = span / 12.0f;
+    float best_t = t0, best = kInf;
+    for (uint32_t k = 0u; k <= 12u; ++k) {
+        const float t = t0 + float(k) * step;
+        const float dist = fabsf(map_sdf(o + d * t, cfg, w));
+        if (!isfinite(dist)) return false;
+        if (dist < best) {
+            best = dist;
+            best_t = t;
+        }
+    }
+    float lo = fmaxf(t0, best_t - step), hi = fminf(t1, best_t + step);
+    for (uint32_t k = 0u; k < 3u; ++k) {
+        const float lt = mixf(lo, hi, 1.0f / 3.0f), rt = mixf(lo, hi, 2.0f / 3.0f);
+        const float ld = fabsf(map_sdf(o + d * lt, cfg, w)), rd = fabsf(map_sdf(o + d * rt, cfg, w));
+        if (!isfinite(ld) || !isfinite(rd)) return false;
+        if (ld < best) {
+            best = ld;
+            best_t = lt;
+        }
+        if (rd < best) {
+            best = rd;
+            best_t = rt;
+        }
+        if (ld <= rd) hi = rt;
+        else lo = lt;
+    }
+    if (best > fmaxf(fmaxf(cfg.render[3], 1.0e-5f), span / 64.0f)) return false;
+    out = clampf(best_t, t0, t1);
+    return true;
+}
+
+// The refinement step of traceVoxel (SM:4583-4602) for a hit h of the ray
+// (o, d): moves h.surface and h.position to the refined surface point when
+// the configured method finds one.
+__device__ void refine_leaf(VHit &h, f3 o, f3 d, const FptRenderConfig &cfg, Work &w) {
+    const uint32_t mode = cfg.voxel_leaf_refinement;
+    if (!h.hit || mode == FPT_VOXEL_LEAF_REFINEMENT_NONE) return;
+    float t = h.t;
+    bool ok = false;
+    if (mode == FPT_VOXEL_LEAF_REFINEMENT_SECANT_BISECTION)
+        ok = refine_secant_bisection(o, d, h.t, h.exit_t, cfg, w, t);
+    else if (mode == FPT_VOXEL_LEAF_REFINEMENT_RESTRICTED_TRACE)
+        ok = refine_restricted_trace(o, d, h.t, h.exit_t, h.cell_center, cell_size_of(cfg) * 0.5f, cfg, w, t);
+    else if (mode == FPT_VOXEL_LEAF_REFINEMENT_FIXED_DE)
+        ok = refine_fixed_de(o, d, h.t, h.exit_t, cfg, w, t);
+    if (ok) {
+        h.surface = t;
+        h.position = o + d * t;
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Cell classification and records (SM:4021-4044, 4196-4244, 5388-5414).
@@ -576,6 +735,7 @@ __device__ f3 render_voxel_path(UV xy, uint32_t sample, float focus, const FptRe
     const float cmin = fminf(cs.x, fminf(cs.y, cs.z));
     for (int b = 0; b < bounces; ++b) {
         VHit h = trace_voxel(pos, dir, cfg, v, NoVisit());
+        refine_leaf(h, pos, dir, cfg, w);
         steps += int(h.steps);
         if (!h.hit) {
             if (b == 0) sky_mask = 1.0f;
@@ -646,7 +806,9 @@ __global__ void voxel_focus_kernel(const FptRenderConfig *__restrict__ cfgp, Vox
     f3 d = rotate_camera_roll(normalize(mk(0.f, 0.f, focal)), cfg.camera_yaw_pitch[0], cfg.camera_yaw_pitch[1],
                               cfg.camera_roll);
     VHit h = trace_voxel(ld3(cfg.camera_position), d, cfg, v, NoVisit());
-    *focus = h.hit && h.t > 0.001f ? h.t : 5.0f;
+    Work w = {0, 0};
+    refine_leaf(h, ld3(cfg.camera_position), d, cfg, w);
+    *focus = h.hit && h.surface > 0.001f ? h.surface : 5.0f;
 }
 
 __global__ __launch_bounds__(256) void voxel_accumulate_kernel(const FptRenderConfig *__restrict__ cfgp, VoxelView v,
@@ -791,6 +953,26 @@ __global__ void voxel_dda_contract_kernel(uint32_t *failures_out, const FptRende
     if (!s_surf.lipschitz || !s_surf.interval || s_in.lipschitz || s_in.interval) f |= 1u << 20;
     const CellClass miss = cls(mk(0.75f, 0.75f, 0.75f), q, pc[0]);
     if (miss.legacy || !miss.lipschitz || !miss.interval || miss.cell_class != CELL_SURFACE) f |= 1u << 21;
+    // Leaf refinement (SM:5309-5341) on the unit sphere: the x axis crosses
+    // its surface at t = 1 inside [0.75, 1.25]; the line y = 1.5 misses it.
+    {
+        Work w = {0, 0};
+        const f3 ox = mk(-2.f, 0.f, 0.f), oy = mk(-2.f, 1.5f, 0.f), dx = mk(1.f, 0.f, 0.f), qh = q;
+        float rt = 0.0f;
+        if (!refine_secant_bisection(ox, dx, 0.75f, 1.25f, pc[0], w, rt) || rt < 0.75f || rt > 1.25f ||
+            fabsf(rt - 1.0f) > 1.0e-4f)
+            f |= 1u << 22;
+        rt = 0.0f;
+        if (refine_secant_bisection(oy, dx, 0.75f, 1.25f, pc[0], w, rt)) f |= 1u << 23;
+        rt = 0.0f;
+        if (!refine_restricted_trace(ox, dx, 0.75f, 1.25f, mk(-1.f, 0.f, 0.f), qh, pc[0], w, rt) || rt < 0.75f ||
+            rt > 1.25f || fabsf(rt - 1.0f) > 1.0e-4f)
+            f |= 1u << 24;
+        rt = 0.0f;
+        if (refine_restricted_trace(oy, dx, 0.75f, 1.25f, mk(-1.f, 1.5f, 0.f), qh, pc[0], w, rt)) f |= 1u << 25;
+        rt = 0.0f;
+        if (refine_fixed_de(ox, dx, 0.75f, 1.25f, pc[0], w, rt)) f |= 1u << 26;  // not a built-in fractal
+    }
     if (!bounds_contain_grid(mk(0.35f, -0.15f, 0.2f), mk(0.4f, 0.3f, 0.2f), pc[1])) f |= 1u << 12;
     if (!program_bounds(mk(0.2f, -0.1f, 0.3f), mk(0.4f, 0.2f, 0.5f), pc[2]).certified ||
         !bounds_contain_grid(mk(0.2f, -0.1f, 0.3f), mk(0.4f, 0.2f, 0.5f), pc[2]))
@@ -1008,12 +1190,13 @@ __device__ f3 voxel_diagnostic(UV xy, const FptRenderConfig &cfg, const FptDiagn
     const float focal = 1.0f / tanf(cfg.camera_fov / 2.0f * kPi / 180.0f);
     const f3 dir = rotate_camera_roll(normalize(mk(xy.x, xy.y, focal)), cfg.camera_yaw_pitch[0],
                                       cfg.camera_yaw_pitch[1], cfg.camera_roll);
-    const VHit h = trace_voxel(ld3(cfg.camera_position), dir, cfg, v, NoVisit());
+    VHit h = trace_voxel(ld3(cfg.camera_position), dir, cfg, v, NoVisit());
+    refine_leaf(h, ld3(cfg.camera_position), dir, cfg, w);
     if (mode == FPT_DIAGNOSTIC_HIT_MASK) return h.hit ? mk(1.f, 1.f, 1.f) : mk(0.f, 0.f, 0.f);
-    if (mode == FPT_DIAGNOSTIC_DEPTH) {  // SM:6827-6832 on the entry distance
+    if (mode == FPT_DIAGNOSTIC_DEPTH) {  // SM:6827-6832 on the surface distance (SM:7410)
         if (!h.hit) return mk(0.f, 0.f, 0.f);
         const float md = dg.max_distance > 0.001f ? dg.max_distance : cfg.render[4];
-        const float k = 1.0f - clampf(h.t / fmaxf(md, 0.001f), 0.f, 1.f);
+        const float k = 1.0f - clampf(h.surface / fmaxf(md, 0.001f), 0.f, 1.f);
         return mk(k, k, k);
     }
     if (mode == FPT_DIAGNOSTIC_SDF_PRIMARY_STEPS) {
@@ -1136,9 +1319,9 @@ void contract_programs(FptRenderConfig *pc) {
 }  // namespace
 
 // Runs the known-answer checks of the reference's contract (Shaders.metal:
-// 5156-5386) that this library implements: bits 0-21, 27-29.  Bits 22-26
-// (voxel leaf refinement), 30 (Cage-fractal bounds) and 31 (derivative
-// bounds) belong to features this library does not build.
+// 5156-5386) that this library implements: bits 0-29.  Bits 30 (Cage-fractal
+// bounds) and 31 (derivative bounds) belong to features this library does not
+// build.
 extern "C" int fpt_test_voxel_dda(const char *metallib_path, char *error, size_t error_len) {
     (void)metallib_path;
     std::vector<FptRenderConfig> pc(9);

@@ -63,6 +63,12 @@ VOXEL_COVERAGE_LIPSCHITZ = 1
 VOXEL_COVERAGE_INTERVAL = 2
 VOXEL_COVERAGE_SHELL = 3
 
+# FptVoxelLeafRefinement (metal_bridge.h, Shaders.metal:45-48).
+VOXEL_LEAF_REFINEMENT_NONE = 0
+VOXEL_LEAF_REFINEMENT_SECANT_BISECTION = 1
+VOXEL_LEAF_REFINEMENT_RESTRICTED_TRACE = 2
+VOXEL_LEAF_REFINEMENT_FIXED_DE = 3
+
 u32 = C.c_uint32
 f32 = C.c_float
 

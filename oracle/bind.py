@@ -44,6 +44,7 @@ def lib():
                                          C.c_uint32, dp]
         L.orc_trace_primary.argtypes = [C.c_void_p, C.c_int, C.c_int, dp]
         L.orc_environment.argtypes = [C.c_void_p, C.c_int, dp, C.c_size_t, dp]
+        L.orc_leaf_refine.argtypes = [C.c_void_p, C.c_int, C.c_uint32, dp, dp]
         L.orc_focus_estimate.argtypes = [C.c_void_p, C.c_int]
         L.orc_focus_estimate.restype = C.c_double
         L.orc_map_distance.argtypes = [C.c_void_p, C.c_int, dp, C.c_size_t, dp]
@@ -153,6 +154,14 @@ def counters() -> dict:
     keys = ("distance_evals", "orbit_iterations", "primary_steps", "secondary_steps",
             "shadow_steps", "normal_evals", "paths", "far_steps", "far_orbit_iterations")
     return dict(zip(keys, list(a)))
+
+
+def leaf_refine(cfg, mode: int, origin, direction, t0, t1, center=(0, 0, 0), half=(0, 0, 0), precision: int = 64):
+    """Voxel leaf refinement of one ray interval: (refined?, distance)."""
+    ray = np.array([*origin, *direction, t0, t1, *center, *half], np.float64)
+    out = np.zeros(1)
+    ok = lib().orc_leaf_refine(C.byref(cfg), precision, mode, _dp(ray), _dp(out))
+    return bool(ok), float(out[0])
 
 
 def environment(cfg, dirs, precision: int = 64) -> np.ndarray:

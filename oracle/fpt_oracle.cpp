@@ -1301,6 +1301,26 @@ int orc_render_samples(const FptRenderConfig *cfg, int precision, int threads, u
     return 0;
 }
 
+// Leaf refinement probe: mode 1 secant/bisection, 2 restricted trace, 3 fixed
+// DE; ray[0..5] origin and direction, ray[6..7] the interval, ray[8..13] the
+// cell centre and half extent.  Returns 1 when refined (out = distance).
+int orc_leaf_refine(const FptRenderConfig *cfg, int precision, uint32_t mode, const double *ray, double *out) {
+    if (!cfg || !ray || !out) return 0;
+    auto run = [&](auto zero) {
+        using T = decltype(zero);
+        const V3<T> o{T(ray[0]), T(ray[1]), T(ray[2])}, d{T(ray[3]), T(ray[4]), T(ray[5])};
+        const V3<T> c{T(ray[8]), T(ray[9]), T(ray[10])}, h{T(ray[11]), T(ray[12]), T(ray[13])};
+        T t = T(0);
+        bool ok = false;
+        if (mode == 1u) ok = refine_secant_bisection<T>(o, d, T(ray[6]), T(ray[7]), *cfg, t);
+        else if (mode == 2u) ok = refine_restricted_trace<T>(o, d, T(ray[6]), T(ray[7]), c, h, *cfg, t);
+        else if (mode == 3u) ok = refine_fixed_de<T>(o, d, T(ray[6]), T(ray[7]), *cfg, t);
+        *out = double(t);
+        return ok ? 1 : 0;
+    };
+    return precision == 64 ? run(0.0) : run(0.0f);
+}
+
 int orc_environment(const FptRenderConfig *cfg, int precision, const double *dirs, size_t n, double *out) {
     if (!cfg || (n && (!dirs || !out))) return 1;
     for (size_t i = 0; i < n; ++i) {

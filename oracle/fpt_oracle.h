@@ -42,6 +42,8 @@ int orc_render_samples(const struct FptRenderConfig *cfg, int precision, int thr
 /* Un-jittered primary ray per pixel: march + normal (Shaders.metal:
  * 3753-3783, 3871-3971).  out: 9 doubles per pixel {t, px, py, pz, nx, ny,
  * nz, hit, steps}. */
+/* Voxel leaf refinement (Shaders.metal:4312-4490) of one ray interval. */
+int orc_leaf_refine(const struct FptRenderConfig *cfg, int precision, uint32_t mode, const double *ray, double *out);
 /* environment (Shaders.metal:5823-5844) for n view directions (xyz). */
 int orc_environment(const struct FptRenderConfig *cfg, int precision, const double *dirs, size_t n, double *out);
 /* estimateFocusDistance (Shaders.metal:5869-5878) of the config's camera. */

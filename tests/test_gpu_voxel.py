@@ -126,3 +126,48 @@ def test_voxel_render_matches_fp32_restatement(storage):
     print("voxel image rmse %.3e, pixels within 1e-3: %.4f, active cells %d" % (rmse, close, res.voxel["active_cells"]))
     assert rmse <= 2e-2 and close >= 0.9
     assert res.voxel["active_cells"] > 0 and res.build_ms > 0
+
+
+def _refined(mode):
+    """Secant/bisection and restricted trace on the typed program (box minus
+    sphere), fixed DE on the built-in Menger sponge (initial cut scale 1,
+    divisor 3: its distance is tight enough for the method's acceptance
+    bound; the Mandelbox's conservative estimate rarely is)."""
+    if mode == ffi.VOXEL_LEAF_REFINEMENT_FIXED_DE:
+        c = _cfg(ffi.VOXEL_STORAGE_SPARSE_BRICKS8, 0, 128)
+        c.sdf_id = ffi.SDF_MENGER_SPONGE
+        c.set_values[0], c.set_values[1], c.set_values[2], c.set_values[3] = 1.0, 3.0, 0.0, 0.0
+        for i in range(3):
+            c.voxel_bounds_min[i], c.voxel_bounds_max[i] = -1.2, 1.2
+        c.camera_position[0], c.camera_position[1], c.camera_position[2] = 0.0, 0.0, -3.5
+        c.camera_yaw_pitch[0], c.camera_yaw_pitch[1] = 0.5, 0.4
+    else:
+        c = _program_cfg(ffi.VOXEL_STORAGE_SPARSE_BRICKS8, 0, 48)
+        c.camera_position[2] = -3.5
+        c.camera_fov = 50.0
+    c.renderer_backend = ffi.RENDERER_VOXEL
+    c.voxel_leaf_refinement = mode
+    return c
+
+
+@pytest.mark.parametrize("mode", [ffi.VOXEL_LEAF_REFINEMENT_SECANT_BISECTION,
+                                  ffi.VOXEL_LEAF_REFINEMENT_RESTRICTED_TRACE,
+                                  ffi.VOXEL_LEAF_REFINEMENT_FIXED_DE])
+def test_leaf_refinement_render_and_depth_match_oracle(mode):
+    cfg = _refined(mode)
+    res = api.render_full(cfg)
+    ref = bind.voxel_render(cfg)
+    d = np.abs(res.linear[..., :3] - ref[..., :3])
+    rmse = float(np.sqrt(np.mean(d ** 2)))
+    close = float(np.mean(d.max(axis=-1) <= 1e-3))
+    plain = scenes.clone(cfg)
+    plain.voxel_leaf_refinement = ffi.VOXEL_LEAF_REFINEMENT_NONE
+    changed = float(np.mean(np.any(api.render_full(plain).linear != res.linear, axis=-1)))
+    img, _ = api.diagnostic(cfg, "depth", max_distance=30.0)
+    dref = bind.diagnostic(cfg, ffi.DIAGNOSTIC_MODES["depth"], max_distance=30.0)
+    dclose = float(np.mean(np.abs(img.astype(int) - dref.astype(int)) <= 1))
+    print(mode, "rmse %.3e within 1e-3 %.4f; pixels changed by refinement %.3f; depth within 1: %.4f" % (
+        rmse, close, changed, dclose))
+    assert rmse <= 2e-2 and close >= 0.9
+    assert changed > 0.01
+    assert dclose >= 0.99
