@@ -181,6 +181,14 @@ class FptStitchPipelineStats(C.Structure):
         ("runtime_library_compile_ms", C.c_double), ("runtime_pipeline_link_ms", C.c_double)]
 
 
+class FptSceneOptions(C.Structure):
+    """Render arguments that shape a loaded scene (include/fpt_abi.h)."""
+    _fields_ = [(n, u32) for n in (
+        "width", "height", "samples", "preview", "sdf_profile", "glass_mode", "program_optimization",
+        "geometry_split", "canonical_ir", "flat_union", "typed_soa", "renderer_backend",
+        "topology_specialization", "_pad0")]
+
+
 class FptSdfProfileStats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "primary_steps", "secondary_steps", "shadow_steps", "normal_evals", "bounces", "pixels",
@@ -259,6 +267,7 @@ EXPECTED_SIZES = {
     FptRenderConfig: 30448, FptSdfProfileStats: 288, FptDiagnosticConfig: 24,
     FptMandelbulberFieldSample: 16, FptAsyncJitStats: 40, FptRayHit: 40, FptVoxelBuildInfo: 40,
     FptTuneResult: 24, FptProposerSettings: 16, FptProposerStats: 32, FptSpecializationStats: 80,
+    FptSceneOptions: 56,
 }
 
 PKG_DIR = Path(__file__).resolve().parent
@@ -277,6 +286,9 @@ EXPORTED_SYMBOLS = (
     "fpt_hip_stream_create", "fpt_hip_stream_destroy", "fpt_hip_synchronize", "fpt_hip_time_renders",
     "fpt_hip_proposer_samples", "fpt_hip_proposer_eval", "fpt_hip_plan_set_proposer", "fpt_hip_plan_profile",
     "fpt_hip_diagnostic_render", "fpt_hip_plan_specialize", "fpt_hip_topology_source", "fpt_hip_hydrate_hdri",
+    "fpt_scene_default_config", "fpt_scene_load", "fpt_scene_load_fpt_settings", "fpt_scene_apply_program_json",
+    "fpt_scene_optimize_program", "fpt_scene_split_program", "fpt_scene_compile_canonical",
+    "fpt_scene_lower_flat_union", "fpt_scene_lower_typed_soa",
 )
 
 _lib = None
@@ -309,6 +321,16 @@ def _declare(lib):
     lib.fpt_hip_trace_primary.argtypes = [P(FptRenderConfig), P(FptRayHit), C.c_size_t, cp,
                                           C.c_size_t]
     lib.fpt_hip_trace_primary.restype = C.c_int
+    lib.fpt_scene_default_config.argtypes = [P(FptRenderConfig)]
+    lib.fpt_scene_load.argtypes = [cp, cp, P(FptSceneOptions), P(FptRenderConfig), cp, C.c_size_t, cp, C.c_size_t]
+    lib.fpt_scene_load_fpt_settings.argtypes = [cp, P(FptRenderConfig), cp, C.c_size_t]
+    lib.fpt_scene_apply_program_json.argtypes = [P(FptRenderConfig), cp, cp, C.c_size_t]
+    lib.fpt_scene_optimize_program.argtypes = [P(FptRenderConfig)]
+    lib.fpt_scene_optimize_program.restype = None
+    lib.fpt_scene_split_program.argtypes = [P(FptRenderConfig), u32]
+    lib.fpt_scene_split_program.restype = None
+    for name in ("fpt_scene_compile_canonical", "fpt_scene_lower_flat_union", "fpt_scene_lower_typed_soa"):
+        getattr(lib, name).argtypes = [P(FptRenderConfig)]
     lib.fpt_hip_hydrate_hdri.argtypes = [P(FptRenderConfig), cp, C.c_size_t]
     lib.fpt_hip_hydrate_hdri.restype = C.c_int
     lib.fpt_hip_plan_create.argtypes = [P(FptRenderConfig), P(C.c_void_p), cp, C.c_size_t]

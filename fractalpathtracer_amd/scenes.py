@@ -82,6 +82,24 @@ def set_program(c: FptRenderConfig, ops) -> None:
             _set(dst.data, d[:4])
 
 
+def load_scene(path, fpt_root=".", **options):
+    """Load a JSON scene with the library's C++ front end (fpt_scene_load,
+    src/scene.rs:1619-1976): returns (config, output_name).  `options` are
+    FptSceneOptions fields (width, height, samples, program_optimization,
+    geometry_split, canonical_ir, flat_union, typed_soa, renderer_backend,
+    topology_specialization, ...)."""
+    lib = ffi.load_library()
+    opt = ffi.FptSceneOptions()
+    for k, v in options.items():
+        setattr(opt, k, int(v))
+    cfg = FptRenderConfig()
+    name = C.create_string_buffer(512)
+    err = ffi.error_buffer()
+    ffi.check(lib.fpt_scene_load(str(path).encode(), str(fpt_root).encode(), C.byref(opt), C.byref(cfg), name,
+                                 len(name), err, len(err)), err)
+    return cfg, name.value.decode()
+
+
 def set_hdri_path(c: FptRenderConfig, path) -> None:
     """world.hdri (src/scene.rs:873-879): the path as NUL-terminated bytes,
     truncated to 255; HDRI is enabled iff the path is not empty.  The render

@@ -692,6 +692,54 @@ int fpt_hip_program_surface(const struct FptRenderConfig *config,
 int fpt_hip_hydrate_hdri(struct FptRenderConfig *config, char *error, size_t error_len);
 
 /*
+ * Scene front end (host only): JSON scene files and .fpt presets into a
+ * config, and the typed-program passes run on it before rendering.  These
+ * replace the Rust loader src/scene.rs:743-1976 (load_scene_config and the
+ * functions it calls), which the reference keeps above its C ABI.
+ *
+ * FptSceneOptions carries the render arguments that shape a loaded config
+ * (RenderArgs, src/scene.rs:199-273); zero-initialised means "as the scene
+ * says, no optional passes, SDF renderer".
+ */
+struct FptSceneOptions {
+    uint32_t width, height, samples; /* 0: the scene's value (defaults 768, 768, 64) */
+    uint32_t preview, sdf_profile, glass_mode;
+    uint32_t program_optimization;   /* 0 off, 1 basic (optimize_sdf_program) */
+    uint32_t geometry_split;         /* split_sdf_geometry_program(enabled) */
+    uint32_t canonical_ir;           /* compile_canonical_geometry_program */
+    uint32_t flat_union;             /* lower_flat_union_program */
+    uint32_t typed_soa;              /* 0 off, 1 required, 2 when possible (automatic backend probing) */
+    uint32_t renderer_backend;       /* FPT_RENDERER_* of the render that follows */
+    uint32_t topology_specialization;
+    uint32_t _pad0;
+};
+FPT_ABI_ASSERT(sizeof(struct FptSceneOptions) == 56, "FptSceneOptions");
+
+/* default_config (src/scene.rs:743-778). */
+int fpt_scene_default_config(struct FptRenderConfig *config);
+/* load_scene_config (src/scene.rs:1619-1976); output_name receives the
+ * scene's "output" or "<preset stem>.png". */
+int fpt_scene_load(const char *scene_path, const char *fpt_root, const struct FptSceneOptions *options,
+                   struct FptRenderConfig *config, char *output_name, size_t output_name_len, char *error,
+                   size_t error_len);
+/* load_fpt_settings (src/scene.rs:1583-1617): SET and vector values, and the
+ * Gradient(Orbit_Trap) example program. */
+int fpt_scene_load_fpt_settings(const char *preset_path, struct FptRenderConfig *config, char *error,
+                                size_t error_len);
+/* apply_sdf_program (src/scene.rs:963-1021) of one sdf_program JSON object. */
+int fpt_scene_apply_program_json(struct FptRenderConfig *config, const char *json, char *error, size_t error_len);
+/* optimize_sdf_program (src/scene.rs:1124-1157). */
+void fpt_scene_optimize_program(struct FptRenderConfig *config);
+/* split_sdf_geometry_program (src/scene.rs:1159-1184). */
+void fpt_scene_split_program(struct FptRenderConfig *config, uint32_t enabled);
+/* compile_canonical_geometry_program (src/scene.rs:1211-1358); 1 when compiled. */
+int fpt_scene_compile_canonical(struct FptRenderConfig *config);
+/* lower_flat_union_program (src/scene.rs:1360-1446); 1 when lowered. */
+int fpt_scene_lower_flat_union(struct FptRenderConfig *config);
+/* lower_typed_soa_program (src/scene.rs:1448-1530); 1 when lowered. */
+int fpt_scene_lower_typed_soa(struct FptRenderConfig *config);
+
+/*
  * Device-resident render plan: the config (and any derived device tables:
  * voxel pool, step-proposer weights) uploaded once, so repeated renders
  * touch only HBM.  fpt_hip_plan_render enqueues the render on `stream`
