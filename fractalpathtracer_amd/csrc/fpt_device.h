@@ -253,16 +253,27 @@ struct Leaf {
 };
 
 // |z| robust to float32 overflow of |z|^2 (escaping Mandelbulb orbits pass
-// |z| ~ 1e20 one iteration after |p| ~ 235).
+// |z| ~ 1e20 one iteration after |p| ~ 235): the scaled form for lanes whose
+// |z|^2 is not finite.
 template <class S>
-__device__ __forceinline__ S orbit_radius(v3<S> z) {
-    S r2 = dot3(z);
-    if (__builtin_expect(isfinite(val(r2)), 1)) return f_sqrt(r2);
+__device__ __forceinline__ S orbit_radius_slow(v3<S> z, S r2) {
     float m = fmaxf(fabsf(val(z.x)), fmaxf(fabsf(val(z.y)), fabsf(val(z.z))));
     if (!isfinite(m)) return r2;
     float im = 1.0f / m;
     v3<S> q = {s_scale(z.x, im), s_scale(z.y, im), s_scale(z.z, im)};
     return s_scale(s_sqrt(dot3(q)), m);
+}
+template <class S>
+__device__ __forceinline__ S orbit_radius(v3<S> z) {
+    S r2 = dot3(z);
+    S r = f_sqrt(r2);
+    const bool slow = !isfinite(val(r2));
+    // A wave-uniform test: the common case (no lane overflows) costs one
+    // ballot instead of a divergent branch's exec-mask setup per call.
+    if (__builtin_expect(__any(slow), 0)) {
+        if (slow) r = orbit_radius_slow(z, r2);
+    }
+    return r;
 }
 
 // (c + i s)^n for integer n >= 1 by binary exponentiation.
