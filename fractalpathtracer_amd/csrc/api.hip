@@ -105,13 +105,20 @@ int validate_config(const FptRenderConfig *config, char *error, size_t error_len
         set_error(error, error_len, "renderer backend %u is not supported by this library", config->renderer_backend);
         return 1;
     }
-    if (config->sdf_id == FPT_SDF_MANDELBULBER) {
-        set_error(error, error_len,
-                  "sdf_id MANDELBULBER needs generated Mandelbulber2 formula code; use the typed program "
-                  "fractal opcodes instead");
+    if (config->sdf_id == FPT_SDF_MANDELBULBER && config->renderer_backend != FPT_RENDERER_SDF) {
+        set_error(error, error_len, "imported Mandelbulber scenes render on the SDF backend only");
         return 1;
     }
     return 0;
+}
+
+// Imported Mandelbulber scenes (the native KIFS field) render on the
+// one-path-per-lane kernel; the entry points built on the persistent kernel,
+// the typed-program tools and the diagnostic images refuse them.
+int reject_mandelbulber(const FptRenderConfig *config, const char *what, char *error, size_t error_len) {
+    if (!config || config->sdf_id != FPT_SDF_MANDELBULBER) return 0;
+    set_error(error, error_len, "%s does not support imported Mandelbulber scenes", what);
+    return 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -543,6 +550,7 @@ int fpt_hip_plan_create(const FptRenderConfig *config, FptHipPlan **plan, char *
     config = &hydrated;
     FptHipPlan *p = new FptHipPlan();
     p->host = *config;
+    if (config->sdf_id == FPT_SDF_MANDELBULBER) p->kernel = 0u;
     hipError_t e = hipMalloc(&p->dcfg, sizeof(FptRenderConfig));
     if (e == hipSuccess) e = hipMemcpy(p->dcfg, config, sizeof(FptRenderConfig), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
@@ -618,6 +626,7 @@ int fpt_hip_plan_tune(FptHipPlan *plan, uint32_t kernel, uint32_t workgroups, ui
         set_error(error, error_len, "learned stepping runs on the persistent kernel (kernel 1)");
         return 1;
     }
+    if (kernel == 1u && reject_mandelbulber(&plan->host, "the persistent kernel", error, error_len)) return 1;
     plan->kernel = kernel;
     plan->workgroups = workgroups;
     plan->event_batch = event_batch ? event_batch : 24u;
@@ -750,6 +759,7 @@ int fpt_hip_proposer_samples(const FptRenderConfig *config, float box_half, floa
                              uint64_t seed, uint32_t max_attempts, float *out_xyzd, uint32_t *failures,
                              char *error, size_t error_len) {
     if (validate_config(config, error, error_len)) return 1;
+    if (reject_mandelbulber(config, "learned stepping", error, error_len)) return 1;
     if (!out_xyzd || !failures) {
         set_error(error, error_len, "missing sample or failure-count buffer");
         return 1;
@@ -859,6 +869,7 @@ int fpt_hip_batch_create(const FptRenderConfig *configs, uint32_t count, FptHipB
     }
     for (uint32_t i = 0; i < count; ++i) {
         if (validate_config(&configs[i], error, error_len)) return 1;
+        if (reject_mandelbulber(&configs[i], "batched rendering", error, error_len)) return 1;
         if (configs[i].renderer_backend != FPT_RENDERER_SDF) {
             set_error(error, error_len, "scene %u: batched rendering supports the SDF backend only", i);
             return 1;
@@ -1035,6 +1046,7 @@ int fpt_hip_autotune(FptHipPlan *plan, FptHipBatch *batch, float *device_accum, 
         set_error(error, error_len, "autotune applies to the SDF backend");
         return 1;
     }
+    if (plan && reject_mandelbulber(&plan->host, "the persistent-kernel autotuner", error, error_len)) return 1;
     hipStream_t s = static_cast<hipStream_t>(stream);
     const uint32_t cus = default_workgroups() / 2u;
     std::vector<std::pair<uint32_t, uint32_t>> cand;
@@ -1342,6 +1354,7 @@ int fpt_hip_voxel_build(const FptRenderConfig *config, uint32_t *page_table, siz
         set_error(error, error_len, "missing render config");
         return 1;
     }
+    if (reject_mandelbulber(config, "the voxel builder", error, error_len)) return 1;
     if (check_device(error, error_len)) return 1;
     Stream st;
     HIP_TRY(hipStreamCreate(&st.s), "hipStreamCreate");
@@ -1450,6 +1463,7 @@ int diagnostic_image(const FptRenderConfig *config, const FptDiagnosticConfig *d
         return 1;
     }
     if (validate_config(config, error, error_len)) return 1;
+    if (reject_mandelbulber(config, "the diagnostic renderer", error, error_len)) return 1;
     if (diagnostic->mode > FPT_DIAGNOSTIC_SDF_BOUNCE_CONTRIBUTION) {
         set_error(error, error_len, "unknown diagnostic mode %u", diagnostic->mode);
         return 1;

@@ -1230,4 +1230,250 @@ __device__ float4 kifs_field_sample(f3 p, const FptRenderConfig &cfg, int mult) 
     return make_float4(d, r, der, escaped ? -float(done) : float(done));
 }
 
+// ---------------------------------------------------------------------------
+// Rendering imported Mandelbulber scenes (sdf_id MANDELBULBER): the
+// reference's Mandelbulber branches of the camera (SM:473-511), the distance
+// (SM:975-995), the march (SM:3627-3751) and the normal (SM:3871-3960).
+// Parameters: vset_values[107] projection (0 perspective, 1 fish-eye,
+// 2 equirectangular, 3 fish-eye cut at 180 degrees), [108..113] step
+// clamps and DE factor, [114] normal step per threshold, [115] threshold
+// mode (0 fixed [117], 1 distance-scaled by [116], 2 iteration-limit
+// interior), [118..119] threshold clamp, [129] detail level.
+//
+// Only the one-path-per-lane kernel and the per-ray hooks call these, through
+// the *_any entry points below; the persistent kernel's march, material and
+// normal stay as they are.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float mb_threshold(f3 p, const FptRenderConfig &cfg) {  // SM:3634-3640
+    const float *v = cfg.vset_values;
+    const float t = v[115] > 0.5f ? length(ld3(cfg.camera_position) - p) * v[116] : v[117];
+    return clampf(t, v[118], v[119]);
+}
+
+// deMandelbulber: iteration-limit interior in mode 2, orbit = |iterations| /
+// configured iterations.
+__device__ float de_kifs(f3 p, const FptRenderConfig &cfg, float *orbit, Work &w) {  // SM:975-995
+    const float4 s = kifs_field_sample(p, cfg, 1);
+    w.orbit_iterations += int(fabsf(s.w));
+    float d = s.x;
+    const float *v = cfg.vset_values;
+    if (v[115] > 1.5f) {
+        const float thr = clampf(length(ld3(cfg.camera_position) - p) * v[116], v[118], v[119]);
+        if (s.w > 0.0f) d = 0.0f;
+        else if (d < thr) d = thr * 1.01f;
+    }
+    if (orbit) *orbit = fabsf(s.w) / fmaxf(cfg.set_values[1], 1.0f);
+    return d;
+}
+
+__device__ __forceinline__ float map_kifs(f3 p, const FptRenderConfig &cfg, Work &w) {  // SM:3612-3615
+    w.distance_evals++;
+    const float d = de_kifs(p, cfg, nullptr, w);
+    return mixf(d, d / 2.0f, cfg.render[6]);
+}
+
+__device__ __forceinline__ bool mb_projection_visible(float x, float y, const FptRenderConfig &cfg) {  // SM:473-480
+    const float proj = cfg.vset_values[107];
+    if (proj < 2.5f || proj > 3.5f) return true;
+    const float fov = cfg.camera_fov * kPi / 180.0f;
+    return sqrtf(x * x + y * y) <= 0.5f * kPi / fmaxf(fov, 1.0e-6f);
+}
+
+__device__ f3 mb_camera_ray(float x, float y, const FptRenderConfig &cfg) {  // SM:482-511
+    const float proj = cfg.vset_values[107];
+    const float fov = cfg.camera_fov * kPi / 180.0f;
+    f3 d;
+    if ((proj > 0.5f && proj < 1.5f) || (proj > 2.5f && proj < 3.5f)) {
+        const float r = sqrtf(x * x + y * y);
+        if (r <= 1.0e-8f) {
+            d = mk(0.f, 0.f, 1.f);
+        } else {
+            const float k = sinf(r * fov) / r;
+            d = mk(x * k, y * k, cosf(r * fov));
+        }
+    } else if (proj > 1.5f && proj < 2.5f) {
+        const float aspect = float(cfg.width) / float(cfg.height);
+        const float ax = x * (2.0f / aspect) * (0.5f * fov), ay = y * (0.5f * fov);
+        d = mk(sinf(ax) * cosf(ay), sinf(ay), cosf(ax) * cosf(ay));
+    } else {
+        d = normalize(mk(x, y, 1.0f / tanf(0.5f * fov)));
+    }
+    return rotate_camera_roll(normalize(d), cfg.camera_yaw_pitch[0], cfg.camera_yaw_pitch[1], cfg.camera_roll);
+}
+
+__device__ __forceinline__ float mb_step(float d, float thr, const FptRenderConfig &cfg) {  // SM:3642-3676
+    const float *v = cfg.vset_values;
+    float step = fmaxf(d - 0.5f * thr, 0.0f) * v[113];
+    if (v[108] > 0.5f) {
+        step = clampf(step, v[109], v[110]);
+        if (thr > v[109]) step = clampf(step, v[111] * thr, v[112] * thr);
+    } else {
+        step = fminf(step, v[110]);
+    }
+    return step;
+}
+
+__device__ __forceinline__ bool same3(f3 a, f3 b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
+
+// marchMandelbulber: march until d < threshold (the threshold grows with
+// camera distance), stopping where FP32 no longer moves the point, then up
+// to 30 half-steps bracketing the band (threshold * (1 - 0.001 detail),
+// threshold).  steps counts the march iterations, not the refinement.
+__device__ f3 march_mandelbulber(f3 dr, f3 rp, int ni, const FptRenderConfig &cfg, Work &w, int &steps) {
+    const f3 start = rp;
+    float d = 0.0f, thr = mb_threshold(rp, cfg), step = 0.0f;
+    bool found = false;
+    const int max_iter = ni < 10000 ? ni : 10000;
+    steps = 0;
+    for (int i = 0; i < max_iter; ++i) {
+        thr = mb_threshold(rp, cfg);
+        d = map_kifs(rp, cfg, w);
+        ++steps;
+        if (!isfinite(d)) break;
+        if (d < thr) {
+            found = true;
+            break;
+        }
+        step = mb_step(d, thr, cfg);
+        const f3 nx = rp + dr * step;
+        if (same3(nx, rp)) break;
+        rp = nx;
+        if (length(rp - start) >= cfg.render[4]) break;
+    }
+    if (!found) return rp;
+    const float limit = 1.0f - 0.001f * fmaxf(cfg.vset_values[129], 0.0f);
+    step *= 0.5f;
+    for (int r = 0; r < 30; ++r) {
+        if (d < thr && d > thr * limit) break;
+        if (d > thr) {
+            const f3 nx = rp + dr * step;
+            if (same3(nx, rp)) break;
+            rp = nx;
+        } else if (d < thr * limit) {
+            const f3 nx = rp - dr * step;
+            if (same3(nx, rp)) break;
+            rp = nx;
+        }
+        d = map_kifs(rp, cfg, w);
+        step *= 0.5f;
+    }
+    return rp;
+}
+
+// Normal-stencil field sample: 5x the iterations in iteration-limit mode,
+// the raw field distance (no interior rule, no half-step mix).
+__device__ __forceinline__ float4 mb_normal_sample(f3 p, const FptRenderConfig &cfg, Work &w) {  // SM:964-974
+    w.distance_evals++;
+    const float4 s = kifs_field_sample(p, cfg, cfg.vset_values[115] > 1.5f ? 5 : 1);
+    w.orbit_iterations += int(fabsf(s.w));
+    return s;
+}
+
+__device__ f3 normal_mandelbulber(f3 p, const FptRenderConfig &cfg, Work &w) {  // SM:3871-3960
+    const float *v = cfg.vset_values;
+    if (cfg.render[7] > 0.0f) {
+        // slow shading: the iteration-count field on an 11^3 stencil
+        const float delta = length(ld3(cfg.camera_position) - p) * cfg.render[7] * v[114];
+        const float max_it = cfg.set_values[1];
+        f3 g = mk(0.f, 0.f, 0.f);
+        for (int ix = -5; ix <= 5; ++ix)
+            for (int iy = -5; iy <= 5; ++iy)
+                for (int iz = -5; iz <= 5; ++iz) {
+                    const f3 sd = mk(float(ix), float(iy), float(iz)) * 0.2f;
+                    w.distance_evals++;
+                    const float4 s = kifs_field_sample(p + sd * delta, cfg, 1);
+                    w.orbit_iterations += int(fabsf(s.w));
+                    g = g + sd * (1.0f + max_it - fabsf(s.w));
+                }
+        const float m = fmaxf(fabsf(g.x), fmaxf(fabsf(g.y), fabsf(g.z)));
+        if (!(m > 0.0f) || !isfinite(m)) return mk(1.f, 0.f, 0.f);
+        return normalize(g / m);
+    }
+    const float e = mb_threshold(p, cfg) * v[114];
+    if (cfg.sdf_normal_mode == 1u) {
+        const float te = e * 0.57735026919f;
+        const f3 k0 = mk(1.f, -1.f, -1.f), k1 = mk(-1.f, -1.f, 1.f), k2 = mk(-1.f, 1.f, -1.f), k3 = mk(1.f, 1.f, 1.f);
+        const float d0 = mb_normal_sample(p + k0 * te, cfg, w).x, d1 = mb_normal_sample(p + k1 * te, cfg, w).x;
+        const float d2 = mb_normal_sample(p + k2 * te, cfg, w).x, d3 = mb_normal_sample(p + k3 * te, cfg, w).x;
+        return normalize_fd(k0 * d0 + k1 * d1 + k2 * d2 + k3 * d3);
+    }
+    const f3 ex = mk(e, 0.f, 0.f), ey = mk(0.f, e, 0.f), ez = mk(0.f, 0.f, e);
+    f3 n = mk(mb_normal_sample(p + ex, cfg, w).x - mb_normal_sample(p - ex, cfg, w).x,
+              mb_normal_sample(p + ey, cfg, w).x - mb_normal_sample(p - ey, cfg, w).x,
+              mb_normal_sample(p + ez, cfg, w).x - mb_normal_sample(p - ez, cfg, w).x);
+    if (!(fmaxf(fabsf(n.x), fmaxf(fabsf(n.y), fabsf(n.z))) > 0.0f)) {
+        // samples collapsed to one float: the log orbit radius has the same
+        // level-set orientation
+        auto lr = [&](f3 q) { return logf(fmaxf(fabsf(mb_normal_sample(q, cfg, w).y), 1.0e-30f)); };
+        n = mk(lr(p + ex) - lr(p - ex), lr(p + ey) - lr(p - ey), lr(p + ez) - lr(p - ez));
+    }
+    return normalize_fd(n);
+}
+
+__device__ __forceinline__ bool is_mandelbulber(const FptRenderConfig &cfg) {
+    return cfg.sdf_id == FPT_SDF_MANDELBULBER;
+}
+
+// Entry points for every SDF scene: the Mandelbulber branches above for
+// imported scenes, the functions of the SDF path otherwise.
+__device__ __forceinline__ f3 march_any(f3 dr, f3 rp, int ni, float min_dist, float lod_falloff,
+                                        const FptRenderConfig &cfg, Work &w, int &steps) {  // SM:3753-3760
+    if (is_mandelbulber(cfg)) return march_mandelbulber(dr, rp, ni, cfg, w, steps);
+    return march(dr, rp, ni, min_dist, lod_falloff, cfg, w, steps);
+}
+__device__ __forceinline__ f3 normal_any(f3 p, const FptRenderConfig &cfg, Work &w) {
+    if (is_mandelbulber(cfg)) return normal_mandelbulber(p, cfg, w);
+    return normal_at(p, cfg, w);
+}
+__device__ __forceinline__ float map_any(f3 p, const FptRenderConfig &cfg, Work &w) {
+    if (is_mandelbulber(cfg)) return map_kifs(p, cfg, w);
+    return map_sdf(p, cfg, w);
+}
+__device__ __forceinline__ Material material_any(f3 p, const FptRenderConfig &cfg, Work &w) {  // SM:3539-3562
+    if (!is_mandelbulber(cfg)) return user_material(p, cfg, w);
+    float orbit = 0.0f;
+    // a solid style does not read the orbit
+    if (cfg.fractal_style_mode != 2u) de_kifs(p, cfg, &orbit, w);
+    return material_at_param(material_param(orbit, cfg), cfg);
+}
+
+// estimateFocusDistance (SM:5869-5878) for every SDF scene: the centre ray
+// of the pinhole camera, marched as the scene marches.
+__device__ float focus_estimate_any(const FptRenderConfig &cfg, Work &w) {
+    if (!is_mandelbulber(cfg)) return focus_estimate(cfg, w);
+    const float focal = 1.0f / tanf(cfg.camera_fov / 2.0f * kPi / 180.0f);
+    const f3 dr = rotate_camera_roll(normalize(mk(0.f, 0.f, focal)), cfg.camera_yaw_pitch[0], cfg.camera_yaw_pitch[1],
+                                     cfg.camera_roll);
+    const f3 cam = ld3(cfg.camera_position);
+    int steps = 0;
+    const f3 hit = march_mandelbulber(dr, cam, 120, cfg, w, steps);
+    const float d = length(hit - cam);
+    if (!isfinite(d) || d <= 0.001f || d > cfg.render[4] * 0.99f) return 5.0f;
+    return d;
+}
+
+// The camera ray through screen point (x, y) (SM:5938-5949); false when a
+// cut fish-eye does not see it.
+__device__ __forceinline__ bool camera_ray_any(float x, float y, const FptRenderConfig &cfg, f3 &dr) {
+    if (is_mandelbulber(cfg)) {
+        if (!mb_projection_visible(x, y, cfg)) return false;
+        dr = mb_camera_ray(x, y, cfg);
+        return true;
+    }
+    const float focal = 1.0f / tanf(cfg.camera_fov / 2.0f * kPi / 180.0f);
+    dr = rotate_camera_roll(normalize(mk(x, y, focal)), cfg.camera_yaw_pitch[0], cfg.camera_yaw_pitch[1],
+                            cfg.camera_roll);
+    return true;
+}
+
+// sdfScreenUv (SM:6847-6855): Mandelbulber scenes sample pixel corners
+// offset (0, 1), the others pixel centres.
+__device__ __forceinline__ UV screen_uv_any(uint32_t x, uint32_t y, const FptRenderConfig &cfg) {
+    if (!is_mandelbulber(cfg)) return screen_uv(x, y, cfg.width, cfg.height);
+    const float fw = float(cfg.width), fh = float(cfg.height);
+    const float u = __fsub_rn(__fdiv_rn(float(x), fw), 0.5f);
+    const float v = __fsub_rn(__fdiv_rn(__fadd_rn(float(y), 1.0f), fh), 0.5f);
+    return {__fmul_rn(u, __fdiv_rn(fw, fh)), v};
+}
+
 }  // namespace fpt

@@ -4,6 +4,8 @@
 #ifndef __HIPCC_RTC__  // the runtime compiler provides these itself
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <string>
 #endif
 
 #include "../../include/fpt_abi.h"
@@ -55,6 +57,10 @@ hipError_t launch_sdf_batch(const FptRenderConfig *dcfgs, const BatchScene *dsce
 
 hipError_t launch_sdf_accumulate(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, float4 *accum,
                                  uint32_t start, uint32_t count, DeviceProfile *prof, hipStream_t stream);
+// Scene front end (scene_front.hip, fract_import.hip).
+void scene_default_config(FptRenderConfig &c);
+bool fract_load(const std::string &path, const FptSceneOptions &opt, FptRenderConfig &c, std::string &output_name,
+                char *error, size_t error_len);
 // HDRI hydration of a host config (hdri.hip): clears the LUT, then, for
 // world mode 2 with HDRI enabled, fills it from the Radiance file at
 // hdri_path.  Returns false with a message on failure.

@@ -28,7 +28,7 @@ __device__ f3 sun_contribution(f3 rp, UV xy, float seed, const Material &mat, f3
     float m = sqrtf(h2) * cfg.sun[4];
     ld = rotate_camera(ld, c * m, s * m);
     int steps = 0;
-    rp = march(ld, rp, int(cfg.render[1]), cfg.render[3], 0.0002f, cfg, w, steps);
+    rp = march_any(ld, rp, int(cfg.render[1]), cfg.render[3], 0.0002f, cfg, w, steps);
     shadow_steps += steps;
     if (length(rp0 - rp) > cfg.render[4] * 0.99f) {
         float k = cfg.sun[3] * fmaxf(dot(n, ld), 0.0f) * p1 * p2;
@@ -49,9 +49,8 @@ __device__ f3 render_path(UV xy, uint32_t sample_idx, const FptRenderConfig &cfg
     const float aa = 0.3f / fmaxf(float(cfg.width), float(cfg.height));
     float jx, jy;
     random_point(aa, xy, frame, jx, jy);
-    const float focal = 1.0f / tanf(cfg.camera_fov / 2.0f * kPi / 180.0f);
-    f3 dr = rotate_camera_roll(normalize(mk(xy.x + jx, xy.y + jy, focal)), cfg.camera_yaw_pitch[0],
-                               cfg.camera_yaw_pitch[1], cfg.camera_roll);
+    f3 dr;
+    if (!camera_ray_any(xy.x + jx, xy.y + jy, cfg, dr)) return mk(0.f, 0.f, 0.f);
     apply_dof(xy, frame, cfg, rp, dr);
     int local_ni = int(cfg.render[1]);
     float side = 1.0f;
@@ -67,7 +66,7 @@ __device__ f3 render_path(UV xy, uint32_t sample_idx, const FptRenderConfig &cfg
     const float far2 = far * far;
     for (int i = 0; i < bounces; ++i) {
         int steps = 0;
-        rp = march(dr, rp, local_ni, cfg.render[3], 0.0002f, cfg, w, steps);
+        rp = march_any(dr, rp, local_ni, cfg.render[3], 0.0002f, cfg, w, steps);
         if (i == 0) pc.primary_steps += steps; else pc.secondary_steps += steps;
         pc.bounces++;
         f3 tv = rp - cam;
@@ -78,9 +77,9 @@ __device__ f3 render_path(UV xy, uint32_t sample_idx, const FptRenderConfig &cfg
             light = light + environment(dr, cfg);
             break;
         }
-        Material mat = user_material(rp, cfg, w);
+        Material mat = material_any(rp, cfg, w);
         if (mat.emission > 0.001f) light = light + mat.rgb * mat.emission;
-        f3 n = normal_at(rp, cfg, w);
+        f3 n = normal_any(rp, cfg, w);
         pc.normal_evals++;
         if (cfg.sun[0] == 1.0f) light = light + sun_contribution(rp, xy, frame, mat, n, cfg, w, pc.shadow_steps);
         const float fi = float(i);
@@ -153,7 +152,7 @@ __global__ __launch_bounds__(256) void sdf_accumulate_kernel(const FptRenderConf
     Work w = {0, 0};
     PathCounts pc = {0, 0, 0, 0, 0};
     if (active) {
-        const UV uv = screen_uv(x, y, cfg.width, cfg.height);
+        const UV uv = screen_uv_any(x, y, cfg);
         const size_t idx = size_t(y) * cfg.width + x;
         f3 c = mk(0.f, 0.f, 0.f);
         if (start != 0u) {
@@ -187,19 +186,18 @@ __global__ __launch_bounds__(256) void trace_primary_kernel(const FptRenderConfi
     uint32_t x, y;
     tile_pixel(x, y);
     if (x >= cfg.width || y >= cfg.height) return;
-    const UV uv = screen_uv(x, y, cfg.width, cfg.height);
+    const UV uv = screen_uv_any(x, y, cfg);
     const f3 cam = ld3(cfg.camera_position);
-    const float focal = 1.0f / tanf(cfg.camera_fov / 2.0f * kPi / 180.0f);
-    f3 dr = rotate_camera_roll(normalize(mk(uv.x, uv.y, focal)), cfg.camera_yaw_pitch[0], cfg.camera_yaw_pitch[1],
-                               cfg.camera_roll);
+    f3 dr = mk(0.f, 0.f, 0.f);
+    const bool seen = camera_ray_any(uv.x, uv.y, cfg, dr);
     Work w = {0, 0};
     int steps = 0;
-    f3 rp = march(dr, cam, int(cfg.render[1]), cfg.render[3], 0.0002f, cfg, w, steps);
+    f3 rp = seen ? march_any(dr, cam, int(cfg.render[1]), cfg.render[3], 0.0002f, cfg, w, steps) : cam;
     f3 tv = rp - cam;
     float travel2 = dot(tv, tv);
     float far = cfg.render[4] * 0.99f;
-    bool hit = travel2 <= far * far;
-    f3 n = hit ? normal_at(rp, cfg, w) : mk(0.f, 0.f, 0.f);
+    bool hit = seen && travel2 <= far * far;
+    f3 n = hit ? normal_any(rp, cfg, w) : mk(0.f, 0.f, 0.f);
     FptRayHit h;
     h.t = sqrtf(travel2);
     h.position[0] = rp.x; h.position[1] = rp.y; h.position[2] = rp.z;
@@ -564,7 +562,7 @@ __global__ void focus_prepass_kernel(FptRenderConfig *cfgs, uint32_t n) {
     FptRenderConfig &cfg = cfgs[i];
     if (!(cfg.camera_dof > 0.0f) || cfg.focus_distance > 0.0f) return;
     Work w = {0, 0};
-    const float f = focus_estimate(cfg, w);
+    const float f = focus_estimate_any(cfg, w);
     cfg.focus_distance = f;
 }
 

@@ -1119,10 +1119,7 @@ bool scene_apply_program_json(FptRenderConfig &c, const std::string &text, char 
 
 bool scene_load(const std::string &scene_path, const std::string &fpt_root, const FptSceneOptions &opt,
                 FptRenderConfig &c, std::string &output_name, char *error, size_t error_len) {
-    if (extension_of(scene_path) == "fract") {
-        seterr(error, error_len, "Mandelbulber .fract scenes need the Mandelbulber2 formula source; not supported");
-        return false;
-    }
+    if (extension_of(scene_path) == "fract") return fract_load(scene_path, opt, c, output_name, error, error_len);
     std::string text;  // scene.rs:1619-1976
     if (!read_file(scene_path, text)) {
         seterr(error, error_len, "failed to read scene %s: %s", scene_path.c_str(), std::strerror(errno ? errno : ENOENT));

@@ -49,6 +49,8 @@ def lib():
         L.orc_focus_estimate.restype = C.c_double
         L.orc_map_distance.argtypes = [C.c_void_p, C.c_int, dp, C.c_size_t, dp]
         L.orc_normal.argtypes = [C.c_void_p, C.c_int, dp, C.c_size_t, dp]
+        L.orc_camera_ray.argtypes = [C.c_void_p, C.c_int, P(C.c_uint32), C.c_size_t, dp]
+        L.orc_march.argtypes = [C.c_void_p, C.c_int, dp, C.c_size_t, dp]
         L.orc_program_surface.argtypes = [C.c_void_p, C.c_int, dp, C.c_size_t, dp]
         L.orc_leaf_orbit.argtypes = [C.c_uint32, P(C.c_float), C.c_int, dp, dp]
         L.orc_mandelbulb_step.argtypes = [dp, C.c_double]
@@ -256,6 +258,28 @@ def normal(cfg, points, precision: int = 64) -> np.ndarray:
     p = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
     out = np.zeros((len(p), 3), np.float64)
     lib().orc_normal(C.addressof(cfg), precision, _dp(p), len(p), _dp(out))
+    return out
+
+
+def camera_ray(cfg, pixels, precision: int = 64) -> np.ndarray:
+    """(n, 6) {u, v, dx, dy, dz, seen} of the un-jittered camera ray of pixels
+    (n, 2) {x, y}."""
+    xy = np.ascontiguousarray(pixels, np.uint32).reshape(-1, 2)
+    out = np.zeros((len(xy), 6), np.float64)
+    if lib().orc_camera_ray(C.addressof(cfg), precision, xy.ctypes.data_as(C.POINTER(C.c_uint32)), len(xy),
+                            _dp(out)) != 0:
+        raise RuntimeError("orc_camera_ray failed")
+    return out
+
+
+def march(cfg, origins, directions, precision: int = 64) -> np.ndarray:
+    """(n, 4) {x, y, z, steps}: the scene's march along each ray."""
+    o = np.asarray(origins, np.float64).reshape(-1, 3)
+    d = np.asarray(directions, np.float64).reshape(-1, 3)
+    rays = np.ascontiguousarray(np.concatenate([o, d], axis=1))
+    out = np.zeros((len(rays), 4), np.float64)
+    if lib().orc_march(C.addressof(cfg), precision, _dp(rays), len(rays), _dp(out)) != 0:
+        raise RuntimeError("orc_march failed")
     return out
 
 

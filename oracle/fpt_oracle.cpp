@@ -20,6 +20,14 @@
 //                          analytic hit t for a sphere at the origin.
 //   render                 FP32 twin vs FP64 (agreement), sphere scene
 //                          direct-light closed form at the image centre.
+//   Mandelbulber scenes    (tests/test_mandelbulber_render_oracle.py) a KIFS
+//                          whose field is an exact sphere distance: march
+//                          end points in the refinement band, radial
+//                          normals (central, tetrahedral, slow shading), the
+//                          iteration-limit interior rule; the projections'
+//                          defining angles; the (0, 1) pixel offset.
+//                          normal_mandelbulber's log-orbit-radius fallback
+//                          (six equal central samples): parity unpinned.
 //
 // The RNG (SM:541-576) keys on the IEEE bit patterns of float32 values, so
 // every RNG key and the pixel uv are computed in float32 with no contraction
@@ -246,11 +254,13 @@ V3<T> rotate_camera_roll(V3<T> v, T yaw, T pitch, T roll) {
     return rotate_camera(r, yaw, pitch);
 }
 
-// sdfScreenUv (SM:6847-6855) in float32, as the GPU computes it.
+// sdfScreenUv (SM:6847-6855) in float32, as the GPU computes it: pixel
+// centres, or the pixel offset (0, 1) for Mandelbulber scenes.
 UV screen_uv(uint32_t x, uint32_t y, const FptRenderConfig &cfg) {
+    const bool mb = cfg.sdf_id == FPT_SDF_MANDELBULBER;
     volatile float fw = float(cfg.width), fh = float(cfg.height);
-    volatile float ux = (float(x) + 0.5f) / fw;
-    volatile float uy = (float(y) + 0.5f) / fh;
+    volatile float ux = (float(x) + (mb ? 0.0f : 0.5f)) / fw;
+    volatile float uy = (float(y) + (mb ? 1.0f : 0.5f)) / fh;
     float u0 = ux - 0.5f, u1 = uy - 0.5f;
     volatile float aspect = fw / fh;
     volatile float u0a = u0 * aspect;
@@ -262,6 +272,40 @@ V3<T> camera_dir(T sx, T sy, const FptRenderConfig &cfg) {
     T focal = T(1) / std::tan(T(cfg.camera_fov) / T(2) * T(kPi) / T(180));
     return rotate_camera_roll(normalize(V3<T>(sx, sy, focal)), T(cfg.camera_yaw_pitch[0]),
                               T(cfg.camera_yaw_pitch[1]), T(cfg.camera_roll));
+}
+
+// The camera ray of renderPath (SM:5938-5949): Mandelbulber scenes use
+// mandelbulberCameraRay (SM:482-511) with vset_values[107] as the projection
+// (0 perspective, 1 fish-eye, 2 equirectangular, 3 fish-eye cut at 180
+// degrees, whose outside is not seen: false).
+template <class T>
+bool camera_ray(T sx, T sy, const FptRenderConfig &cfg, V3<T> &dr) {
+    if (cfg.sdf_id != FPT_SDF_MANDELBULBER) {
+        dr = camera_dir<T>(sx, sy, cfg);
+        return true;
+    }
+    const float proj = cfg.vset_values[107];
+    const T fov = T(cfg.camera_fov) * T(kPi) / T(180);
+    const T radius = std::sqrt(sx * sx + sy * sy);
+    if (proj >= 2.5f && proj <= 3.5f && !(radius <= T(0.5) * T(kPi) / m_max(fov, T(1.0e-6f)))) return false;
+    V3<T> d;
+    if ((proj > 0.5f && proj < 1.5f) || (proj > 2.5f && proj < 3.5f)) {
+        if (radius <= T(1.0e-8f)) {
+            d = V3<T>(T(0), T(0), T(1));
+        } else {
+            const T k = std::sin(radius * fov) / radius;
+            d = V3<T>(sx * k, sy * k, std::cos(radius * fov));
+        }
+    } else if (proj > 1.5f && proj < 2.5f) {
+        const T aspect = T(float(cfg.width) / float(cfg.height));
+        const T ax = sx * (T(2) / aspect) * (T(0.5) * fov), ay = sy * (T(0.5) * fov);
+        d = V3<T>(std::sin(ax) * std::cos(ay), std::sin(ay), std::cos(ax) * std::cos(ay));
+    } else {
+        d = normalize(V3<T>(sx, sy, T(1) / std::tan(T(0.5) * fov)));
+    }
+    dr = rotate_camera_roll(normalize(d), T(cfg.camera_yaw_pitch[0]), T(cfg.camera_yaw_pitch[1]),
+                            T(cfg.camera_roll));
+    return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -801,11 +845,44 @@ bool is_program(const FptRenderConfig &cfg) {
     return cfg.sdf_id == FPT_SDF_PROGRAM && cfg.sdf_program_count > 0;
 }
 
+#include "fpt_oracle_kifs.inc"
+
+// Mandelbulber march threshold (SM:3634-3640): fixed (vset_values[117]) or
+// scaled by the camera distance (mode [115] > 0.5, factor [116]), clamped
+// to [118..119].
+template <class T>
+T mb_threshold(V3<T> p, const FptRenderConfig &cfg) {
+    const float *v = cfg.vset_values;
+    const T t = v[115] > 0.5f ? length(load3<T>(cfg.camera_position) - p) * T(v[116]) : T(v[117]);
+    return m_clamp(t, T(v[118]), T(v[119]));
+}
+
+// deMandelbulber (SM:975-995): in iteration-limit mode (vset_values[115] >
+// 1.5) points that reach the iteration limit are interior (distance 0) and
+// escaped points stay just outside the threshold; orbit = |iterations| /
+// configured iterations.
+template <class T>
+T de_kifs(V3<T> p, const FptRenderConfig &cfg, T *orbit) {
+    T s[4];
+    kifs_field_sample<T>(p, cfg, 1, s);
+    count_orbit(uint64_t(std::fabs(s[3])));
+    T d = s[0];
+    const float *v = cfg.vset_values;
+    if (v[115] > 1.5f) {
+        const T thr = m_clamp(length(load3<T>(cfg.camera_position) - p) * T(v[116]), T(v[118]), T(v[119]));
+        if (s[3] > T(0)) d = T(0);
+        else if (d < thr) d = thr * T(1.01f);
+    }
+    if (orbit) *orbit = std::fabs(s[3]) / m_max(T(cfg.set_values[1]), T(1));
+    return d;
+}
+
 template <class T>
 T distance_sdf(V3<T> p, const FptRenderConfig &cfg) {  // SM:3564-3610
     if (tl_counters) tl_counters->distance_evals++;
     if (is_program<T>(cfg)) return program_eval<T>(geometry_program(cfg), p, nullptr);
     switch (cfg.sdf_id) {
+        case FPT_SDF_MANDELBULBER: return de_kifs<T>(p, cfg, nullptr);
         case FPT_SDF_MANDELBOX_FRACTAL: return de_mandelbox_builtin<T>(p, cfg, nullptr);
         case FPT_SDF_MENGER_SPONGE: return de_menger_builtin<T>(p, cfg);
         default: return T(1000);
@@ -829,6 +906,7 @@ T material_orbit(V3<T> p, const FptRenderConfig &cfg) {  // SM:3485-3562 (orbit 
     }
     if (cfg.sdf_id == FPT_SDF_MANDELBOX_FRACTAL) de_mandelbox_builtin<T>(p, cfg, &orbit);
     else if (cfg.sdf_id == FPT_SDF_MENGER_SPONGE) orbit = T(1);
+    else if (cfg.sdf_id == FPT_SDF_MANDELBULBER && cfg.fractal_style_mode != 2u) de_kifs<T>(p, cfg, &orbit);
     return orbit;
 }
 
@@ -888,10 +966,75 @@ bool has_translucent(const FptRenderConfig &cfg) {  // SM:3617-3624 (program sce
     return cfg.sdf_id == FPT_SDF_PROGRAM && cfg.program_material[5] > 0.0f;
 }
 
-// march (SM:3753-3783) for non-Mandelbulber scenes; steps counted.
+// marchMandelbulber (SM:3700-3751): march until d < threshold with the
+// clamped Mandelbulber step (SM:3642-3676), stopping where the point no
+// longer moves or max_distance is travelled, then up to 30 half-steps
+// towards the band (threshold * (1 - 0.001 detail), threshold); steps counts
+// the march iterations.
+template <class T>
+T mb_step(T d, T thr, const FptRenderConfig &cfg) {
+    const float *v = cfg.vset_values;
+    T step = m_max(d - T(0.5f) * thr, T(0)) * T(v[113]);
+    if (v[108] > 0.5f) {
+        step = m_clamp(step, T(v[109]), T(v[110]));
+        if (thr > T(v[109])) step = m_clamp(step, T(v[111]) * thr, T(v[112]) * thr);
+    } else {
+        step = m_min(step, T(v[110]));
+    }
+    return step;
+}
+
+template <class T>
+bool same_point(V3<T> a, V3<T> b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
+
+template <class T>
+V3<T> march_mandelbulber(V3<T> dr, V3<T> rp, int ni, const FptRenderConfig &cfg, int *steps_out) {
+    const V3<T> start = rp;
+    T d = T(0), thr = mb_threshold(rp, cfg), step = T(0);
+    bool found = false;
+    int steps = 0;
+    const int max_iter = std::min(ni, 10000);
+    for (int i = 0; i < max_iter; ++i) {
+        thr = mb_threshold(rp, cfg);
+        d = map_sdf(rp, cfg);
+        ++steps;
+        if (!std::isfinite(d)) break;
+        if (d < thr) {
+            found = true;
+            break;
+        }
+        step = mb_step(d, thr, cfg);
+        const V3<T> nx = rp + dr * step;
+        if (same_point(nx, rp)) break;
+        rp = nx;
+        if (length(rp - start) >= T(cfg.render[4])) break;
+    }
+    if (steps_out) *steps_out = steps;
+    if (!found) return rp;
+    const T limit = T(1) - T(0.001f) * m_max(T(cfg.vset_values[129]), T(0));
+    step *= T(0.5f);
+    for (int r = 0; r < 30; ++r) {
+        if (d < thr && d > thr * limit) break;
+        if (d > thr) {
+            const V3<T> nx = rp + dr * step;
+            if (same_point(nx, rp)) break;
+            rp = nx;
+        } else if (d < thr * limit) {
+            const V3<T> nx = rp - dr * step;
+            if (same_point(nx, rp)) break;
+            rp = nx;
+        }
+        d = map_sdf(rp, cfg);
+        step *= T(0.5f);
+    }
+    return rp;
+}
+
+// march (SM:3753-3783); Mandelbulber scenes take marchMandelbulber.
 template <class T>
 V3<T> march(V3<T> dr, V3<T> rp, int ni, T min_dist, T lod_falloff, const FptRenderConfig &cfg,
             int *steps_out) {
+    if (cfg.sdf_id == FPT_SDF_MANDELBULBER) return march_mandelbulber(dr, rp, ni, cfg, steps_out);
     V3<T> cam = rp;
     int max_iter = std::min(ni, 360);
     bool check_t = has_translucent<T>(cfg);
@@ -967,10 +1110,64 @@ V3<T> normalize_fd(V3<T> g) {  // SM:3863-3869
     return normalize(g / m);
 }
 
-// normalAt (SM:3871-3971), non-Mandelbulber branches.
+// The Mandelbulber normal stencil's field sample (SM:964-974): 5x the
+// iterations in iteration-limit mode, the raw field distance.
+template <class T>
+void mb_normal_sample(V3<T> p, const FptRenderConfig &cfg, int mult, T s[4]) {
+    if (tl_counters) tl_counters->distance_evals++;
+    kifs_field_sample<T>(p, cfg, mult, s);
+    count_orbit(uint64_t(std::fabs(s[3])));
+}
+
+// normalAt's Mandelbulber branches (SM:3871-3960): the slow-shading 11^3
+// iteration-count stencil when render[7] > 0; otherwise the tetrahedral or
+// central stencil at threshold * vset_values[114], with the log orbit
+// radius as the fallback field when the six central samples are equal.
+template <class T>
+V3<T> normal_mandelbulber(V3<T> p, const FptRenderConfig &cfg) {
+    const float *v = cfg.vset_values;
+    T s[4];
+    if (cfg.render[7] > 0.0f) {
+        const T delta = length(load3<T>(cfg.camera_position) - p) * T(cfg.render[7]) * T(v[114]);
+        const T max_it = T(cfg.set_values[1]);
+        V3<T> g;
+        for (int ix = -5; ix <= 5; ++ix)
+            for (int iy = -5; iy <= 5; ++iy)
+                for (int iz = -5; iz <= 5; ++iz) {
+                    const V3<T> sd = V3<T>(T(ix), T(iy), T(iz)) * T(0.2f);
+                    mb_normal_sample<T>(p + sd * delta, cfg, 1, s);
+                    g = g + sd * (T(1) + max_it - std::fabs(s[3]));
+                }
+        const T m = m_max(std::fabs(g.x), m_max(std::fabs(g.y), std::fabs(g.z)));
+        if (!(m > T(0)) || !std::isfinite(m)) return V3<T>(T(1), T(0), T(0));
+        return normalize(g / m);
+    }
+    const int mult = v[115] > 1.5f ? 5 : 1;
+    auto dist = [&](V3<T> q) { mb_normal_sample<T>(q, cfg, mult, s); return s[0]; };
+    const T e = mb_threshold(p, cfg) * T(v[114]);
+    if (cfg.sdf_normal_mode == 1u) {
+        const T te = e * T(0.57735026919f);
+        const V3<T> k0(T(1), T(-1), T(-1)), k1(T(-1), T(-1), T(1)), k2(T(-1), T(1), T(-1)), k3(T(1), T(1), T(1));
+        const T d0 = dist(p + k0 * te), d1 = dist(p + k1 * te), d2 = dist(p + k2 * te), d3 = dist(p + k3 * te);
+        return normalize_fd(k0 * d0 + k1 * d1 + k2 * d2 + k3 * d3);
+    }
+    const V3<T> ex(e, T(0), T(0)), ey(T(0), e, T(0)), ez(T(0), T(0), e);
+    V3<T> n(dist(p + ex) - dist(p - ex), dist(p + ey) - dist(p - ey), dist(p + ez) - dist(p - ez));
+    if (!(m_max(std::fabs(n.x), m_max(std::fabs(n.y), std::fabs(n.z))) > T(0))) {
+        auto lr = [&](V3<T> q) {
+            mb_normal_sample<T>(q, cfg, mult, s);
+            return std::log(m_max(std::fabs(s[1]), T(1.0e-30)));
+        };
+        n = V3<T>(lr(p + ex) - lr(p - ex), lr(p + ey) - lr(p - ey), lr(p + ez) - lr(p - ez));
+    }
+    return normalize_fd(n);
+}
+
+// normalAt (SM:3871-3971).
 template <class T>
 V3<T> normal_at(V3<T> p, const FptRenderConfig &cfg) {
     if (tl_counters) tl_counters->normal_evals++;
+    if (cfg.sdf_id == FPT_SDF_MANDELBULBER) return normal_mandelbulber(p, cfg);
     T e = m_max(T(cfg.render[2]), T(0.0002f));
     if ((cfg.sdf_normal_mode == 0u || cfg.sdf_normal_mode == 2u) && is_program<T>(cfg)) {
         V3<T> g;
@@ -1123,7 +1320,8 @@ V3<T> render_path(UV xy, uint32_t sample_idx, const FptRenderConfig &cfg) {
     T aa = T(0.3f / aa_den);
     T jx, jy;
     random_point<T>(aa, xy, frame, jx, jy);
-    V3<T> dr = camera_dir<T>(T(xy.x) + jx, T(xy.y) + jy, cfg);
+    V3<T> dr;
+    if (!camera_ray<T>(T(xy.x) + jx, T(xy.y) + jy, cfg, dr)) return V3<T>();
     apply_dof<T>(xy, frame, cfg, rp, dr);
     int local_ni = int(cfg.render[1]);
     T side = T(1);
@@ -1254,13 +1452,14 @@ void trace_primary(const FptRenderConfig &cfg, int threads, double *out) {
         uint32_t x = uint32_t(i % W), y = uint32_t(i / W);
         UV uv = screen_uv(x, y, cfg);
         V3<T> cam = load3<T>(cfg.camera_position);
-        V3<T> dr = camera_dir<T>(T(uv.x), T(uv.y), cfg);
+        V3<T> dr;
+        const bool seen = camera_ray<T>(T(uv.x), T(uv.y), cfg, dr);
         int steps = 0;
-        V3<T> rp = march(dr, cam, int(cfg.render[1]), T(cfg.render[3]), T(0.0002f), cfg, &steps);
+        V3<T> rp = seen ? march(dr, cam, int(cfg.render[1]), T(cfg.render[3]), T(0.0002f), cfg, &steps) : cam;
         V3<T> tv = rp - cam;
         T travel2 = dot(tv, tv);
         T far = T(cfg.render[4]) * T(0.99f);
-        bool hit = travel2 <= far * far;
+        bool hit = seen && travel2 <= far * far;
         V3<T> nrm = hit ? normal_at(rp, cfg) : V3<T>();
         double *o = out + size_t(i) * 9;
         o[0] = double(std::sqrt(travel2));
@@ -1272,7 +1471,6 @@ void trace_primary(const FptRenderConfig &cfg, int threads, double *out) {
 }
 
 #include "fpt_oracle_bounds.inc"
-#include "fpt_oracle_kifs.inc"
 #include "fpt_oracle_voxel.inc"
 #include "fpt_oracle_diag.inc"
 
@@ -1367,6 +1565,51 @@ int orc_normal(const FptRenderConfig *cfg, int precision, const double *pts, siz
             V3<float> r = normal_at(V3<float>(float(pts[3 * i]), float(pts[3 * i + 1]), float(pts[3 * i + 2])), *cfg);
             normals[3 * i] = r.x; normals[3 * i + 1] = r.y; normals[3 * i + 2] = r.z;
         }
+    }
+    return 0;
+}
+
+// Pixel (x, y) -> screen uv -> camera ray, un-jittered: out = {u, v, dx,
+// dy, dz, seen} per pixel.
+int orc_camera_ray(const FptRenderConfig *cfg, int precision, const uint32_t *xy, size_t n, double *out) {
+    if (!cfg || cfg->width == 0 || cfg->height == 0) return 1;
+    for (size_t i = 0; i < n; ++i) {
+        const UV uv = screen_uv(xy[2 * i], xy[2 * i + 1], *cfg);
+        double *o = out + 6 * i;
+        o[0] = uv.x;
+        o[1] = uv.y;
+        if (precision == 64) {
+            V3<double> d;
+            o[5] = camera_ray<double>(uv.x, uv.y, *cfg, d) ? 1.0 : 0.0;
+            o[2] = d.x; o[3] = d.y; o[4] = d.z;
+        } else {
+            V3<float> d;
+            o[5] = camera_ray<float>(uv.x, uv.y, *cfg, d) ? 1.0 : 0.0;
+            o[2] = d.x; o[3] = d.y; o[4] = d.z;
+        }
+    }
+    return 0;
+}
+
+// The scene's march from rays {ox, oy, oz, dx, dy, dz}: out = {x, y, z,
+// steps} per ray (threshold render[3], LOD falloff 0.0002, render[1] steps).
+int orc_march(const FptRenderConfig *cfg, int precision, const double *rays, size_t n, double *out) {
+    if (!cfg) return 1;
+    for (size_t i = 0; i < n; ++i) {
+        const double *r = rays + 6 * i;
+        int steps = 0;
+        double *o = out + 4 * i;
+        if (precision == 64) {
+            V3<double> q = march(V3<double>(r[3], r[4], r[5]), V3<double>(r[0], r[1], r[2]), int(cfg->render[1]),
+                                 double(cfg->render[3]), double(0.0002f), *cfg, &steps);
+            o[0] = q.x; o[1] = q.y; o[2] = q.z;
+        } else {
+            V3<float> q = march(V3<float>(float(r[3]), float(r[4]), float(r[5])),
+                                V3<float>(float(r[0]), float(r[1]), float(r[2])), int(cfg->render[1]),
+                                cfg->render[3], 0.0002f, *cfg, &steps);
+            o[0] = q.x; o[1] = q.y; o[2] = q.z;
+        }
+        o[3] = steps;
     }
     return 0;
 }

@@ -58,6 +58,9 @@ int orc_map_distance(const struct FptRenderConfig *cfg, int precision,
 /* Scene surface gradient used by normalAt (analytic for typed programs,
  * central differences otherwise) at n points: out = {d, gx, gy, gz}, and
  * the unit normal normalAt() returns: normals = {nx, ny, nz}. */
+int orc_camera_ray(const struct FptRenderConfig *cfg, int precision, const uint32_t *xy, size_t n,
+                   double *out);
+int orc_march(const struct FptRenderConfig *cfg, int precision, const double *rays, size_t n, double *out);
 int orc_normal(const struct FptRenderConfig *cfg, int precision,
                const double *points, size_t n, double *normals);
 
