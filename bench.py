@@ -70,6 +70,24 @@ TRAFFIC_FETCH_KIB, TRAFFIC_WRITE_KIB = 1596.1875, 116918.0625
 TRAFFIC_FETCH_SCALE, TRAFFIC_WRITE_SCALE = 2.0, 1.0
 TRAFFIC_WORKLOAD = (1920, 1080, 64)
 
+# Typed-program scene: orbit iterations are a mix of Mandelbox (box fold 12,
+# |z|^2 5, sphere fold 6, scale + offset 6, derivative 2 = 31 FLOP) and
+# Menger (abs 3, sort 6, scale + offset 6, cut 2 = 17 FLOP) iterations that
+# the counting pass does not separate; each is charged the smaller figure,
+# so the reported FLOP rate is a lower bound.  Per distance evaluation
+# outside the leaves: translate 3 + rotate 6 + box fold 12 + translate 3 +
+# union 1 + the march step's 8 + 3.
+TYPED_FLOPS_PER_ORBIT_ITERATION = 17
+TYPED_FLOPS_PER_DISTANCE_EVAL = 36
+# Voxel DDA: algorithmic bytes = 4 B per page-table entry read (once per
+# brick entered) + 1 B per sparse-bricks8 material id read.
+VOXEL_PAGE_ENTRY_BYTES, VOXEL_CELL_BYTES = 4, 1
+# HBM traffic of one voxel frame launch (1920x1080, 4 spp), FETCH_SIZE and
+# WRITE_SIZE KiB per dispatch (profiles/voxel_accumulate_table_mandelbox_1024_1080p_4spp.md),
+# scaled like the default line's.
+VOXEL_TRAFFIC_FETCH_KIB, VOXEL_TRAFFIC_WRITE_KIB = 580290.8125, 671285.78125
+VOXEL_TRAFFIC_WORKLOAD = (1920, 1080, 4, 1024)
+HBM_PEAK_GBS = 8000.0      # MI355X published HBM3E bandwidth
 FP32_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
 BF16_PEAK_TFLOPS = 2500.0  # MI355X published dense BF16 matrix peak
 KERNEL = "sdf_persistent_kernel"
@@ -87,6 +105,9 @@ def parse():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=8)
     ap.add_argument("--ref-spp-factor", type=int, default=4)
+    ap.add_argument("--workload", choices=("learned", "typed", "voxel"), default="learned",
+                    help="learned: the learned-step scene above (the default line); typed: the typed-program "
+                         "scene (1280x720, 16 spp); voxel: the 1024^3 sparse-brick Mandelbox (1920x1080, 4 spp)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy kernel of MIB MiB (the known-size reference for "
                          "HBM byte counters) and exit")
@@ -147,6 +168,9 @@ def main():
         src.fill(1.0)
         api.device_copy(dst, src)
         print(json.dumps({"copy_kernel_bytes_read": n * 4, "copy_kernel_bytes_written": n * 4}), flush=True)
+        return
+    if args.workload != "learned":
+        run_workload(args, api, scenes)
         return
     cfg = scenes.learned_step_scene(args.width, args.height, args.spp)
     samples_per_step = cfg.width * cfg.height * cfg.samples
@@ -231,6 +255,65 @@ def main():
     }
     if base is not None:
         line["cpu_baseline"] = base
+    print(json.dumps(line), flush=True)
+
+
+def run_workload(args, api, scenes):
+    """One JSON line for the typed-program or the voxel workload, with the
+    same contract keys as the default line (its own roofline)."""
+    from fractalpathtracer_amd import ffi
+    if args.workload == "typed":
+        cfg = scenes.typed_program_scene()
+        desc = "typed program: Menger(5) box-folded and rotated 30 deg about Y, union Mandelbox(-1.5, 10) " \
+               "offset (2,0,0), 1280x720, 16 spp, 4 bounces, Russian roulette after bounce 2"
+    else:
+        cfg = scenes.voxel_mandelbox_scene()
+        desc = "voxel: Mandelbox(2.0, 12) on a 1024^3 grid of sparse 8^3 bricks (shell coverage), " \
+               "exact DDA, 1920x1080, 4 spp, 3 bounces"
+    samples_per_step = cfg.width * cfg.height * cfg.samples
+    stream = api.Stream()
+    accum = api.DeviceBuffer(cfg.width * cfg.height * 4)
+    t0 = time.perf_counter()
+    plan = api.Plan(cfg)
+    build_s = time.perf_counter() - t0
+    if args.warmup:
+        api.time_renders(plan, accum, stream, args.warmup)
+    ms = api.time_renders(plan, accum, stream, args.steps) / args.steps
+    line = {"metric": f"path samples/s, {args.workload} workload", "value": samples_per_step / (ms / 1e3),
+            "unit": "path samples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic",
+            "config": {"workload": desc, "width": cfg.width, "height": cfg.height, "spp": cfg.samples,
+                       "plan_create_s": build_s}}
+    if args.workload == "typed":
+        prof, _ = plan.profile(accum)
+        flops = (prof["march_orbit_iterations"] * TYPED_FLOPS_PER_ORBIT_ITERATION +
+                 prof["distance_evals"] * TYPED_FLOPS_PER_DISTANCE_EVAL)
+        achieved = flops / (ms / 1e3) / 1e12
+        line["roofline"] = {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                            "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "kernel": KERNEL,
+                            "flops_per_launch": flops, "lower_bound": True}
+        line["config"]["counts"] = {k: prof[k] for k in ("distance_evals", "march_orbit_iterations",
+                                                         "normal_evals", "primary_steps", "shadow_steps")}
+        if not args.no_cpu_baseline:
+            base, _, _, _ = oracle_band(cfg, args.cpu_rows, 1)
+            line["cpu_baseline"] = base
+    else:
+        st = plan.voxel_stats(accum)
+        assert cfg.voxel_storage == ffi.VOXEL_STORAGE_SPARSE_BRICKS8
+        nbytes = st["page_reads"] * VOXEL_PAGE_ENTRY_BYTES + st["cell_reads"] * VOXEL_CELL_BYTES
+        launches = -(-cfg.samples // 8) if cfg.samples > 16 else 1
+        achieved = nbytes / (ms / 1e3) / 1e9
+        traffic = None
+        if (cfg.width, cfg.height, cfg.samples, cfg.voxel_resolution) == VOXEL_TRAFFIC_WORKLOAD:
+            traffic = 1024 * (VOXEL_TRAFFIC_FETCH_KIB * TRAFFIC_FETCH_SCALE +
+                              VOXEL_TRAFFIC_WRITE_KIB * TRAFFIC_WRITE_SCALE)
+        line["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                            "kernel": "voxel_accumulate_table_kernel",
+                            "launches_per_step": launches, "bytes_per_step": nbytes}
+        line["config"]["counts"] = st
+    plan.close()
     print(json.dumps(line), flush=True)
 
 

@@ -426,6 +426,14 @@ class Plan:
         return ({k: getattr(prof, k) for k, _ in ffi.FptSdfProfileStats._fields_[:13]},
                 {k: getattr(lst, k) for k, _ in ffi.FptProposerStats._fields_})
 
+    def voxel_stats(self, buffer: "DeviceBuffer") -> dict:
+        """Counting pass of a voxel plan's full frame: cells visited, page-table
+        entries and cell payloads read by every traversal."""
+        st = ffi.FptVoxelTraceStats()
+        err = ffi.error_buffer()
+        ffi.check(self.lib.fpt_hip_plan_voxel_stats(self.handle, buffer.ptr, C.byref(st), err, len(err)), err)
+        return {k: getattr(st, k) for k, _ in ffi.FptVoxelTraceStats._fields_}
+
     def close(self) -> None:
         if self.handle:
             self.lib.fpt_hip_plan_destroy(self.handle)

@@ -670,6 +670,39 @@ int fpt_hip_plan_set_proposer(FptHipPlan *plan, const float *params, size_t para
     return 0;
 }
 
+int fpt_hip_plan_voxel_stats(FptHipPlan *plan, float *device_accum, FptVoxelTraceStats *stats, char *error,
+                             size_t error_len) {
+    if (!plan || !device_accum || !stats) {
+        set_error(error, error_len, "missing plan, accumulation buffer or stats");
+        return 1;
+    }
+    if (plan->host.renderer_backend != FPT_RENDERER_VOXEL) {
+        set_error(error, error_len, "the voxel trace counters apply to the voxel backend");
+        return 1;
+    }
+    std::memset(stats, 0, sizeof(*stats));
+    Stream st;
+    HIP_TRY(hipStreamCreate(&st.s), "hipStreamCreate");
+    DevBuf counts;
+    HIP_TRY(counts.alloc(3 * sizeof(unsigned long long)), "counter buffer");
+    HIP_TRY(hipMemsetAsync(counts.p, 0, 3 * sizeof(unsigned long long), st.s), "counter clear");
+    const FptRenderConfig &c = plan->host;
+    const uint32_t spp = clamp_samples(c.samples), chunk = chunk_for(c);
+    float4 *acc = reinterpret_cast<float4 *>(device_accum);
+    for (uint32_t s0 = 0; s0 < spp; s0 += chunk)
+        HIP_TRY(fpt::launch_voxel_accumulate(plan->dcfg, c, plan->voxels, acc, s0, chunk, st.s,
+                                             counts.as<unsigned long long>()),
+                "voxel counting launch");
+    unsigned long long h[3] = {0, 0, 0};
+    HIP_TRY(hipMemcpyAsync(h, counts.p, sizeof(h), hipMemcpyDeviceToHost, st.s), "counter readback");
+    HIP_TRY(hipStreamSynchronize(st.s), "voxel counting pass");
+    stats->visited_cells = h[0];
+    stats->page_reads = h[1];
+    stats->cell_reads = h[2];
+    stats->pixels = uint64_t(c.width) * c.height;
+    return 0;
+}
+
 int fpt_hip_plan_profile(FptHipPlan *plan, float *device_accum, FptSdfProfileStats *stats,
                          FptProposerStats *learned, char *error, size_t error_len) {
     if (!plan || !device_accum || !stats) {

@@ -29,7 +29,7 @@ int voxel_build(const FptRenderConfig &cfg, const FptRenderConfig *dcfg, VoxelSt
 void voxel_free(VoxelStore &store);
 hipError_t launch_voxel_accumulate(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, VoxelStore &store,
                                    float4 *accum, uint32_t start, uint32_t count, hipStream_t stream,
-                                   unsigned long long *dda_steps);
+                                   unsigned long long *dda);  // optional {visited, page reads, payload reads}
 hipError_t launch_voxel_diagnostic(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, VoxelStore &store,
                                    const FptDiagnosticConfig &dg, uchar4 *out, hipStream_t stream);
 hipError_t launch_voxel_trace_cells(const FptRenderConfig *dcfg, const VoxelStore &store, const float *rays, size_t n,

@@ -253,6 +253,11 @@ class FptProposerStats(C.Structure):
 PROPOSER_PARAM_COUNT = 20801
 
 
+class FptVoxelTraceStats(C.Structure):
+    _fields_ = [("visited_cells", C.c_uint64), ("page_reads", C.c_uint64), ("cell_reads", C.c_uint64),
+                ("pixels", C.c_uint64)]
+
+
 class FptSpecializationStats(C.Structure):
     _fields_ = [("cache_status", u32), ("instruction_count", u32), ("source_bytes", C.c_uint64),
                 ("code_bytes", C.c_uint64), ("sample_count", C.c_uint64), ("distance_failures", C.c_uint64),
@@ -267,7 +272,7 @@ EXPECTED_SIZES = {
     FptRenderConfig: 30448, FptSdfProfileStats: 288, FptDiagnosticConfig: 24,
     FptMandelbulberFieldSample: 16, FptAsyncJitStats: 40, FptRayHit: 40, FptVoxelBuildInfo: 40,
     FptTuneResult: 24, FptProposerSettings: 16, FptProposerStats: 32, FptSpecializationStats: 80,
-    FptSceneOptions: 56,
+    FptSceneOptions: 56, FptVoxelTraceStats: 32,
 }
 
 PKG_DIR = Path(__file__).resolve().parent
@@ -285,6 +290,7 @@ EXPORTED_SYMBOLS = (
     "fpt_hip_autotune", "fpt_hip_device_alloc", "fpt_hip_device_free", "fpt_hip_copy", "fpt_hip_fill_f32",
     "fpt_hip_stream_create", "fpt_hip_stream_destroy", "fpt_hip_synchronize", "fpt_hip_time_renders",
     "fpt_hip_proposer_samples", "fpt_hip_proposer_eval", "fpt_hip_plan_set_proposer", "fpt_hip_plan_profile",
+    "fpt_hip_plan_voxel_stats",
     "fpt_hip_diagnostic_render", "fpt_hip_plan_specialize", "fpt_hip_topology_source", "fpt_hip_hydrate_hdri",
     "fpt_scene_default_config", "fpt_scene_load", "fpt_scene_load_fpt_settings", "fpt_scene_apply_program_json",
     "fpt_scene_optimize_program", "fpt_scene_split_program", "fpt_scene_compile_canonical",
@@ -394,6 +400,8 @@ def _declare(lib):
     lib.fpt_hip_plan_profile.argtypes = [C.c_void_p, C.c_void_p, P(FptSdfProfileStats), P(FptProposerStats), cp,
                                          C.c_size_t]
     lib.fpt_hip_plan_profile.restype = C.c_int
+    lib.fpt_hip_plan_voxel_stats.argtypes = [C.c_void_p, C.c_void_p, P(FptVoxelTraceStats), cp, C.c_size_t]
+    lib.fpt_hip_plan_voxel_stats.restype = C.c_int
     lib.fpt_hip_diagnostic_render.argtypes = [P(FptRenderConfig), P(FptDiagnosticConfig), P(C.c_uint8), C.c_size_t,
                                               P(C.c_double), cp, C.c_size_t]
     lib.fpt_hip_diagnostic_render.restype = C.c_int

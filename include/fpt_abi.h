@@ -1078,6 +1078,29 @@ int fpt_hip_plan_profile(FptHipPlan *plan,
                          char *error,
                          size_t error_len);
 
+/* Memory work of the voxel trace over a plan's full frame: every traversal
+ * (camera, bounce and shadow rays) of every sample.  No counterpart in the
+ * reference (its profile categories, Shaders.metal:7650-7704, are SDF-only);
+ * it gives the voxel DDA's algorithmic bytes: page_reads x 4 B (one
+ * page-table entry per brick entered) + cell_reads x the payload read per
+ * cell (1 B material id in sparse-bricks8, the 4 B packed colour of a
+ * record otherwise). */
+struct FptVoxelTraceStats {
+    uint64_t visited_cells;   /* DDA steps, hits and misses */
+    uint64_t page_reads;
+    uint64_t cell_reads;
+    uint64_t pixels;
+};
+FPT_ABI_ASSERT(sizeof(struct FptVoxelTraceStats) == 32, "FptVoxelTraceStats");
+
+/* Untimed counting pass of a voxel plan's full frame (same samples as a
+ * render; the image goes to device_accum). */
+int fpt_hip_plan_voxel_stats(FptHipPlan *plan,
+                             float *device_accum,
+                             struct FptVoxelTraceStats *stats,
+                             char *error,
+                             size_t error_len);
+
 /* Last error string of the calling thread (empty when none). */
 const char *fpt_hip_last_error(void);
 

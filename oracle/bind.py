@@ -73,6 +73,7 @@ def _voxel_decls(L):
     L.orc_voxel_trace_cells.argtypes = [C.c_void_p, P(C.c_float), C.c_size_t, C.c_uint32, P(C.c_int32),
                                         P(C.c_uint32), P(C.c_float)]
     L.orc_voxel_render.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, P(C.c_double)]
+    L.orc_last_voxel_counts.argtypes = [P(C.c_uint64)]
     L.orc_voxel_dda_contract.restype = C.c_uint32
     L.orc_f32_to_f16.argtypes = [C.c_float]
     L.orc_f32_to_f16.restype = C.c_uint16
@@ -126,6 +127,14 @@ def voxel_render(cfg, threads: int = 0, rows=None) -> np.ndarray:
     if lib().orc_voxel_render(C.addressof(cfg), threads, y0, y1, _dp(out)) != 0:
         raise RuntimeError("orc_voxel_render failed")
     return out
+
+
+def voxel_counts() -> dict:
+    """Memory work of the last voxel_render or voxel_trace_cells: cells
+    visited, page-table reads (once per brick entered), cell payload reads."""
+    out = (C.c_uint64 * 3)()
+    lib().orc_last_voxel_counts(out)
+    return {"visited_cells": out[0], "page_reads": out[1], "cell_reads": out[2]}
 
 
 def voxel_dda_contract() -> int:

@@ -171,3 +171,23 @@ def test_leaf_refinement_render_and_depth_match_oracle(mode):
     assert rmse <= 2e-2 and close >= 0.9
     assert changed > 0.01
     assert dclose >= 0.99
+
+
+@pytest.mark.parametrize("storage", [ffi.VOXEL_STORAGE_SPARSE_BRICKS, ffi.VOXEL_STORAGE_SPARSE_BRICKS8])
+def test_trace_memory_counts_match_oracle(storage):
+    """fpt_hip_plan_voxel_stats against the FP32 restatement's counts of the
+    same frame: within 1% (the camera rays' last-ulp differences move a few
+    traversals, as in the image tolerance above)."""
+    cfg = _cfg(storage, 3, 64)
+    cfg.renderer_backend = ffi.RENDERER_VOXEL
+    with api.Plan(cfg) as plan:
+        buf = api.DeviceBuffer(cfg.width * cfg.height * 4)
+        st = plan.voxel_stats(buf)
+        img = buf.numpy().reshape(cfg.height, cfg.width, 4)
+    ref = bind.voxel_render(cfg)
+    k = bind.voxel_counts()
+    print(storage, st, k)
+    assert st["pixels"] == cfg.width * cfg.height
+    assert float(np.sqrt(np.mean((img[..., :3] - ref[..., :3]) ** 2))) <= 2e-2
+    for key in ("visited_cells", "page_reads", "cell_reads"):
+        assert k[key] > 0 and abs(st[key] - k[key]) <= 0.01 * k[key], (key, st[key], k[key])

@@ -114,3 +114,38 @@ def test_dda_visits_are_face_or_tie_connected_and_monotone():
         for a in range(3):
             s = steps[:, a]
             assert not (np.any(s > 0) and np.any(s < 0))
+
+
+@pytest.mark.parametrize("storage,B", [(ffi.VOXEL_STORAGE_SPARSE_BRICKS, 4), (ffi.VOXEL_STORAGE_SPARSE_BRICKS8, 8),
+                                       (ffi.VOXEL_STORAGE_DENSE, 0)])
+def test_memory_work_counts_follow_the_visit_lists(storage, B):
+    """The counting pass's definition of the DDA's memory work, recomputed in
+    numpy from each ray's visit list and the built page table: every visited
+    cell counts once; a page-table entry is read whenever the visited brick
+    differs from the previous one; a cell payload is read for every visited
+    cell in an occupied page (every cell of a dense grid)."""
+    cfg = _small(storage, 3 if storage != ffi.VOXEL_STORAGE_DENSE else 0, 40)
+    rng = np.random.default_rng(9)
+    o = rng.uniform(-9, 9, (300, 3))
+    d = rng.normal(0, 1, (300, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    cells, counts, _ = bind.voxel_trace_cells(cfg, np.concatenate([o, d], 1), cap=512)
+    got = bind.voxel_counts()
+    v = bind.voxel_build(cfg)
+    visited = pages = payload = 0
+    for i in range(len(o)):
+        k = int(counts[i])
+        assert k <= 512
+        visited += k
+        if k == 0:
+            continue
+        if storage == ffi.VOXEL_STORAGE_DENSE:
+            payload += k
+            continue
+        G = v["bricks_per_axis"]
+        b = cells[i, :k] // B
+        bid = b[:, 0] + G * (b[:, 1] + G * b[:, 2])
+        pages += 1 + int(np.count_nonzero(np.diff(bid)))
+        payload += int(np.count_nonzero(v["page_table"][bid]))
+    assert visited > 100 and (storage == ffi.VOXEL_STORAGE_DENSE or 0 < payload < visited)
+    assert got == {"visited_cells": visited, "page_reads": pages, "cell_reads": payload}
