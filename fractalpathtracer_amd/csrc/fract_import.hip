@@ -13,6 +13,7 @@
 #pragma clang fp contract(off)
 
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>

@@ -11,12 +11,14 @@
 // where the reference does.
 #pragma clang fp contract(off)
 
+#include <cctype>
 #include <cerrno>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <fstream>
 #include <limits>
 #include <memory>
@@ -1343,7 +1345,12 @@ int fpt_scene_load(const char *scene_path, const char *fpt_root, const FptSceneO
     if (options) opt = *options;
     FptRenderConfig c;
     std::string name;
-    if (!fpt::scene_load(scene_path, fpt_root ? fpt_root : ".", opt, c, name, error, error_len)) return 1;
+    try {  // no C++ exception (allocation failure) may cross the C ABI
+        if (!fpt::scene_load(scene_path, fpt_root ? fpt_root : ".", opt, c, name, error, error_len)) return 1;
+    } catch (const std::exception &e) {
+        fpt::seterr(error, error_len, "load scene %s: %s", scene_path, e.what());
+        return 1;
+    }
     *config = c;
     if (output_name && output_name_len) std::snprintf(output_name, output_name_len, "%s", name.c_str());
     return 0;
