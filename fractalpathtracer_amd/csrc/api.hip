@@ -113,8 +113,8 @@ int validate_config(const FptRenderConfig *config, char *error, size_t error_len
 }
 
 // Imported Mandelbulber scenes (the native KIFS field) render on the
-// one-path-per-lane kernel; the entry points built on the persistent kernel,
-// the typed-program tools and the diagnostic images refuse them.
+// one-path-per-lane kernel; the entry points built on the persistent kernel
+// and the typed-program tools refuse them.
 int reject_mandelbulber(const FptRenderConfig *config, const char *what, char *error, size_t error_len) {
     if (!config || config->sdf_id != FPT_SDF_MANDELBULBER) return 0;
     set_error(error, error_len, "%s does not support imported Mandelbulber scenes", what);
@@ -1496,7 +1496,6 @@ int diagnostic_image(const FptRenderConfig *config, const FptDiagnosticConfig *d
         return 1;
     }
     if (validate_config(config, error, error_len)) return 1;
-    if (reject_mandelbulber(config, "the diagnostic renderer", error, error_len)) return 1;
     if (diagnostic->mode > FPT_DIAGNOSTIC_SDF_BOUNCE_CONTRIBUTION) {
         set_error(error, error_len, "unknown diagnostic mode %u", diagnostic->mode);
         return 1;

@@ -1318,8 +1318,10 @@ __device__ __forceinline__ bool same3(f3 a, f3 b) { return a.x == b.x && a.y == 
 // marchMandelbulber: march until d < threshold (the threshold grows with
 // camera distance), stopping where FP32 no longer moves the point, then up
 // to 30 half-steps bracketing the band (threshold * (1 - 0.001 detail),
-// threshold).  steps counts the march iterations, not the refinement.
-__device__ f3 march_mandelbulber(f3 dr, f3 rp, int ni, const FptRenderConfig &cfg, Work &w, int &steps) {
+// threshold).  steps counts the march iterations, not the refinement;
+// found (optional) reports whether the band was reached.
+__device__ f3 march_mandelbulber(f3 dr, f3 rp, int ni, const FptRenderConfig &cfg, Work &w, int &steps,
+                                 bool *found_out = nullptr) {
     const f3 start = rp;
     float d = 0.0f, thr = mb_threshold(rp, cfg), step = 0.0f;
     bool found = false;
@@ -1340,6 +1342,7 @@ __device__ f3 march_mandelbulber(f3 dr, f3 rp, int ni, const FptRenderConfig &cf
         rp = nx;
         if (length(rp - start) >= cfg.render[4]) break;
     }
+    if (found_out) *found_out = found;
     if (!found) return rp;
     const float limit = 1.0f - 0.001f * fmaxf(cfg.vset_values[129], 0.0f);
     step *= 0.5f;
@@ -1356,6 +1359,34 @@ __device__ f3 march_mandelbulber(f3 dr, f3 rp, int ni, const FptRenderConfig &cf
         }
         d = map_kifs(rp, cfg, w);
         step *= 0.5f;
+    }
+    return rp;
+}
+
+// marchCounted / marchStepCount for Mandelbulber scenes (SM:6857-6908), the
+// march of the path-cost diagnostics: the generic loop with the Mandelbulber
+// threshold, step and convergence test (d below the LOD-blended threshold),
+// no bracketing refinement, at most 10000 steps.
+__device__ f3 march_counted_mandelbulber(f3 dr, f3 rp, int ni, float lod_falloff, const FptRenderConfig &cfg,
+                                         Work &w, int &steps) {
+    const f3 cam = rp;
+    const int max_iter = ni < 10000 ? ni : 10000;
+    steps = 0;
+    for (int i = 0; i < max_iter; ++i) {
+        ++steps;
+        const float d = map_kifs(rp, cfg, w);
+        if (!isfinite(d)) break;
+        const float thr = mb_threshold(rp, cfg);
+        const float o = mb_step(d, thr, cfg);
+        rp = rp + dr * o;
+        float lod = thr;
+        if (lod_falloff > 0.00001f) {
+            const f3 df = cam - rp;
+            const float l2 = mixf(lod, 0.1f, dot(df, df) * lod_falloff);
+            lod = mixf(thr, l2, cfg.render[5]);
+        }
+        if (d < lod) break;
+        if (cfg.render[4] < o) break;
     }
     return rp;
 }
@@ -1419,6 +1450,13 @@ __device__ __forceinline__ bool is_mandelbulber(const FptRenderConfig &cfg) {
 __device__ __forceinline__ f3 march_any(f3 dr, f3 rp, int ni, float min_dist, float lod_falloff,
                                         const FptRenderConfig &cfg, Work &w, int &steps) {  // SM:3753-3760
     if (is_mandelbulber(cfg)) return march_mandelbulber(dr, rp, ni, cfg, w, steps);
+    return march(dr, rp, ni, min_dist, lod_falloff, cfg, w, steps);
+}
+// The path-cost diagnostics' march (SM:6857-6908): the scene's march,
+// except that Mandelbulber scenes use the generic loop with their step rules.
+__device__ __forceinline__ f3 march_counted_any(f3 dr, f3 rp, int ni, float min_dist, float lod_falloff,
+                                                const FptRenderConfig &cfg, Work &w, int &steps) {
+    if (is_mandelbulber(cfg)) return march_counted_mandelbulber(dr, rp, ni, lod_falloff, cfg, w, steps);
     return march(dr, rp, ni, min_dist, lod_falloff, cfg, w, steps);
 }
 __device__ __forceinline__ f3 normal_any(f3 p, const FptRenderConfig &cfg, Work &w) {

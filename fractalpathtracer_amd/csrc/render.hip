@@ -326,15 +326,15 @@ __device__ f3 path_cost_diagnostic(UV xy, const FptRenderConfig &cfg, const FptD
         int local_ni = int(cfg.render[1]);
         for (int i = 0; i < bounces; ++i) {
             int steps = 0;
-            rp = march(dr, rp, local_ni, cfg.render[3], 0.0002f, cfg, w, steps);
+            rp = march_counted_any(dr, rp, local_ni, cfg.render[3], 0.0002f, cfg, w, steps);
             t_primary += float(steps);
             local_ni = int(cfg.render[1] / (cfg.render[5] * 2.0f + 1.0f));
             const f3 tv = rp - cam;
             if (dot(tv, tv) > far2) break;
-            const Material mat = user_material(rp, cfg, w);
+            const Material mat = material_any(rp, cfg, w);
             t_bounces += 1.0f;
             t_normal += 1.0f;
-            const f3 n = normal_at(rp, cfg, w);
+            const f3 n = normal_any(rp, cfg, w);
             if (cfg.sun[0] == 1.0f) {  // sunStepCountWithNormal (SM:6910-6917)
                 f3 ld = rotate_camera(mk(0.f, 0.f, 1.f), cfg.sun[1] * kPi / 180.0f, cfg.sun[2] * kPi / 180.0f);
                 const float h1 = hash13(xy.x, xy.y, madd_rn(frame, 5.0f, 1.0f));
@@ -344,7 +344,8 @@ __device__ f3 path_cost_diagnostic(UV xy, const FptRenderConfig &cfg, const FptD
                 const float m = sqrtf(h2) * cfg.sun[4];
                 ld = rotate_camera(ld, cs * m, sn * m);
                 int sst = 0;
-                (void)march(ld, rp + n * 0.001f, int(cfg.render[1]), cfg.render[3], 0.0002f, cfg, w, sst);
+                (void)march_counted_any(ld, rp + n * 0.001f, int(cfg.render[1]), cfg.render[3], 0.0002f, cfg, w,
+                                        sst);
                 t_shadow += float(sst);
             }
             const float fi = float(i);
@@ -398,15 +399,15 @@ __device__ f3 bounce_contribution_diagnostic(UV xy, const FptRenderConfig &cfg, 
         f3 thr = mk(1.f, 1.f, 1.f);
         for (int i = 0; i < bounces; ++i) {
             int steps = 0;
-            rp = march(dr, rp, local_ni, cfg.render[3], 0.0002f, cfg, w, steps);
+            rp = march_any(dr, rp, local_ni, cfg.render[3], 0.0002f, cfg, w, steps);
             const f3 tv = rp - cam;
             local_ni = int(cfg.render[1] / (cfg.render[5] * 2.0f + 1.0f));
             if (dot(tv, tv) > far2) {
                 if (uint32_t(i) == target) total = total + environment(dr, cfg) * thr;
                 break;
             }
-            const Material mat = user_material(rp, cfg, w);
-            const f3 n = normal_at(rp, cfg, w);
+            const Material mat = material_any(rp, cfg, w);
+            const f3 n = normal_any(rp, cfg, w);
             f3 contrib = mk(0.f, 0.f, 0.f);
             if (mat.emission > 0.001f) contrib = contrib + mat.rgb * mat.emission;
             if (cfg.sun[0] == 1.0f) contrib = contrib + sun_contribution(rp, xy, frame, mat, n, cfg, w, unused);
@@ -454,31 +455,36 @@ __device__ f3 sdf_diagnostic(UV xy, const FptRenderConfig &cfg, const FptDiagnos
         for (uint32_t i = 0; i < samples; ++i) c = c + render_path(xy, i, cfg, w, pc);
         return post_process(c / float(samples), cfg);
     }
-    // Un-jittered camera ray with the diagnostic's own march (SM:7304-7339).
+    // Un-jittered camera ray with the diagnostic's own march (SM:7304-7339);
+    // Mandelbulber scenes use their camera and marchMandelbulber.
     const f3 ro = ld3(cfg.camera_position);
-    const float focal = 1.0f / tanf(cfg.camera_fov / 2.0f * kPi / 180.0f);
-    const f3 rd = rotate_camera_roll(normalize(mk(xy.x, xy.y, focal)), cfg.camera_yaw_pitch[0],
-                                     cfg.camera_yaw_pitch[1], cfg.camera_roll);
+    f3 rd;
+    if (!camera_ray_any(xy.x, xy.y, cfg, rd)) return mk(0.f, 0.f, 0.f);
     const int requested = int(fmaxf(cfg.render[1], 32.0f));
-    const int steps = requested < 420 ? requested : 420;
-    const float thr = fmaxf(cfg.render[3], 0.0005f);
     f3 rp = ro;
     bool hit = false;
-    for (int i = 0; i < steps; ++i) {
-        const float d = map_sdf(rp, cfg, w);
-        if (!isfinite(d)) break;
-        if (d < thr && length(rp - ro) > 0.001f) {
-            hit = true;
-            break;
+    if (is_mandelbulber(cfg)) {
+        int unused = 0;
+        rp = march_mandelbulber(rd, ro, requested < 10000 ? requested : 10000, cfg, w, unused, &hit);
+    } else {
+        const int steps = requested < 420 ? requested : 420;
+        const float thr = fmaxf(cfg.render[3], 0.0005f);
+        for (int i = 0; i < steps; ++i) {
+            const float d = map_sdf(rp, cfg, w);
+            if (!isfinite(d)) break;
+            if (d < thr && length(rp - ro) > 0.001f) {
+                hit = true;
+                break;
+            }
+            rp = rp + rd * (fabsf(d) * 0.99f);
+            if (length(rp - ro) > cfg.render[4]) break;
         }
-        rp = rp + rd * (fabsf(d) * 0.99f);
-        if (length(rp - ro) > cfg.render[4]) break;
     }
     if (mode == FPT_DIAGNOSTIC_HIT_MASK) return hit ? mk(1.f, 1.f, 1.f) : mk(0.f, 0.f, 0.f);
     if (mode == FPT_DIAGNOSTIC_DEPTH) return diag_depth(length(rp - ro), hit, cfg, dg);
     if (!hit) return mk(0.f, 0.f, 0.f);
     if (mode == FPT_DIAGNOSTIC_DIFFUSE_NORMAL) {
-        const f3 n = normal_at(rp, cfg, w);
+        const f3 n = normal_any(rp, cfg, w);
         const f3 fwd = rotate_camera_roll(mk(0.f, 0.f, 1.f), cfg.camera_yaw_pitch[0], cfg.camera_yaw_pitch[1],
                                           cfg.camera_roll);
         const float k = 0.12f + 0.88f * fmaxf(dot(n, -fwd), 0.0f);
@@ -488,26 +494,26 @@ __device__ f3 sdf_diagnostic(UV xy, const FptRenderConfig &cfg, const FptDiagnos
     // Mandelbulber material code; without it the reference returns black.
     if (mode == FPT_DIAGNOSTIC_MANDEL_COLOR_INDEX || mode == FPT_DIAGNOSTIC_MANDEL_PALETTE_POSITION)
         return mk(0.f, 0.f, 0.f);
-    const Material mat = user_material(rp, cfg, w);
+    const Material mat = material_any(rp, cfg, w);
     if (mode == FPT_DIAGNOSTIC_MATERIAL)
         return mk(clampf(mat.rgb.x, 0.f, 1.f), clampf(mat.rgb.y, 0.f, 1.f), clampf(mat.rgb.z, 0.f, 1.f));
     if (mode == FPT_DIAGNOSTIC_PATH_DIRECT) {
         f3 c = mk(0.f, 0.f, 0.f);
         if (cfg.sun[0] == 1.0f) {  // sunContribution (SM:5863-5867)
             int unused = 0;
-            c = sun_contribution(rp, xy, 0.0f, user_material(rp, cfg, w), normal_at(rp, cfg, w), cfg, w, unused);
+            c = sun_contribution(rp, xy, 0.0f, material_any(rp, cfg, w), normal_any(rp, cfg, w), cfg, w, unused);
         }
         return diag_hdr(c, cfg);
     }
     if (mode == FPT_DIAGNOSTIC_PATH_ENVIRONMENT) return diag_hdr(environment(rd, cfg), cfg);
     if (mode == FPT_DIAGNOSTIC_PATH_THROUGHPUT) {
-        const f3 n = normal_at(rp, cfg, w);
+        const f3 n = normal_any(rp, cfg, w);
         const f3 next = normalize(vmix(reflect(rd, n), random_vector(n, xy, 0.0f), mat.roughness));
         const float al = clampf(dot(next, n) * 0.5f + 0.5f, 0.f, 1.f);
         const f3 c = mat.rgb * al;
         return mk(clampf(c.x, 0.f, 1.f), clampf(c.y, 0.f, 1.f), clampf(c.z, 0.f, 1.f));
     }
-    return diag_normal(normal_at(rp, cfg, w));
+    return diag_normal(normal_any(rp, cfg, w));
 }
 
 __device__ __forceinline__ uchar4 encode_rgba8(f3 c) {
@@ -523,7 +529,7 @@ __global__ __launch_bounds__(256) void sdf_diagnostic_kernel(const FptRenderConf
     tile_pixel(x, y);
     if (x >= cfg.width || y >= cfg.height) return;
     Work w = {0, 0};
-    const f3 c = sdf_diagnostic(screen_uv(x, y, cfg.width, cfg.height), cfg, dg, w);
+    const f3 c = sdf_diagnostic(screen_uv_any(x, y, cfg), cfg, dg, w);
     out[size_t(cfg.height - 1u - y) * cfg.width + x] = encode_rgba8(c);
 }
 

@@ -988,7 +988,8 @@ template <class T>
 bool same_point(V3<T> a, V3<T> b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
 
 template <class T>
-V3<T> march_mandelbulber(V3<T> dr, V3<T> rp, int ni, const FptRenderConfig &cfg, int *steps_out) {
+V3<T> march_mandelbulber(V3<T> dr, V3<T> rp, int ni, const FptRenderConfig &cfg, int *steps_out,
+                         bool *found_out = nullptr) {
     const V3<T> start = rp;
     T d = T(0), thr = mb_threshold(rp, cfg), step = T(0);
     bool found = false;
@@ -1010,6 +1011,7 @@ V3<T> march_mandelbulber(V3<T> dr, V3<T> rp, int ni, const FptRenderConfig &cfg,
         if (length(rp - start) >= T(cfg.render[4])) break;
     }
     if (steps_out) *steps_out = steps;
+    if (found_out) *found_out = found;
     if (!found) return rp;
     const T limit = T(1) - T(0.001f) * m_max(T(cfg.vset_values[129]), T(0));
     step *= T(0.5f);

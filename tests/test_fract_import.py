@@ -178,8 +178,8 @@ def test_options_and_errors(tmp_path):
 
 def test_imported_scene_refusals_on_the_host():
     """Imported Mandelbulber scenes render on the SDF backend's
-    one-path-per-lane kernel; the voxel backend, batches, learned stepping
-    and diagnostic images refuse them before any device work."""
+    one-path-per-lane kernel; the voxel backend, batches and learned
+    stepping refuse them before any device work."""
     from fractalpathtracer_amd import api
     c, _ = scenes.load_scene(IFS20)
     c.width, c.height, c.samples = 8, 8, 1
@@ -187,8 +187,6 @@ def test_imported_scene_refusals_on_the_host():
         api.Batch([c])
     with pytest.raises(RuntimeError, match="learned stepping"):
         api.proposer_samples(c, 16)
-    with pytest.raises(RuntimeError, match="diagnostic renderer"):
-        api.diagnostic(c, 1)
     v = scenes.load_scene(IFS20)[0]
     v.renderer_backend = ffi.RENDERER_VOXEL
     with pytest.raises(RuntimeError, match="SDF backend"):

@@ -118,3 +118,59 @@ def test_present_png_matches_oracle_present(tmp_path):
         c.post[0] = tm
         res = api.render_full(c, output_png=str(tmp_path / "q.png"))
         assert _close(api.read_png(str(tmp_path / "q.png")), bind.present(c, res.linear.astype(np.float32))) >= 0.999
+
+
+def _kifs_sphere():
+    """The closed-form KIFS sphere with a fixed threshold of 1e-3 (normal
+    step 0.1): the path tracer's 0.001 surface offset then lifts secondary
+    rays clear of the hit band, so paths are well conditioned."""
+    from kifs_sphere import sphere_cfg
+    c = sphere_cfg(mode=0, thr=1e-3)
+    c.vset_values[114] = 100.0
+    c.width, c.height, c.samples = 48, 32, 2
+    c.camera_fov = 20.0
+    c.world[0], c.world[4] = 3.0, 0.3
+    c.sun[:] = [1.0, 135.0, 35.26, 1.0, 0.0]
+    return c
+
+
+def _ifs20():
+    c, _ = scenes.load_scene(GOLDEN / "mandelbulber" / "ifs-20.fract")
+    c.width, c.height, c.samples = 48, 27, 2
+    return c
+
+
+@pytest.mark.parametrize("mode", SINGLE_RAY + PATH_MODES)
+def test_sdf_diagnostic_kifs_sphere_matches_oracle(mode):
+    """Imported-scene diagnostics (Mandelbulber camera, march, normals) on
+    the KIFS whose field is an exact sphere distance.  Single-ray modes: the
+    smooth-scene bar.  Path modes: the FP32 twin's bar, since a few paths
+    graze the silhouette (the twin itself agrees with FP64 on 98% of
+    path_final values)."""
+    c = _kifs_sphere()
+    img, _ = api.diagnostic(c, mode, max_distance=2000.0, bounce=1)
+    ref = bind.diagnostic(c, MODES[mode], bounce=1, max_distance=2000.0)
+    twin = bind.diagnostic(c, MODES[mode], bounce=1, max_distance=2000.0, precision=32)
+    frac = _close(img, ref)
+    mad = float(np.mean(np.abs(img.astype(int) - ref.astype(int))))
+    mad_twin = float(np.mean(np.abs(twin.astype(int) - ref.astype(int))))
+    print(mode, f"within 1: {frac:.4f} twin {_close(twin, ref):.4f}; mean |d| {mad:.3f} twin {mad_twin:.3f}")
+    assert np.all(img[..., 3] == 255)
+    if mode in SINGLE_RAY:
+        assert frac >= 0.995
+    else:
+        assert mad <= 1.25 * mad_twin + 0.5
+
+
+@pytest.mark.parametrize("mode", SINGLE_RAY + PATH_MODES)
+def test_sdf_diagnostic_ifs20_matches_oracle(mode):
+    """The imported ifs-20 scene: the fractal bar (mean 8-bit error within
+    1.25x the FP32 twin's + 0.5 steps)."""
+    c = _ifs20()
+    img, _ = api.diagnostic(c, mode, bounce=0)
+    ref = bind.diagnostic(c, MODES[mode], bounce=0)
+    twin = bind.diagnostic(c, MODES[mode], bounce=0, precision=32)
+    mad = float(np.mean(np.abs(img.astype(int) - ref.astype(int))))
+    mad_twin = float(np.mean(np.abs(twin.astype(int) - ref.astype(int))))
+    print(mode, f"within 1: {_close(img, ref):.4f} twin {_close(twin, ref):.4f}; mean |d| {mad:.3f} twin {mad_twin:.3f}")
+    assert mad <= 1.25 * mad_twin + 0.5

@@ -220,3 +220,30 @@ def test_focus_estimate_is_the_sphere_hit():
     c = sphere_cfg()
     f = bind.focus_estimate(c)
     assert 1000 - R - 2.0 < f < 1000 - R - 2.0 * 0.998
+
+
+def test_diagnostic_hit_mask_and_depth_on_the_sphere():
+    """Single-ray diagnostics of a Mandelbulber scene (SM:7304-7339): the
+    Mandelbulber camera and march.  A pixel is a hit when its ray passes the
+    sphere (plus the threshold band) and a miss when it passes clear of it;
+    the depth is 1 - t / max_distance with t the distance to the band."""
+    c = sphere_cfg(mode=0, thr=2.0)
+    c.width, c.height, c.camera_fov = 48, 32, 20.0
+    md = 2000.0
+    hm = bind.diagnostic(c, ffi.DIAGNOSTIC_MODES["hit_mask"], max_distance=md)[::-1, :, 0]  # PNG rows are flipped
+    dp = bind.diagnostic(c, ffi.DIAGNOSTIC_MODES["depth"], max_distance=md)[::-1, :, 0]
+    px = [[x, y] for y in range(32) for x in range(48)]
+    d = bind.camera_ray(c, px)[:, 2:5]
+    cam = np.array([0.0, 0.0, -1000.0])
+    along = -(d @ cam)  # distance along each ray to its closest approach to the origin
+    miss_dist = np.linalg.norm(cam + d * along[:, None], axis=1)
+    inside, outside = miss_dist < R + 2.0 * LIMIT - 1e-6, miss_dist > R + 2.0 + 1e-6
+    assert inside.sum() > 100 and outside.sum() > 100
+    hm = hm.reshape(-1)
+    assert np.all(hm[inside] == 255) and np.all(hm[outside] == 0)
+    # entry distance into the sphere of radius R + h for h in the band
+    def t_at(radius):
+        return along[inside] - np.sqrt(np.maximum(radius ** 2 - miss_dist[inside] ** 2, 0.0))
+    lo, hi = t_at(R + 2.0), t_at(R + 2.0 * LIMIT)
+    v = dp.reshape(-1)[inside].astype(float)
+    assert np.all(v >= np.floor((1 - hi / md) * 255) - 1) and np.all(v <= np.ceil((1 - lo / md) * 255) + 1)
