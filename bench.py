@@ -105,9 +105,11 @@ def parse():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=8)
     ap.add_argument("--ref-spp-factor", type=int, default=4)
-    ap.add_argument("--workload", choices=("learned", "typed", "voxel"), default="learned",
+    ap.add_argument("--workload", choices=("learned", "typed", "voxel", "fract"), default="learned",
                     help="learned: the learned-step scene above (the default line); typed: the typed-program "
-                         "scene (1280x720, 16 spp); voxel: the 1024^3 sparse-brick Mandelbox (1920x1080, 4 spp)")
+                         "scene (1280x720, 16 spp); voxel: the 1024^3 sparse-brick Mandelbox (1920x1080, 4 spp); "
+                         "fract: the bundled Mandelbulber scene ifs-20.fract (Kaleidoscopic IFS, its own "
+                         "1920x1080, 64 spp)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy kernel of MIB MiB (the known-size reference for "
                          "HBM byte counters) and exit")
@@ -258,6 +260,21 @@ def main():
     print(json.dumps(line), flush=True)
 
 
+def one_path_launches(cfg) -> int:
+    """Launches of the one-path-per-lane kernel per frame: the library's
+    accumulation policy (one launch up to 16 spp in auto mode, else chunks of
+    sdf_chunk_samples, default 8)."""
+    from fractalpathtracer_amd import ffi
+    spp = min(max(cfg.samples, 1), 512)
+    mode = cfg.sdf_accumulation_mode
+    if mode == ffi.SDF_ACCUMULATION_PER_SAMPLE:
+        return spp
+    if mode == ffi.SDF_ACCUMULATION_BATCH or (mode == ffi.SDF_ACCUMULATION_AUTO and spp <= 16):
+        return 1
+    chunk = min(max(cfg.sdf_chunk_samples or 8, 1), 64)
+    return -(-spp // chunk)
+
+
 def run_workload(args, api, scenes):
     """One JSON line for the typed-program or the voxel workload, with the
     same contract keys as the default line (its own roofline)."""
@@ -266,6 +283,10 @@ def run_workload(args, api, scenes):
         cfg = scenes.typed_program_scene()
         desc = "typed program: Menger(5) box-folded and rotated 30 deg about Y, union Mandelbox(-1.5, 10) " \
                "offset (2,0,0), 1280x720, 16 spp, 4 bounces, Russian roulette after bounce 2"
+    elif args.workload == "fract":
+        cfg, _ = scenes.load_scene(ROOT / "tests" / "golden" / "mandelbulber" / "ifs-20.fract")
+        desc = "Mandelbulber ifs-20.fract: Kaleidoscopic IFS (250 iterations), Mandelbulber camera and march, " \
+               f"{cfg.width}x{cfg.height}, {cfg.samples} spp, {int(cfg.render[0])} bounces"
     else:
         cfg = scenes.voxel_mandelbox_scene()
         desc = "voxel: Mandelbox(2.0, 12) on a 1024^3 grid of sparse 8^3 bricks (shell coverage), " \
@@ -285,7 +306,22 @@ def run_workload(args, api, scenes):
             "dtype": "f32", "data": "synthetic",
             "config": {"workload": desc, "width": cfg.width, "height": cfg.height, "spp": cfg.samples,
                        "plan_create_s": build_s}}
-    if args.workload == "typed":
+    if args.workload == "fract":
+        prof, _ = plan.profile(accum)
+        # per KIFS iteration: 3 abs folds, 3 plane reflections (dot 5, test 1,
+        # reflect 7 each), rotation about an offset (3 + 9 + 3 + 3), scaling
+        # (6), derivative (1), |z| (6) = 73 FLOP with every plane enabled;
+        # per distance evaluation the finaliser (4), threshold (9) and step (8)
+        flops = prof["march_orbit_iterations"] * 73 + prof["distance_evals"] * 21
+        launches = one_path_launches(cfg)
+        achieved = flops / (ms / 1e3) / 1e12
+        line["roofline"] = {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                            "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                            "kernel": "sdf_accumulate_kernel", "launches_per_step": launches,
+                            "launch_ms": ms / launches, "flops_per_launch": flops / launches}
+        line["config"]["counts"] = {k: prof[k] for k in ("distance_evals", "march_orbit_iterations",
+                                                         "normal_evals", "primary_steps", "shadow_steps")}
+    elif args.workload == "typed":
         prof, _ = plan.profile(accum)
         flops = (prof["march_orbit_iterations"] * TYPED_FLOPS_PER_ORBIT_ITERATION +
                  prof["distance_evals"] * TYPED_FLOPS_PER_DISTANCE_EVAL)

@@ -1445,30 +1445,45 @@ __device__ __forceinline__ bool is_mandelbulber(const FptRenderConfig &cfg) {
     return cfg.sdf_id == FPT_SDF_MANDELBULBER;
 }
 
+// Scene kinds a kernel is built for: SCENE_ANY decides per config at run
+// time; SCENE_SDF (typed programs and built-in fractals) and SCENE_MB
+// (imported Mandelbulber scenes) leave the other kind's code out, so each
+// kernel instance carries only the registers its scenes need.
+enum : int { SCENE_ANY = 0, SCENE_SDF = 1, SCENE_MB = 2 };
+template <int K>
+__device__ __forceinline__ bool mb_scene(const FptRenderConfig &cfg) {
+    return K == SCENE_MB || (K == SCENE_ANY && is_mandelbulber(cfg));
+}
+
 // Entry points for every SDF scene: the Mandelbulber branches above for
 // imported scenes, the functions of the SDF path otherwise.
+template <int K = SCENE_ANY>
 __device__ __forceinline__ f3 march_any(f3 dr, f3 rp, int ni, float min_dist, float lod_falloff,
                                         const FptRenderConfig &cfg, Work &w, int &steps) {  // SM:3753-3760
-    if (is_mandelbulber(cfg)) return march_mandelbulber(dr, rp, ni, cfg, w, steps);
+    if (mb_scene<K>(cfg)) return march_mandelbulber(dr, rp, ni, cfg, w, steps);
     return march(dr, rp, ni, min_dist, lod_falloff, cfg, w, steps);
 }
 // The path-cost diagnostics' march (SM:6857-6908): the scene's march,
 // except that Mandelbulber scenes use the generic loop with their step rules.
+template <int K = SCENE_ANY>
 __device__ __forceinline__ f3 march_counted_any(f3 dr, f3 rp, int ni, float min_dist, float lod_falloff,
                                                 const FptRenderConfig &cfg, Work &w, int &steps) {
-    if (is_mandelbulber(cfg)) return march_counted_mandelbulber(dr, rp, ni, lod_falloff, cfg, w, steps);
+    if (mb_scene<K>(cfg)) return march_counted_mandelbulber(dr, rp, ni, lod_falloff, cfg, w, steps);
     return march(dr, rp, ni, min_dist, lod_falloff, cfg, w, steps);
 }
+template <int K = SCENE_ANY>
 __device__ __forceinline__ f3 normal_any(f3 p, const FptRenderConfig &cfg, Work &w) {
-    if (is_mandelbulber(cfg)) return normal_mandelbulber(p, cfg, w);
+    if (mb_scene<K>(cfg)) return normal_mandelbulber(p, cfg, w);
     return normal_at(p, cfg, w);
 }
+template <int K = SCENE_ANY>
 __device__ __forceinline__ float map_any(f3 p, const FptRenderConfig &cfg, Work &w) {
-    if (is_mandelbulber(cfg)) return map_kifs(p, cfg, w);
+    if (mb_scene<K>(cfg)) return map_kifs(p, cfg, w);
     return map_sdf(p, cfg, w);
 }
+template <int K = SCENE_ANY>
 __device__ __forceinline__ Material material_any(f3 p, const FptRenderConfig &cfg, Work &w) {  // SM:3539-3562
-    if (!is_mandelbulber(cfg)) return user_material(p, cfg, w);
+    if (!mb_scene<K>(cfg)) return user_material(p, cfg, w);
     float orbit = 0.0f;
     // a solid style does not read the orbit
     if (cfg.fractal_style_mode != 2u) de_kifs(p, cfg, &orbit, w);
@@ -1492,8 +1507,9 @@ __device__ float focus_estimate_any(const FptRenderConfig &cfg, Work &w) {
 
 // The camera ray through screen point (x, y) (SM:5938-5949); false when a
 // cut fish-eye does not see it.
+template <int K = SCENE_ANY>
 __device__ __forceinline__ bool camera_ray_any(float x, float y, const FptRenderConfig &cfg, f3 &dr) {
-    if (is_mandelbulber(cfg)) {
+    if (mb_scene<K>(cfg)) {
         if (!mb_projection_visible(x, y, cfg)) return false;
         dr = mb_camera_ray(x, y, cfg);
         return true;
@@ -1506,8 +1522,9 @@ __device__ __forceinline__ bool camera_ray_any(float x, float y, const FptRender
 
 // sdfScreenUv (SM:6847-6855): Mandelbulber scenes sample pixel corners
 // offset (0, 1), the others pixel centres.
+template <int K = SCENE_ANY>
 __device__ __forceinline__ UV screen_uv_any(uint32_t x, uint32_t y, const FptRenderConfig &cfg) {
-    if (!is_mandelbulber(cfg)) return screen_uv(x, y, cfg.width, cfg.height);
+    if (!mb_scene<K>(cfg)) return screen_uv(x, y, cfg.width, cfg.height);
     const float fw = float(cfg.width), fh = float(cfg.height);
     const float u = __fsub_rn(__fdiv_rn(float(x), fw), 0.5f);
     const float v = __fsub_rn(__fdiv_rn(__fadd_rn(float(y), 1.0f), fh), 0.5f);

@@ -14,7 +14,9 @@
 
 namespace fpt {
 
-// sunContributionWithSurface (SM:5846-5861).
+// sunContributionWithSurface (SM:5846-5861); K: the scene kinds this
+// instance serves (SCENE_ANY decides at run time).
+template <int K>
 __device__ f3 sun_contribution(f3 rp, UV xy, float seed, const Material &mat, f3 n, const FptRenderConfig &cfg,
                                Work &w, int &shadow_steps) {
     const float p1 = mat.roughness, p2 = 1.0f - mat.translucency;
@@ -28,7 +30,7 @@ __device__ f3 sun_contribution(f3 rp, UV xy, float seed, const Material &mat, f3
     float m = sqrtf(h2) * cfg.sun[4];
     ld = rotate_camera(ld, c * m, s * m);
     int steps = 0;
-    rp = march_any(ld, rp, int(cfg.render[1]), cfg.render[3], 0.0002f, cfg, w, steps);
+    rp = march_any<K>(ld, rp, int(cfg.render[1]), cfg.render[3], 0.0002f, cfg, w, steps);
     shadow_steps += steps;
     if (length(rp0 - rp) > cfg.render[4] * 0.99f) {
         float k = cfg.sun[3] * fmaxf(dot(n, ld), 0.0f) * p1 * p2;
@@ -41,7 +43,8 @@ struct PathCounts {
     int primary_steps, secondary_steps, shadow_steps, normal_evals, bounces;
 };
 
-// renderPath (SM:5926-6026).
+// renderPath (SM:5926-6026); K as for sun_contribution.
+template <int K>
 __device__ f3 render_path(UV xy, uint32_t sample_idx, const FptRenderConfig &cfg, Work &w, PathCounts &pc) {
     const float frame = float(sample_idx);
     const f3 cam = ld3(cfg.camera_position);
@@ -50,7 +53,7 @@ __device__ f3 render_path(UV xy, uint32_t sample_idx, const FptRenderConfig &cfg
     float jx, jy;
     random_point(aa, xy, frame, jx, jy);
     f3 dr;
-    if (!camera_ray_any(xy.x + jx, xy.y + jy, cfg, dr)) return mk(0.f, 0.f, 0.f);
+    if (!camera_ray_any<K>(xy.x + jx, xy.y + jy, cfg, dr)) return mk(0.f, 0.f, 0.f);
     apply_dof(xy, frame, cfg, rp, dr);
     int local_ni = int(cfg.render[1]);
     float side = 1.0f;
@@ -66,7 +69,7 @@ __device__ f3 render_path(UV xy, uint32_t sample_idx, const FptRenderConfig &cfg
     const float far2 = far * far;
     for (int i = 0; i < bounces; ++i) {
         int steps = 0;
-        rp = march_any(dr, rp, local_ni, cfg.render[3], 0.0002f, cfg, w, steps);
+        rp = march_any<K>(dr, rp, local_ni, cfg.render[3], 0.0002f, cfg, w, steps);
         if (i == 0) pc.primary_steps += steps; else pc.secondary_steps += steps;
         pc.bounces++;
         f3 tv = rp - cam;
@@ -77,11 +80,11 @@ __device__ f3 render_path(UV xy, uint32_t sample_idx, const FptRenderConfig &cfg
             light = light + environment(dr, cfg);
             break;
         }
-        Material mat = material_any(rp, cfg, w);
+        Material mat = material_any<K>(rp, cfg, w);
         if (mat.emission > 0.001f) light = light + mat.rgb * mat.emission;
-        f3 n = normal_any(rp, cfg, w);
+        f3 n = normal_any<K>(rp, cfg, w);
         pc.normal_evals++;
-        if (cfg.sun[0] == 1.0f) light = light + sun_contribution(rp, xy, frame, mat, n, cfg, w, pc.shadow_steps);
+        if (cfg.sun[0] == 1.0f) light = light + sun_contribution<K>(rp, xy, frame, mat, n, cfg, w, pc.shadow_steps);
         const float fi = float(i);
         float r1 = hash13(xy.x, xy.y, madd_rn(frame, 1.37f, fi));
         float r2 = hash13(xy.x, xy.y, madd_rn(frame, 7.91f, fi));
@@ -141,8 +144,20 @@ __device__ __forceinline__ void wave_add_u64(unsigned long long *dst, unsigned l
     if ((threadIdx.x & 63u) == 0u && v) atomicAdd(dst, v);
 }
 
-template <bool PROFILE>
-__global__ __launch_bounds__(256) void sdf_accumulate_kernel(const FptRenderConfig *__restrict__ cfgp,
+// Minimum resident waves per SIMD for the imported-Mandelbulber instance.
+// Its KIFS orbit reads the fold planes from the config with scalar loads,
+// so more resident waves hide them: measured on ifs-20.fract at 1080p,
+// 8 spp: 3 waves (134 VGPRs, no hint) 838 ms, 4 waves 691 ms, 5 waves (96
+// VGPRs, 128 B spilled per lane) 616 ms, identical images.
+#ifndef FPT_MANDELBULBER_WAVES
+#define FPT_MANDELBULBER_WAVES 5
+#endif
+
+// K: SCENE_SDF (typed programs and built-in fractals) or SCENE_MB
+// (imported Mandelbulber scenes), chosen per config at launch.
+template <bool PROFILE, int K>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K == SCENE_MB ? FPT_MANDELBULBER_WAVES : 1)))
+void sdf_accumulate_kernel(const FptRenderConfig *__restrict__ cfgp,
                                                              float4 *__restrict__ accum, uint32_t start,
                                                              uint32_t count, DeviceProfile *prof) {
     const FptRenderConfig &cfg = *cfgp;
@@ -152,7 +167,7 @@ __global__ __launch_bounds__(256) void sdf_accumulate_kernel(const FptRenderConf
     Work w = {0, 0};
     PathCounts pc = {0, 0, 0, 0, 0};
     if (active) {
-        const UV uv = screen_uv_any(x, y, cfg);
+        const UV uv = screen_uv_any<K>(x, y, cfg);
         const size_t idx = size_t(y) * cfg.width + x;
         f3 c = mk(0.f, 0.f, 0.f);
         if (start != 0u) {
@@ -162,7 +177,7 @@ __global__ __launch_bounds__(256) void sdf_accumulate_kernel(const FptRenderConf
         uint32_t total = cfg.samples < 1u ? 1u : (cfg.samples > 512u ? 512u : cfg.samples);
         uint32_t end = start + count < total ? start + count : total;
         for (uint32_t s = start; s < end; ++s) {
-            f3 sc = render_path(uv, s, cfg, w, pc);
+            f3 sc = render_path<K>(uv, s, cfg, w, pc);
             c = vmix(c, sc, 1.0f / float(s + 1u));
         }
         accum[idx] = make_float4(c.x, c.y, c.z, 1.0f);
@@ -410,7 +425,7 @@ __device__ f3 bounce_contribution_diagnostic(UV xy, const FptRenderConfig &cfg, 
             const f3 n = normal_any(rp, cfg, w);
             f3 contrib = mk(0.f, 0.f, 0.f);
             if (mat.emission > 0.001f) contrib = contrib + mat.rgb * mat.emission;
-            if (cfg.sun[0] == 1.0f) contrib = contrib + sun_contribution(rp, xy, frame, mat, n, cfg, w, unused);
+            if (cfg.sun[0] == 1.0f) contrib = contrib + sun_contribution<SCENE_ANY>(rp, xy, frame, mat, n, cfg, w, unused);
             if (uint32_t(i) == target) total = total + contrib * thr;
             const float fi = float(i);
             const float r1 = hash13(xy.x, xy.y, madd_rn(frame, 1.37f, fi));
@@ -452,7 +467,7 @@ __device__ f3 sdf_diagnostic(UV xy, const FptRenderConfig &cfg, const FptDiagnos
         const uint32_t samples = cfg.samples < 16u ? 16u : (cfg.samples > 64u ? 64u : cfg.samples);
         f3 c = mk(0.f, 0.f, 0.f);
         PathCounts pc = {0, 0, 0, 0, 0};
-        for (uint32_t i = 0; i < samples; ++i) c = c + render_path(xy, i, cfg, w, pc);
+        for (uint32_t i = 0; i < samples; ++i) c = c + render_path<SCENE_ANY>(xy, i, cfg, w, pc);
         return post_process(c / float(samples), cfg);
     }
     // Un-jittered camera ray with the diagnostic's own march (SM:7304-7339);
@@ -501,7 +516,8 @@ __device__ f3 sdf_diagnostic(UV xy, const FptRenderConfig &cfg, const FptDiagnos
         f3 c = mk(0.f, 0.f, 0.f);
         if (cfg.sun[0] == 1.0f) {  // sunContribution (SM:5863-5867)
             int unused = 0;
-            c = sun_contribution(rp, xy, 0.0f, material_any(rp, cfg, w), normal_any(rp, cfg, w), cfg, w, unused);
+            c = sun_contribution<SCENE_ANY>(rp, xy, 0.0f, material_any(rp, cfg, w), normal_any(rp, cfg, w), cfg, w,
+                                            unused);
         }
         return diag_hdr(c, cfg);
     }
@@ -543,11 +559,19 @@ hipError_t launch_sdf_diagnostic(const FptRenderConfig *dcfg, const FptRenderCon
 hipError_t launch_sdf_accumulate(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, float4 *accum,
                                  uint32_t start, uint32_t count, DeviceProfile *prof, hipStream_t stream) {
     dim3 grid = tile_grid(hcfg.width, hcfg.height);
-    if (prof)
-        hipLaunchKernelGGL(sdf_accumulate_kernel<true>, grid, dim3(256), 0, stream, dcfg, accum, start, count, prof);
+    const bool mb = hcfg.sdf_id == FPT_SDF_MANDELBULBER;
+    if (prof && mb)
+        hipLaunchKernelGGL((sdf_accumulate_kernel<true, SCENE_MB>), grid, dim3(256), 0, stream, dcfg, accum, start,
+                           count, prof);
+    else if (prof)
+        hipLaunchKernelGGL((sdf_accumulate_kernel<true, SCENE_SDF>), grid, dim3(256), 0, stream, dcfg, accum, start,
+                           count, prof);
+    else if (mb)
+        hipLaunchKernelGGL((sdf_accumulate_kernel<false, SCENE_MB>), grid, dim3(256), 0, stream, dcfg, accum, start,
+                           count, nullptr);
     else
-        hipLaunchKernelGGL(sdf_accumulate_kernel<false>, grid, dim3(256), 0, stream, dcfg, accum, start, count,
-                           nullptr);
+        hipLaunchKernelGGL((sdf_accumulate_kernel<false, SCENE_SDF>), grid, dim3(256), 0, stream, dcfg, accum, start,
+                           count, nullptr);
     return hipGetLastError();
 }
 

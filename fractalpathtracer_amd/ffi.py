@@ -53,6 +53,7 @@ OP_ORBIT_ADD = 32
 
 RENDERER_SDF = 0
 RENDERER_VOXEL = 1
+SDF_ACCUMULATION_AUTO, SDF_ACCUMULATION_PER_SAMPLE, SDF_ACCUMULATION_BATCH, SDF_ACCUMULATION_CHUNKED = 0, 1, 2, 3
 
 VOXEL_STORAGE_DENSE = 0
 VOXEL_STORAGE_SPARSE_BRICKS = 1
