@@ -115,13 +115,21 @@ __host__ __device__ __forceinline__ uint32_t slot_count(uint32_t w, uint32_t h) 
 #ifndef FPT_ORBIT_BATCH
 #define FPT_ORBIT_BATCH 32
 #endif
+// Minimum resident waves per SIMD for imported Mandelbulber scenes (MB).
+#ifndef FPT_MANDELBULBER_PERSISTENT_WAVES
+#define FPT_MANDELBULBER_PERSISTENT_WAVES 2
+#endif
 
-template <bool PROFILE, bool MULTI, bool LEARNED, bool BULB = false>
+// MB = true (single scene): an imported Mandelbulber scene, with the
+// Mandelbulber camera, march (threshold, clamped step, FP32 fixed-point and
+// distance stops; the bracketing refinement runs at the march's event),
+// material and normal (Shaders.metal:473-511, 975-995, 3627-3751, 3871-3960).
+template <bool PROFILE, bool MULTI, bool LEARNED, bool BULB = false, bool MB = false>
 // flatten: every device function the kernel reaches is inlined, so no call
 // (with its stack traffic for live state) ever sits on the march or the
 // per-hit path, however the inliner's size budget would decide.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LEARNED ? FPT_LEARNED_WAVES
-                                                                             : FPT_PERSISTENT_WAVES), flatten))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
+    LEARNED ? FPT_LEARNED_WAVES : (MB ? FPT_MANDELBULBER_PERSISTENT_WAVES : FPT_PERSISTENT_WAVES)), flatten))
 void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
                                                              const BatchScene *__restrict__ scenes, uint32_t nscenes,
                                                              float4 *__restrict__ accum, uint32_t start,
@@ -207,7 +215,18 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
         const int ipw = int(bpower);
         bip = (float(ipw) == bpower && ipw >= 1 && ipw <= 32) ? ipw : 0;
     }
+    // Mandelbulber march state (MB only): the last distance, threshold and
+    // step, whether the threshold band was reached, and a camera sample
+    // outside a cut fish-eye (the path is black).
+    bool mfound = false, invisible = false;
+    float mdist = 0.f, mthr = 0.f, mstep = 0.f;
     bool retry = false;  // slot was tile padding: take another
+
+    // sdfMarchIterationLimit (SM:3626-3633).
+    auto march_limit = [&](int ni) __attribute__((always_inline)) {
+        const int cap = MB ? 10000 : 360;
+        return ni < cap ? ni : cap;
+    };
 
     // Start the camera path of sample `s` of the current pixel.
     auto start_path = [&]() __attribute__((always_inline)) {
@@ -215,10 +234,15 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
         const float frame = float(s);
         float jx, jy;
         random_point(aa, uv, frame, jx, jy);
-        dr = rotate_camera_roll(normalize(mk(uv.x + jx, uv.y + jy, focal)), cfg.camera_yaw_pitch[0],
-                                cfg.camera_yaw_pitch[1], cfg.camera_roll);
+        if (MB) {
+            invisible = !mb_projection_visible(uv.x + jx, uv.y + jy, cfg);
+            dr = invisible ? mk(0.f, 0.f, 1.f) : mb_camera_ray(uv.x + jx, uv.y + jy, cfg);
+        } else {
+            dr = rotate_camera_roll(normalize(mk(uv.x + jx, uv.y + jy, focal)), cfg.camera_yaw_pitch[0],
+                                    cfg.camera_yaw_pitch[1], cfg.camera_roll);
+        }
         p = cam;
-        apply_dof(uv, frame, cfg, p, dr);
+        if (!MB || !invisible) apply_dof(uv, frame, cfg, p, dr);
         const float gt = clampf((dr.y + 1.0f) / 2.0f, 0.f, 1.f);
         gcol = vmix(ld3(&cfg.background_gradient[3]), ld3(&cfg.background_gradient[0]), gt);
         bounce = 0;
@@ -230,12 +254,14 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
         mstart = p;
         mdir = dr;
         it = 0;
-        max_iter = local_ni < 360 ? local_ni : 360;
+        max_iter = march_limit(local_ni);
         shadow = false;
         lfar = LEARNED;
         lastfar = false;
         fit = 0;
-        phase = PH_MARCH;
+        mfound = false;
+        mstep = 0.f;  // a march that starts inside the band refines in place
+        phase = MB && invisible ? PH_EVENT : PH_MARCH;
     };
 
     // Take a work slot (or go idle).  Wave-uniform call.
@@ -271,7 +297,7 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
             return;
         }
         pix = y * W + x;
-        uv = screen_uv(x, y, W, H);
+        uv = MB ? screen_uv_any<SCENE_MB>(x, y, *cp) : screen_uv(x, y, W, H);
         s = s_begin;
         if (s_begin != 0u) {
             float4 prev = out[pix];
@@ -352,6 +378,31 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
                 }
                 if (stop) phase = PH_EVENT;
             }
+        } else if (MB && phase == PH_MARCH) {
+            // marchMandelbulber's loop (SM:3711-3733), one step per trip.
+            const FptRenderConfig &cfg = *cp;
+            bool stop = it >= max_iter;
+            if (!stop) {
+                mthr = mb_threshold(p, cfg);
+                mdist = map_kifs(p, cfg, w);
+                ++it;
+                if (!isfinite(mdist)) {
+                    stop = true;
+                } else if (mdist < mthr) {
+                    mfound = true;
+                    stop = true;
+                } else {
+                    mstep = mb_step(mdist, mthr, cfg);
+                    const f3 nx = p + mdir * mstep;
+                    if (same3(nx, p)) {
+                        stop = true;
+                    } else {
+                        p = nx;
+                        stop = length(p - mstart) >= maxd;
+                    }
+                }
+            }
+            if (stop) phase = PH_EVENT;
         } else if (!BULB && phase == PH_MARCH) {
             const FptRenderConfig &cfg = *cp;
             if (LEARNED && use_q && q > prop.switch_distance) {
@@ -398,7 +449,33 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
             const float frame = float(s);
             bool path_done = false;
             bool do_scatter = false;
-            if (shadow) {
+            if (MB && mfound) {
+                // marchMandelbulber's bracketing refinement (SM:3737-3750):
+                // up to 30 half-steps towards (threshold * (1 - 0.001
+                // detail), threshold), with the last march threshold.
+                const float limit = 1.0f - 0.001f * fmaxf(cfg.vset_values[129], 0.0f);
+                float step = mstep * 0.5f, d = mdist;
+                for (int r = 0; r < 30; ++r) {
+                    if (d < mthr && d > mthr * limit) break;
+                    if (d > mthr) {
+                        const f3 nx = p + mdir * step;
+                        if (same3(nx, p)) break;
+                        p = nx;
+                    } else if (d < mthr * limit) {
+                        const f3 nx = p - mdir * step;
+                        if (same3(nx, p)) break;
+                        p = nx;
+                    }
+                    d = map_kifs(p, cfg, w);
+                    step *= 0.5f;
+                }
+                mfound = false;
+            }
+            if (MB && invisible) {
+                // Outside a cut fish-eye: renderPath returns black (SM:5938-5941).
+                invisible = false;
+                path_done = true;
+            } else if (shadow) {
                 // End of the sun shadow march (SM:5856-5860).
                 pc.shadow_steps += it;
                 if (length(hit_p - p) > maxd * 0.99f) {
@@ -419,9 +496,9 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
                     path_done = true;
                 } else {
                     hit_p = p;
-                    mat = user_material(p, cfg, w);
+                    mat = MB ? material_any<SCENE_MB>(p, cfg, w) : user_material(p, cfg, w);
                     if (mat.emission > 0.001f) light = light + mat.rgb * mat.emission;
-                    hit_n = normal_at(p, cfg, w);
+                    hit_n = MB ? normal_mandelbulber(p, cfg, w) : normal_at(p, cfg, w);
                     pc.normal_evals++;
                     if (sun_on) {
                         // Shadow ray (SM:5846-5856).
@@ -434,7 +511,8 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
                         p = hit_p + hit_n * 0.001f;
                         mstart = p;
                         it = 0;
-                        max_iter = ni_first < 360 ? ni_first : 360;
+                        max_iter = march_limit(ni_first);
+                        mstep = 0.f;
                         shadow = true;
                         lfar = false;
                         lastfar = false;
@@ -498,7 +576,8 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
                     mstart = rp;
                     mdir = dr;
                     it = 0;
-                    max_iter = local_ni < 360 ? local_ni : 360;
+                    max_iter = march_limit(local_ni);
+                    mstep = 0.f;
                     lfar = false;
                     lastfar = false;
                     fit = 0;

@@ -33,6 +33,13 @@ hipError_t launch_sdf_persistent(const FptRenderConfig *dcfg, const FptRenderCon
         else
             hipLaunchKernelGGL((sdf_persistent_kernel<false, false, true>), grid, block, 0, stream, dcfg, nullptr, 1u,
                                accum, start, count, counter, slots, event_batch, nullptr, *prop);
+    } else if (hcfg.sdf_id == FPT_SDF_MANDELBULBER) {
+        if (prof)
+            hipLaunchKernelGGL((sdf_persistent_kernel<true, false, false, false, true>), grid, block, 0, stream, dcfg,
+                               nullptr, 1u, accum, start, count, counter, slots, event_batch, prof, none);
+        else
+            hipLaunchKernelGGL((sdf_persistent_kernel<false, false, false, false, true>), grid, block, 0, stream,
+                               dcfg, nullptr, 1u, accum, start, count, counter, slots, event_batch, nullptr, none);
     } else if (single_bulb_program(hcfg)) {
         if (prof)
             hipLaunchKernelGGL((sdf_persistent_kernel<true, false, false, true>), grid, block, 0, stream, dcfg, nullptr,

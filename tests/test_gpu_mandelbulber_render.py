@@ -2,7 +2,7 @@
 field, formula 10) against the oracle: the reference's Mandelbulber camera,
 distance, march and normal branches (Shaders.metal:473-511, 975-995,
 3627-3751, 3871-3960) inside renderPath (5926-6026), on the
-one-path-per-lane kernel.
+one-path-per-lane kernel (and the persistent kernel as its A/B twin).
 
 Tolerances, as for the other SDF scenes:
   closed-form sphere   every GPU hit ends in the refinement band above the
@@ -118,13 +118,22 @@ def test_profile_counters_match_oracle_work():
         assert abs(p[key] - k[key]) <= 0.02 * k[key], (key, p[key], k[key])
 
 
-def test_plan_runs_the_one_path_kernel():
-    c = _ifs20(32, 18, 1)
-    with api.Plan(c) as plan:
-        with pytest.raises(RuntimeError, match="persistent kernel"):
-            plan.tune(kernel=1)
-        plan.tune(kernel=0)
-        buf = api.DeviceBuffer(32 * 18 * 4)
-        plan.render_into(buf.address)
-        img = buf.numpy().reshape(18, 32, 4)
-    assert np.allclose(img, api.render(c), atol=0, rtol=0)
+def test_persistent_and_one_path_kernels_agree():
+    """Imported scenes render on the one-path-per-lane kernel by default (the
+    faster one for them) and on the persistent kernel when a plan asks for
+    it: two independent implementations of the same path, whose images must
+    be bit-identical (cut fish-eye too)."""
+    for proj, fov in ((0.0, None), (3.0, 200.0)):
+        c = _ifs20(48, 27, 3)
+        c.vset_values[107] = proj
+        if fov:
+            c.camera_fov = fov
+        imgs = []
+        for kernel in (1, 0):
+            with api.Plan(c) as plan:
+                plan.tune(kernel=kernel)
+                buf = api.DeviceBuffer(48 * 27 * 4)
+                plan.render_into(buf.address)
+                imgs.append(buf.numpy().reshape(27, 48, 4))
+        assert np.array_equal(imgs[0], imgs[1]), proj
+        assert np.array_equal(imgs[0], api.render(c))
