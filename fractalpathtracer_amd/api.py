@@ -156,6 +156,21 @@ def test_voxel_dda() -> None:
     ffi.check(lib.fpt_test_voxel_dda(None, err, len(err)), err)
 
 
+def test_typed_soa(cfg: FptRenderConfig) -> tuple[bool, dict, str]:
+    """fpt_test_typed_soa (metal_bridge.mm:4116-4203): validate the config's
+    typed structure-of-arrays table against its interpreted program at 2^20
+    hashed points.  Returns (passed, stats, message); invalid input raises."""
+    lib = ffi.load_library()
+    st = ffi.FptStitchValidationStats()
+    err = ffi.error_buffer()
+    rc = lib.fpt_test_typed_soa(None, C.byref(cfg), C.byref(st), err, len(err))
+    stats = {f: getattr(st, f) for f, _ in ffi.FptStitchValidationStats._fields_}
+    msg = err.value.decode()
+    if rc != 0 and st.sample_count == 0:
+        raise RuntimeError(msg)
+    return rc == 0, stats, msg
+
+
 def hydrate_hdri(cfg: FptRenderConfig) -> FptRenderConfig:
     """A copy of `cfg` with its HDRI LUT loaded from hdri_path, as every
     render entry point does (fpt_hip_hydrate_hdri; host only)."""

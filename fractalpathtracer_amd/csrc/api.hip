@@ -869,6 +869,7 @@ struct FptHipBatch {
     uint32_t *counter = nullptr;
     uint32_t total_slots = 0;
     size_t pixels = 0;
+    bool lowered = false;  // some scene carries lowered program tables
     uint32_t workgroups = 0;
     uint32_t event_batch = 24;
     uint32_t use_graph = 1;
@@ -939,6 +940,7 @@ int fpt_hip_batch_create(const FptRenderConfig *configs, uint32_t count, FptHipB
     }
     b->total_slots = uint32_t(slot);
     b->pixels = size_t(off);
+    for (const FptRenderConfig &c : b->host) b->lowered = b->lowered || fpt::has_lowered_tables(c);
     hipError_t e = hipMalloc(&b->dcfgs, sizeof(FptRenderConfig) * count);
     if (e == hipSuccess) e = hipMalloc(&b->dscenes, sizeof(fpt::BatchScene) * count);
     if (e == hipSuccess) e = hipMalloc(&b->counter, sizeof(uint32_t));
@@ -992,7 +994,7 @@ int fpt_hip_batch_render(FptHipBatch *batch, float *device_accum, void *stream, 
     float4 *acc = reinterpret_cast<float4 *>(device_accum);
     if (!batch->use_graph) {
         HIP_TRY(fpt::launch_sdf_batch(batch->dcfgs, batch->dscenes, uint32_t(batch->host.size()), batch->total_slots, acc,
-                                      batch->counter, wg, int(batch->event_batch), 0u, nullptr, s),
+                                      batch->counter, wg, int(batch->event_batch), 0u, nullptr, batch->lowered, s),
                 "batch kernel launch");
         return 0;
     }
@@ -1004,7 +1006,7 @@ int fpt_hip_batch_render(FptHipBatch *batch, float *device_accum, void *stream, 
         hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess)
             e = fpt::launch_sdf_batch(batch->dcfgs, batch->dscenes, uint32_t(batch->host.size()), batch->total_slots,
-                                      acc, batch->counter, wg, int(batch->event_batch), 0u, nullptr, cap);
+                                      acc, batch->counter, wg, int(batch->event_batch), 0u, nullptr, batch->lowered, cap);
         hipGraph_t g = nullptr;
         hipError_t e2 = hipStreamEndCapture(cap, &g);
         (void)hipStreamDestroy(cap);
@@ -1100,7 +1102,7 @@ int fpt_hip_autotune(FptHipPlan *plan, FptHipBatch *batch, float *device_accum, 
             return fpt::launch_sdf_persistent(plan->dcfg, plan->host, acc, 0u, 1u, plan->counter, wg, int(eb), nullptr,
                                               plan->proposer.enabled ? &plan->proposer.dev : nullptr, s);
         return fpt::launch_sdf_batch(batch->dcfgs, batch->dscenes, uint32_t(batch->host.size()), batch->total_slots, acc,
-                                     batch->counter, wg, int(eb), 1u, nullptr, s);
+                                     batch->counter, wg, int(eb), 1u, nullptr, batch->lowered, s);
     };
     HIP_TRY(launch(cand[0].first, cand[0].second), "autotune warm-up");
     for (uint32_t r = 0; r < rounds; ++r) {
@@ -1647,10 +1649,45 @@ int fpt_test_async_stitch_context(const char *, const char *, const char *, cons
     return 1;
 }
 
-int fpt_test_typed_soa(const char *, const FptRenderConfig *, FptStitchValidationStats *, char *error,
+// metal_bridge.mm:4116-4203: validates a lowered typed structure-of-arrays
+// table against the interpreted program at 2^20 hashed points; fails when
+// any point misses a tolerance.  The library path is ignored.
+int fpt_test_typed_soa(const char *, const FptRenderConfig *config, FptStitchValidationStats *stats, char *error,
                        size_t error_len) {
-    set_error(error, error_len, "typed SoA lowering has no counterpart in this library");
-    return 1;
+    if (!config || !stats) {
+        set_error(error, error_len, "missing typed-SoA validation input");
+        return 1;
+    }
+    std::memset(stats, 0, sizeof(*stats));
+    const FptTypedSoAProgram &t = config->sdf_typed_soa;
+    if (config->sdf_id != FPT_SDF_PROGRAM || t.sphere_count + t.box_count + t.plane_count == 0u) {
+        set_error(error, error_len, "typed-SoA validation requires a lowered typed program");
+        return 1;
+    }
+    if (check_device(error, error_len)) return 1;
+    constexpr uint32_t kSamples = 1u << 20;
+    Stream st;
+    HIP_TRY(hipStreamCreate(&st.s), "hipStreamCreate");
+    DevBuf dcfg, dcounts;
+    HIP_TRY(dcfg.alloc(sizeof(FptRenderConfig)), "config allocation");
+    HIP_TRY(dcounts.alloc(4 * sizeof(uint32_t)), "counter allocation");
+    HIP_TRY(hipMemcpyAsync(dcfg.p, config, sizeof(FptRenderConfig), hipMemcpyHostToDevice, st.s), "config upload");
+    HIP_TRY(hipMemsetAsync(dcounts.p, 0, 4 * sizeof(uint32_t), st.s), "counter reset");
+    HIP_TRY(fpt::launch_typed_soa_validation(dcfg.as<FptRenderConfig>(), kSamples, dcounts.as<uint32_t>(), st.s),
+            "typed-SoA validation launch");
+    uint32_t counts[4] = {};
+    HIP_TRY(hipMemcpyAsync(counts, dcounts.p, sizeof(counts), hipMemcpyDeviceToHost, st.s), "readback");
+    HIP_TRY(hipStreamSynchronize(st.s), "typed-SoA validation");
+    stats->sample_count = kSamples;
+    stats->distance_failures = counts[0];
+    stats->gradient_failures = counts[1];
+    std::memcpy(&stats->max_distance_error, &counts[2], sizeof(float));
+    std::memcpy(&stats->max_gradient_error, &counts[3], sizeof(float));
+    if (counts[0] != 0u || counts[1] != 0u) {
+        set_error(error, error_len, "typed-SoA validation failed: %u distance, %u gradient", counts[0], counts[1]);
+        return 1;
+    }
+    return 0;
 }
 
 int fpt_metal_preview(const char *, const char *, const char *const *, const char *, size_t,

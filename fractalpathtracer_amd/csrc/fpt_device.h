@@ -873,6 +873,161 @@ __device__ __forceinline__ uint32_t prog_count(const FptRenderConfig &cfg) {
     return cfg.sdf_program_count < 64u ? cfg.sdf_program_count : 64u;
 }
 
+// ---------------------------------------------------------------------------
+// Direct evaluators of lowered programs (SM:2713-2970): the typed
+// structure-of-arrays, canonical affine and flat-union tables the scene front
+// end lowers a union of spheres, boxes and planes into.  A program that
+// carries them is evaluated from the tables, as the reference does
+// (SM:3576-3584 distance, SM:3403-3408 surface).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t lowered_cap(uint32_t n) {
+    return n < uint32_t(FPT_SDF_FLAT_UNION_MAX_PRIMITIVES) ? n : uint32_t(FPT_SDF_FLAT_UNION_MAX_PRIMITIVES);
+}
+__device__ __forceinline__ uint32_t typed_soa_count(const FptRenderConfig &cfg) {  // SM:2848-2855
+    const FptTypedSoAProgram &t = cfg.sdf_typed_soa;
+    return lowered_cap(t.sphere_count) + lowered_cap(t.box_count) + lowered_cap(t.plane_count);
+}
+
+// Instance point: the 3x4 row-major affine transform applied to the source.
+__device__ __forceinline__ f3 instance_point(f3 s, const float *t) {
+    return mk(dot(mk(t[0], t[1], t[2]), s) + t[3], dot(mk(t[4], t[5], t[6]), s) + t[7],
+              dot(mk(t[8], t[9], t[10]), s) + t[11]);
+}
+
+__device__ __forceinline__ float instance_distance(f3 s, const FptPrimitiveInstance &in) {  // SM:2713-2737
+    const f3 p = instance_point(s, in.transform);
+    const float div = fmaxf(in.distance_scale, 1.0e-6f);
+    if (in.opcode == FPT_SDF_OP_SPHERE) return (length(p) - in.data[0]) / div;
+    if (in.opcode == FPT_SDF_OP_BOX) {
+        const f3 q = vabs(p) - vabs(ld3(in.data));
+        return (fminf(fmaxf(q.x, fmaxf(q.y, q.z)), 0.0f) + length(vmax0(q))) / div;
+    }
+    if (in.opcode == FPT_SDF_OP_PLANE) return (dot(p, ld3(in.data)) + in.data[3]) / div;
+    return kInf;
+}
+
+__device__ float flat_union_distance(f3 s, const FptRenderConfig &cfg) {  // SM:2739-2749
+    float d = kInf;
+    const uint32_t n = lowered_cap(cfg.sdf_flat_union_count);
+    for (uint32_t i = 0; i < n; ++i) d = fminf(d, instance_distance(s, cfg.sdf_flat_union_instances[i]));
+    return d;
+}
+
+// Canonical CSG list: _pad0 0 union, 1 intersection, 2 subtraction.
+__device__ float canonical_distance(f3 s, const FptRenderConfig &cfg) {  // SM:2751-2770
+    float d = kInf;
+    const uint32_t n = lowered_cap(cfg.sdf_canonical_count);
+    for (uint32_t i = 0; i < n; ++i) {
+        const FptPrimitiveInstance &in = cfg.sdf_canonical_primitives[i];
+        const float c = instance_distance(s, in);
+        if (i == 0u) d = c;
+        else if (in._pad0 == 1u) d = fmaxf(d, c);
+        else if (in._pad0 == 2u) d = fmaxf(d, -c);
+        else d = fminf(d, c);
+    }
+    return d;
+}
+
+__device__ float canonical_surface(f3 s, const FptRenderConfig &cfg, f3 &grad) {  // SM:2772-2846
+    float d = kInf;
+    grad = mk(0.f, 0.f, 0.f);
+    const uint32_t n = lowered_cap(cfg.sdf_canonical_count);
+    for (uint32_t i = 0; i < n; ++i) {
+        const FptPrimitiveInstance &in = cfg.sdf_canonical_primitives[i];
+        const float *t = in.transform;
+        const f3 r0 = mk(t[0], t[1], t[2]), r1 = mk(t[4], t[5], t[6]), r2 = mk(t[8], t[9], t[10]);
+        const f3 p = mk(dot(r0, s) + t[3], dot(r1, s) + t[7], dot(r2, s) + t[11]);
+        const float div = fmaxf(in.distance_scale, 1.0e-6f);
+        float c = kInf;
+        f3 lg = mk(0.f, 1.f, 0.f);
+        if (in.opcode == FPT_SDF_OP_SPHERE) {
+            const float r = length(p);
+            c = (r - in.data[0]) / div;
+            if (r > 1.0e-8f) lg = p / r;
+        } else if (in.opcode == FPT_SDF_OP_BOX) {
+            const f3 q = vabs(p) - vabs(ld3(in.data));
+            const f3 o = vmax0(q);
+            const float ol = length(o);
+            c = (fminf(fmaxf(q.x, fmaxf(q.y, q.z)), 0.0f) + ol) / div;
+            if (ol > 1.0e-8f) lg = mk(signf(p.x), signf(p.y), signf(p.z)) * o / ol;
+            else if (q.x >= q.y && q.x >= q.z) lg = mk(signf(p.x), 0.f, 0.f);
+            else if (q.y >= q.z) lg = mk(0.f, signf(p.y), 0.f);
+            else lg = mk(0.f, 0.f, signf(p.z));
+        } else if (in.opcode == FPT_SDF_OP_PLANE) {
+            lg = ld3(in.data);
+            c = (dot(p, lg) + in.data[3]) / div;
+        }
+        const f3 cg = (r0 * lg.x + r1 * lg.y + r2 * lg.z) / div;
+        const bool choose = i == 0u || (in._pad0 == 1u ? c > d : (in._pad0 == 2u ? -c > d : c < d));
+        if (choose) {
+            const bool neg = in._pad0 == 2u && i > 0u;
+            d = neg ? -c : c;
+            grad = neg ? -cg : cg;
+        }
+    }
+    return d;
+}
+
+__device__ float typed_soa_distance(f3 s, const FptRenderConfig &cfg) {  // SM:2857-2896
+    const FptTypedSoAProgram &t = cfg.sdf_typed_soa;
+    float d = kInf;
+    for (uint32_t i = 0, n = lowered_cap(t.sphere_count); i < n; ++i)
+        d = fminf(d, length(s - mk(t.sphere_x[i], t.sphere_y[i], t.sphere_z[i])) - t.sphere_radius[i]);
+    for (uint32_t i = 0, n = lowered_cap(t.box_count); i < n; ++i) {
+        const f3 q = vabs(s - mk(t.box_x[i], t.box_y[i], t.box_z[i])) - mk(t.box_half_x[i], t.box_half_y[i], t.box_half_z[i]);
+        d = fminf(d, fminf(fmaxf(q.x, fmaxf(q.y, q.z)), 0.0f) + length(vmax0(q)));
+    }
+    for (uint32_t i = 0, n = lowered_cap(t.plane_count); i < n; ++i) {
+        const f3 nrm = normalize(mk(t.plane_x[i], t.plane_y[i], t.plane_z[i]));
+        const f3 c = mk(t.plane_center_x[i], t.plane_center_y[i], t.plane_center_z[i]);
+        d = fminf(d, dot(s - c, nrm) + t.plane_offset[i]);
+    }
+    return d;
+}
+
+// Union winner with its gradient; ties go to the earlier source instruction.
+__device__ float typed_soa_surface(f3 s, const FptRenderConfig &cfg, f3 &grad) {  // SM:2898-2970
+    const FptTypedSoAProgram &t = cfg.sdf_typed_soa;
+    float d = kInf;
+    grad = mk(0.f, 0.f, 0.f);
+    uint32_t winner = 0xffffffffu;
+    for (uint32_t i = 0, n = lowered_cap(t.sphere_count); i < n; ++i) {
+        const f3 l = s - mk(t.sphere_x[i], t.sphere_y[i], t.sphere_z[i]);
+        const float r = length(l), c = r - t.sphere_radius[i];
+        if (c < d || (c == d && t.sphere_source[i] < winner)) {
+            d = c;
+            grad = r > 1.0e-8f ? l / r : mk(0.f, 1.f, 0.f);
+            winner = t.sphere_source[i];
+        }
+    }
+    for (uint32_t i = 0, n = lowered_cap(t.box_count); i < n; ++i) {
+        const f3 l = s - mk(t.box_x[i], t.box_y[i], t.box_z[i]);
+        const f3 q = vabs(l) - mk(t.box_half_x[i], t.box_half_y[i], t.box_half_z[i]);
+        const f3 o = vmax0(q);
+        const float ol = length(o);
+        const float c = fminf(fmaxf(q.x, fmaxf(q.y, q.z)), 0.0f) + ol;
+        if (c < d || (c == d && t.box_source[i] < winner)) {
+            d = c;
+            if (ol > 1.0e-8f) grad = mk(signf(l.x), signf(l.y), signf(l.z)) * o / ol;
+            else if (q.x >= q.y && q.x >= q.z) grad = mk(signf(l.x), 0.f, 0.f);
+            else if (q.y >= q.z) grad = mk(0.f, signf(l.y), 0.f);
+            else grad = mk(0.f, 0.f, signf(l.z));
+            winner = t.box_source[i];
+        }
+    }
+    for (uint32_t i = 0, n = lowered_cap(t.plane_count); i < n; ++i) {
+        const f3 nrm = normalize(mk(t.plane_x[i], t.plane_y[i], t.plane_z[i]));
+        const f3 c0 = mk(t.plane_center_x[i], t.plane_center_y[i], t.plane_center_z[i]);
+        const float c = dot(s - c0, nrm) + t.plane_offset[i];
+        if (c < d || (c == d && t.plane_source[i] < winner)) {
+            d = c;
+            grad = nrm;
+            winner = t.plane_source[i];
+        }
+    }
+    return d;
+}
+
 __device__ __forceinline__ float distance_sdf(f3 p, const FptRenderConfig &cfg, Work &w) {
     w.distance_evals++;
 #ifdef FPT_TOPOLOGY_SPECIALIZED
@@ -889,6 +1044,32 @@ __device__ __forceinline__ float map_sdf(f3 p, const FptRenderConfig &cfg, Work 
     return mixf(d, d / 2.0f, cfg.render[6]);
 }
 
+// The scene's distance and map with the lowered-program evaluators (the
+// normal is normal_scene, after normal_at):
+// a program that carries typed-SoA, canonical or flat-union tables is
+// evaluated from them (SM:3576-3584, 3403-3408); everything else is
+// distance_sdf / map_sdf / normal_at.  (A runtime-specialised build evaluates
+// its topology instead, as the reference's does.)  The render kernels have
+// instances with and without the tables' code: only configs that carry
+// tables pay for it.
+__device__ __forceinline__ bool has_lowered(const FptRenderConfig &cfg) {
+    return is_program(cfg) && (typed_soa_count(cfg) > 0u || cfg.sdf_canonical_count > 0u || cfg.sdf_flat_union_count > 0u);
+}
+__device__ __forceinline__ float distance_scene(f3 p, const FptRenderConfig &cfg, Work &w) {
+#ifndef FPT_TOPOLOGY_SPECIALIZED
+    if (has_lowered(cfg)) {
+        w.distance_evals++;
+        if (typed_soa_count(cfg) > 0u) return typed_soa_distance(p, cfg);
+        if (cfg.sdf_canonical_count > 0u) return canonical_distance(p, cfg);
+        return flat_union_distance(p, cfg);
+    }
+#endif
+    return distance_sdf(p, cfg, w);
+}
+__device__ __forceinline__ float map_scene(f3 p, const FptRenderConfig &cfg, Work &w) {  // SM:3612-3615
+    const float d = distance_scene(p, cfg, w);
+    return mixf(d, d / 2.0f, cfg.render[6]);
+}
 __device__ __forceinline__ f3 hsv2rgb(f3 c) {  // SM:534-539
     f3 p = mk(fabsf(fractf(c.x + 1.0f) * 6.0f - 3.0f), fabsf(fractf(c.x + 2.0f / 3.0f) * 6.0f - 3.0f),
               fabsf(fractf(c.x + 1.0f / 3.0f) * 6.0f - 3.0f));
@@ -978,7 +1159,10 @@ __device__ Material user_material(f3 p, const FptRenderConfig &cfg, Work &w) {  
     return material_at_param(material_param(material_orbit(p, cfg, w), cfg), cfg);
 }
 
-// march (SM:3753-3783), non-Mandelbulber branch.
+// march (SM:3753-3783), non-Mandelbulber branch.  LOWERED: the distance
+// reads lowered tables when the config carries them (map_scene); without it
+// the march is map_sdf's, for kernel instances whose configs carry none.
+template <bool LOWERED = true>
 __device__ f3 march(f3 dr, f3 rp, int ni, float min_dist, float lod_falloff, const FptRenderConfig &cfg,
                     Work &w, int &steps) {
     const f3 cam = rp;
@@ -988,7 +1172,7 @@ __device__ f3 march(f3 dr, f3 rp, int ni, float min_dist, float lod_falloff, con
     const float maxd = cfg.render[4];
     steps = 0;
     for (int i = 0; i < max_iter; ++i) {
-        float d = map_sdf(rp, cfg, w);
+        float d = LOWERED ? map_scene(rp, cfg, w) : map_sdf(rp, cfg, w);
         ++steps;
         if (!isfinite(d)) break;
         float o = fabsf(d) * 0.99f;
@@ -1063,6 +1247,35 @@ __device__ f3 normal_at(f3 p, const FptRenderConfig &cfg, Work &w) {
     f3 n = mk(map_sdf(p + ex, cfg, w) - map_sdf(p - ex, cfg, w), map_sdf(p + ey, cfg, w) - map_sdf(p - ey, cfg, w),
               map_sdf(p + ez, cfg, w) - map_sdf(p - ez, cfg, w));
     return normalize_fd(n);
+}
+
+__device__ f3 normal_scene(f3 p, const FptRenderConfig &cfg, Work &w) {  // SM:3871-3971
+#ifndef FPT_TOPOLOGY_SPECIALIZED
+    if (has_lowered(cfg)) {
+        const float e = fmaxf(cfg.render[2], 0.0002f);
+        if (cfg.sdf_normal_mode == 0u || cfg.sdf_normal_mode == 2u) {
+            // programSurface: typed SoA, canonical, else the interpreted
+            // program (a flat union has no surface form)
+            f3 g;
+            if (typed_soa_count(cfg) > 0u) typed_soa_surface(p, cfg, g);
+            else if (cfg.sdf_canonical_count > 0u) canonical_surface(p, cfg, g);
+            else program_surface(cfg.sdf_program, prog_count(cfg), p, g);
+            if (dot(g, g) >= 1.0e-12f && finite3(g)) return normalize(g);
+        }
+        if (cfg.sdf_normal_mode == 1u) {
+            const float te = e * 0.57735026919f;
+            const f3 k0 = mk(1.f, -1.f, -1.f), k1 = mk(-1.f, -1.f, 1.f), k2 = mk(-1.f, 1.f, -1.f), k3 = mk(1.f, 1.f, 1.f);
+            const float d0 = map_scene(p + k0 * te, cfg, w), d1 = map_scene(p + k1 * te, cfg, w);
+            const float d2 = map_scene(p + k2 * te, cfg, w), d3 = map_scene(p + k3 * te, cfg, w);
+            return normalize_fd(k0 * d0 + k1 * d1 + k2 * d2 + k3 * d3);
+        }
+        const f3 ex = mk(e, 0.f, 0.f), ey = mk(0.f, e, 0.f), ez = mk(0.f, 0.f, e);
+        return normalize_fd(mk(map_scene(p + ex, cfg, w) - map_scene(p - ex, cfg, w),
+                               map_scene(p + ey, cfg, w) - map_scene(p - ey, cfg, w),
+                               map_scene(p + ez, cfg, w) - map_scene(p - ez, cfg, w)));
+    }
+#endif
+    return normal_at(p, cfg, w);
 }
 
 // ---------------------------------------------------------------------------
@@ -1446,14 +1659,17 @@ __device__ __forceinline__ bool is_mandelbulber(const FptRenderConfig &cfg) {
 }
 
 // Scene kinds a kernel is built for: SCENE_ANY decides per config at run
-// time; SCENE_SDF (typed programs and built-in fractals) and SCENE_MB
-// (imported Mandelbulber scenes) leave the other kind's code out, so each
-// kernel instance carries only the registers its scenes need.
-enum : int { SCENE_ANY = 0, SCENE_SDF = 1, SCENE_MB = 2 };
+// time; SCENE_SDF (typed programs and built-in fractals without lowered
+// tables), SCENE_LOWERED (the same, tables allowed) and SCENE_MB (imported
+// Mandelbulber scenes) leave the other kinds' code out, so each kernel
+// instance carries only the registers its scenes need.
+enum : int { SCENE_ANY = 0, SCENE_SDF = 1, SCENE_MB = 2, SCENE_LOWERED = 3 };
 template <int K>
 __device__ __forceinline__ bool mb_scene(const FptRenderConfig &cfg) {
     return K == SCENE_MB || (K == SCENE_ANY && is_mandelbulber(cfg));
 }
+template <int K>
+constexpr bool lowered_kind = K == SCENE_ANY || K == SCENE_LOWERED;
 
 // Entry points for every SDF scene: the Mandelbulber branches above for
 // imported scenes, the functions of the SDF path otherwise.
@@ -1461,7 +1677,7 @@ template <int K = SCENE_ANY>
 __device__ __forceinline__ f3 march_any(f3 dr, f3 rp, int ni, float min_dist, float lod_falloff,
                                         const FptRenderConfig &cfg, Work &w, int &steps) {  // SM:3753-3760
     if (mb_scene<K>(cfg)) return march_mandelbulber(dr, rp, ni, cfg, w, steps);
-    return march(dr, rp, ni, min_dist, lod_falloff, cfg, w, steps);
+    return march<lowered_kind<K>>(dr, rp, ni, min_dist, lod_falloff, cfg, w, steps);
 }
 // The path-cost diagnostics' march (SM:6857-6908): the scene's march,
 // except that Mandelbulber scenes use the generic loop with their step rules.
@@ -1469,17 +1685,17 @@ template <int K = SCENE_ANY>
 __device__ __forceinline__ f3 march_counted_any(f3 dr, f3 rp, int ni, float min_dist, float lod_falloff,
                                                 const FptRenderConfig &cfg, Work &w, int &steps) {
     if (mb_scene<K>(cfg)) return march_counted_mandelbulber(dr, rp, ni, lod_falloff, cfg, w, steps);
-    return march(dr, rp, ni, min_dist, lod_falloff, cfg, w, steps);
+    return march<lowered_kind<K>>(dr, rp, ni, min_dist, lod_falloff, cfg, w, steps);
 }
 template <int K = SCENE_ANY>
 __device__ __forceinline__ f3 normal_any(f3 p, const FptRenderConfig &cfg, Work &w) {
     if (mb_scene<K>(cfg)) return normal_mandelbulber(p, cfg, w);
-    return normal_at(p, cfg, w);
+    return lowered_kind<K> ? normal_scene(p, cfg, w) : normal_at(p, cfg, w);
 }
 template <int K = SCENE_ANY>
 __device__ __forceinline__ float map_any(f3 p, const FptRenderConfig &cfg, Work &w) {
     if (mb_scene<K>(cfg)) return map_kifs(p, cfg, w);
-    return map_sdf(p, cfg, w);
+    return lowered_kind<K> ? map_scene(p, cfg, w) : map_sdf(p, cfg, w);
 }
 template <int K = SCENE_ANY>
 __device__ __forceinline__ Material material_any(f3 p, const FptRenderConfig &cfg, Work &w) {  // SM:3539-3562

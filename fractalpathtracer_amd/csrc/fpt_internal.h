@@ -51,9 +51,13 @@ uint32_t persistent_slots(uint32_t width, uint32_t height);
 // The scene's program is one Mandelbulb leaf: the persistent kernel's
 // interleaved-orbit variant applies.
 bool single_bulb_program(const FptRenderConfig &c);
+// The scene's program carries flat-union, canonical or typed-SoA tables: the
+// kernels' instances with the table evaluators apply.
+bool has_lowered_tables(const FptRenderConfig &c);
 hipError_t launch_sdf_batch(const FptRenderConfig *dcfgs, const BatchScene *dscenes, uint32_t nscenes,
                             uint32_t total_slots, float4 *accum, uint32_t *counter, uint32_t workgroups,
-                            int event_batch, uint32_t spp_cap, DeviceProfile *prof, hipStream_t stream);
+                            int event_batch, uint32_t spp_cap, DeviceProfile *prof, bool lowered,
+                            hipStream_t stream);
 
 hipError_t launch_sdf_accumulate(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, float4 *accum,
                                  uint32_t start, uint32_t count, DeviceProfile *prof, hipStream_t stream);
@@ -75,6 +79,10 @@ hipError_t launch_field_sample(const FptRenderConfig *dcfg, const float4 *pts, s
                                FptMandelbulberFieldSample *out, hipStream_t stream);
 hipError_t launch_surface_sample(const FptRenderConfig *dcfg, const float4 *pts, size_t n, float4 *out,
                                  hipStream_t stream);
+// counts: {distance failures, gradient failures, max distance error bits,
+// max gradient error bits}, zeroed by the caller.
+hipError_t launch_typed_soa_validation(const FptRenderConfig *dcfg, uint32_t n, uint32_t *counts,
+                                       hipStream_t stream);
 hipError_t launch_sdf_persistent(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, float4 *accum,
                                  uint32_t start, uint32_t count, uint32_t *counter, uint32_t workgroups,
                                  int event_batch, DeviceProfile *prof, const ProposerDevice *prop,

@@ -161,7 +161,7 @@ __global__ __launch_bounds__(256) void proposer_sample_kernel(const FptRenderCon
     for (uint32_t a = 0; a < max_attempts && !ok; ++a) {
         const float ux = splitmix_unit(state), uy = splitmix_unit(state), uz = splitmix_unit(state);
         p = mk((2.0f * ux - 1.0f) * half, (2.0f * uy - 1.0f) * half, (2.0f * uz - 1.0f) * half);
-        d = map_sdf(p, *cfg, w);
+        d = map_scene(p, *cfg, w);
         ok = isfinite(d) && (!near || fabsf(d) < band);
     }
     if (!ok) atomicAdd(failures, 1u);

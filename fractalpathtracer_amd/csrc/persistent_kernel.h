@@ -49,6 +49,10 @@
 // surface and the march backs up to the point before it.  Far steps have their
 // own budget (the march's step limit); when it runs out the march stays
 // exact.  Hits, normals, shading and termination tests are the exact ones.
+//
+// LOWERED = true: configs whose program carries flat-union, canonical or
+// typed structure-of-arrays tables march and shade through them
+// (map_scene / normal_scene); the other instances leave that code out.
 #pragma once
 
 #ifndef __HIPCC_RTC__
@@ -124,7 +128,7 @@ __host__ __device__ __forceinline__ uint32_t slot_count(uint32_t w, uint32_t h) 
 // Mandelbulber camera, march (threshold, clamped step, FP32 fixed-point and
 // distance stops; the bracketing refinement runs at the march's event),
 // material and normal (Shaders.metal:473-511, 975-995, 3627-3751, 3871-3960).
-template <bool PROFILE, bool MULTI, bool LEARNED, bool BULB = false, bool MB = false>
+template <bool PROFILE, bool MULTI, bool LEARNED, bool BULB = false, bool MB = false, bool LOWERED = false>
 // flatten: every device function the kernel reaches is inlined, so no call
 // (with its stack traffic for live state) ever sits on the march or the
 // per-hit path, however the inliner's size budget would decide.
@@ -414,7 +418,7 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
                 if (PROFILE) lc.far_steps++;
             } else {
                 if (LEARNED && use_q) lfar = false;  // within switch_distance: exact from here
-                const float d = map_sdf(p, cfg, w);
+                const float d = LOWERED ? map_scene(p, cfg, w) : map_sdf(p, cfg, w);
                 ++it;
                 if (LEARNED && lastfar && use_q && d < 0.0f) {
                     // The last far step crossed the surface: back up.
@@ -498,7 +502,7 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
                     hit_p = p;
                     mat = MB ? material_any<SCENE_MB>(p, cfg, w) : user_material(p, cfg, w);
                     if (mat.emission > 0.001f) light = light + mat.rgb * mat.emission;
-                    hit_n = MB ? normal_mandelbulber(p, cfg, w) : normal_at(p, cfg, w);
+                    hit_n = MB ? normal_mandelbulber(p, cfg, w) : (LOWERED ? normal_scene(p, cfg, w) : normal_at(p, cfg, w));
                     pc.normal_evals++;
                     if (sun_on) {
                         // Shadow ray (SM:5846-5856).

@@ -246,7 +246,7 @@ __global__ __launch_bounds__(256) void field_sample_kernel(const FptRenderConfig
         out[i] = s;
         return;
     }
-    s.distance = map_sdf(p, cfg, w);
+    s.distance = map_scene(p, cfg, w);
     s.radius = 0.f;
     s.derivative = 0.f;
     s.iterations = 0.f;
@@ -280,6 +280,63 @@ __global__ __launch_bounds__(256) void surface_sample_kernel(const FptRenderConf
     f3 g;
     float d = program_surface(cfg.sdf_program, prog_count(cfg), mk(q.x, q.y, q.z), g);
     out[i] = make_float4(d, g.x, g.y, g.z);
+}
+
+// typed_soa_validation_kernel (SM:5106-5154): the typed structure-of-arrays
+// table against the interpreted program at hashed points of the voxel
+// bounds (lowbias32, SM:4992-4999), distance within 2e-5 (1 + |d|) and
+// gradient within 1e-4 per component.  Counts are reduced across the wave
+// first: one atomic per wave and counter.
+__device__ __forceinline__ uint32_t validation_hash(uint32_t v) {  // SM:4992-4999
+    v ^= v >> 16u;
+    v *= 0x7feb352du;
+    v ^= v >> 15u;
+    v *= 0x846ca68bu;
+    v ^= v >> 16u;
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, uint32_t(__shfl_xor(int(v), o, 64)));
+    return v;
+}
+__global__ __launch_bounds__(256) void typed_soa_validation_kernel(const FptRenderConfig *__restrict__ cfgp,
+                                                                   uint32_t n, uint32_t *__restrict__ counts) {
+    const FptRenderConfig &cfg = *cfgp;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    bool dfail = false, gfail = false;
+    uint32_t dbits = 0u, gbits = 0u;
+    if (gid < n) {
+        const uint32_t hx = validation_hash(gid * 3u + 0x9e3779b9u);
+        const uint32_t hy = validation_hash(gid * 3u + 0x243f6a88u);
+        const uint32_t hz = validation_hash(gid * 3u + 0xb7e15162u);
+        const f3 unit = mk(float(hx), float(hy), float(hz)) * (1.0f / 4294967295.0f);
+        const f3 lo = ld3(cfg.voxel_bounds_min), hi = ld3(cfg.voxel_bounds_max);
+        f3 p = mk(mixf(lo.x, hi.x, unit.x), mixf(lo.y, hi.y, unit.y), mixf(lo.z, hi.z, unit.z));
+        if (gid == 0u) p = mk(0.f, 0.f, 0.f);
+        if (gid == 1u) p = lo;
+        if (gid == 2u) p = hi;
+        if (gid == 3u) p = (lo + hi) * 0.5f;
+        const float ref = program_eval<false>(cfg.sdf_program, prog_count(cfg), p, nullptr, nullptr);
+        const float direct = typed_soa_distance(p, cfg);
+        const float derr = fabsf(ref - direct);
+        dfail = !isfinite(ref) || !isfinite(direct) || derr > 2.0e-5f * (1.0f + fabsf(ref));
+        dbits = __float_as_uint(derr);
+        f3 gr, gd;
+        program_surface(cfg.sdf_program, prog_count(cfg), p, gr);
+        typed_soa_surface(p, cfg, gd);
+        const float gerr = fmaxf(fabsf(gr.x - gd.x), fmaxf(fabsf(gr.y - gd.y), fabsf(gr.z - gd.z)));
+        gfail = !finite3(gr) || !finite3(gd) || gerr > 1.0e-4f;
+        gbits = __float_as_uint(gerr);
+    }
+    const uint32_t nd = uint32_t(__popcll(__ballot(dfail))), ng = uint32_t(__popcll(__ballot(gfail)));
+    dbits = wave_max_u32(dbits);
+    gbits = wave_max_u32(gbits);
+    if ((threadIdx.x & 63u) == 0u) {
+        if (nd) atomicAdd(&counts[0], nd);
+        if (ng) atomicAdd(&counts[1], ng);
+        atomicMax(&counts[2], dbits);
+        atomicMax(&counts[3], gbits);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -485,7 +542,7 @@ __device__ f3 sdf_diagnostic(UV xy, const FptRenderConfig &cfg, const FptDiagnos
         const int steps = requested < 420 ? requested : 420;
         const float thr = fmaxf(cfg.render[3], 0.0005f);
         for (int i = 0; i < steps; ++i) {
-            const float d = map_sdf(rp, cfg, w);
+            const float d = map_scene(rp, cfg, w);
             if (!isfinite(d)) break;
             if (d < thr && length(rp - ro) > 0.001f) {
                 hit = true;
@@ -558,20 +615,24 @@ hipError_t launch_sdf_diagnostic(const FptRenderConfig *dcfg, const FptRenderCon
 
 hipError_t launch_sdf_accumulate(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, float4 *accum,
                                  uint32_t start, uint32_t count, DeviceProfile *prof, hipStream_t stream) {
-    dim3 grid = tile_grid(hcfg.width, hcfg.height);
-    const bool mb = hcfg.sdf_id == FPT_SDF_MANDELBULBER;
-    if (prof && mb)
-        hipLaunchKernelGGL((sdf_accumulate_kernel<true, SCENE_MB>), grid, dim3(256), 0, stream, dcfg, accum, start,
-                           count, prof);
-    else if (prof)
-        hipLaunchKernelGGL((sdf_accumulate_kernel<true, SCENE_SDF>), grid, dim3(256), 0, stream, dcfg, accum, start,
-                           count, prof);
-    else if (mb)
-        hipLaunchKernelGGL((sdf_accumulate_kernel<false, SCENE_MB>), grid, dim3(256), 0, stream, dcfg, accum, start,
-                           count, nullptr);
-    else
-        hipLaunchKernelGGL((sdf_accumulate_kernel<false, SCENE_SDF>), grid, dim3(256), 0, stream, dcfg, accum, start,
-                           count, nullptr);
+    const dim3 grid = tile_grid(hcfg.width, hcfg.height), block(256);
+    const int kind = hcfg.sdf_id == FPT_SDF_MANDELBULBER ? SCENE_MB
+                     : has_lowered_tables(hcfg)           ? SCENE_LOWERED
+                                                          : SCENE_SDF;
+#define FPT_ACCUMULATE(K)                                                                                           \
+    if (prof)                                                                                                       \
+        hipLaunchKernelGGL((sdf_accumulate_kernel<true, K>), grid, block, 0, stream, dcfg, accum, start, count, prof); \
+    else                                                                                                            \
+        hipLaunchKernelGGL((sdf_accumulate_kernel<false, K>), grid, block, 0, stream, dcfg, accum, start, count,     \
+                           nullptr)
+    if (kind == SCENE_MB) {
+        FPT_ACCUMULATE(SCENE_MB);
+    } else if (kind == SCENE_LOWERED) {
+        FPT_ACCUMULATE(SCENE_LOWERED);
+    } else {
+        FPT_ACCUMULATE(SCENE_SDF);
+    }
+#undef FPT_ACCUMULATE
     return hipGetLastError();
 }
 
@@ -606,6 +667,12 @@ hipError_t launch_field_sample(const FptRenderConfig *dcfg, const float4 *pts, s
                                FptMandelbulberFieldSample *out, hipStream_t stream) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(field_sample_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream, dcfg, pts, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_typed_soa_validation(const FptRenderConfig *dcfg, uint32_t n, uint32_t *counts,
+                                       hipStream_t stream) {
+    hipLaunchKernelGGL(typed_soa_validation_kernel, dim3((n + 255u) / 256u), dim3(256), 0, stream, dcfg, n, counts);
     return hipGetLastError();
 }
 

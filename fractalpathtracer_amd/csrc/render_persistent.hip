@@ -18,6 +18,39 @@ bool single_bulb_program(const FptRenderConfig &c) {
            c.sdf_program[0].opcode == FPT_SDF_OP_MANDELBULB;
 }
 
+bool has_lowered_tables(const FptRenderConfig &c) {  // the dispatch condition of SM:3576-3584
+    if (c.sdf_id != FPT_SDF_PROGRAM || c.sdf_program_count == 0u) return false;
+    const FptTypedSoAProgram &t = c.sdf_typed_soa;
+    return t.sphere_count + t.box_count + t.plane_count > 0u || c.sdf_canonical_count > 0u ||
+           c.sdf_flat_union_count > 0u;
+}
+
+namespace {
+
+template <bool PROFILE, bool MULTI, bool LEARNED, bool BULB, bool MB, bool LOWERED>
+void launch_instance(dim3 grid, hipStream_t stream, const FptRenderConfig *cfgs, const BatchScene *scenes,
+                     uint32_t nscenes, float4 *accum, uint32_t start, uint32_t count, uint32_t *counter,
+                     uint32_t slots, int event_batch, DeviceProfile *prof, const ProposerDevice &prop) {
+    hipLaunchKernelGGL((sdf_persistent_kernel<PROFILE, MULTI, LEARNED, BULB, MB, LOWERED>), grid, dim3(256), 0,
+                       stream, cfgs, scenes, nscenes, accum, start, count, counter, slots, event_batch,
+                       PROFILE ? prof : nullptr, prop);
+}
+
+template <bool MULTI, bool LEARNED, bool BULB, bool MB, bool LOWERED>
+void launch_profiled(bool profiled, dim3 grid, hipStream_t stream, const FptRenderConfig *cfgs,
+                     const BatchScene *scenes, uint32_t nscenes, float4 *accum, uint32_t start, uint32_t count,
+                     uint32_t *counter, uint32_t slots, int event_batch, DeviceProfile *prof,
+                     const ProposerDevice &prop) {
+    if (profiled)
+        launch_instance<true, MULTI, LEARNED, BULB, MB, LOWERED>(grid, stream, cfgs, scenes, nscenes, accum, start,
+                                                                  count, counter, slots, event_batch, prof, prop);
+    else
+        launch_instance<false, MULTI, LEARNED, BULB, MB, LOWERED>(grid, stream, cfgs, scenes, nscenes, accum, start,
+                                                                   count, counter, slots, event_batch, prof, prop);
+}
+
+}  // namespace
+
 hipError_t launch_sdf_persistent(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, float4 *accum,
                                  uint32_t start, uint32_t count, uint32_t *counter, uint32_t workgroups,
                                  int event_batch, DeviceProfile *prof, const ProposerDevice *prop,
@@ -25,49 +58,34 @@ hipError_t launch_sdf_persistent(const FptRenderConfig *dcfg, const FptRenderCon
     const uint32_t slots = slot_count(hcfg.width, hcfg.height);
     hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, counter);
     const ProposerDevice none{};
-    const dim3 grid(workgroups), block(256);
-    if (prop) {
-        if (prof)
-            hipLaunchKernelGGL((sdf_persistent_kernel<true, false, true>), grid, block, 0, stream, dcfg, nullptr, 1u,
-                               accum, start, count, counter, slots, event_batch, prof, *prop);
-        else
-            hipLaunchKernelGGL((sdf_persistent_kernel<false, false, true>), grid, block, 0, stream, dcfg, nullptr, 1u,
-                               accum, start, count, counter, slots, event_batch, nullptr, *prop);
-    } else if (hcfg.sdf_id == FPT_SDF_MANDELBULBER) {
-        if (prof)
-            hipLaunchKernelGGL((sdf_persistent_kernel<true, false, false, false, true>), grid, block, 0, stream, dcfg,
-                               nullptr, 1u, accum, start, count, counter, slots, event_batch, prof, none);
-        else
-            hipLaunchKernelGGL((sdf_persistent_kernel<false, false, false, false, true>), grid, block, 0, stream,
-                               dcfg, nullptr, 1u, accum, start, count, counter, slots, event_batch, nullptr, none);
-    } else if (single_bulb_program(hcfg)) {
-        if (prof)
-            hipLaunchKernelGGL((sdf_persistent_kernel<true, false, false, true>), grid, block, 0, stream, dcfg, nullptr,
-                               1u, accum, start, count, counter, slots, event_batch, prof, none);
-        else
-            hipLaunchKernelGGL((sdf_persistent_kernel<false, false, false, true>), grid, block, 0, stream, dcfg,
-                               nullptr, 1u, accum, start, count, counter, slots, event_batch, nullptr, none);
-    } else if (prof) {
-        hipLaunchKernelGGL((sdf_persistent_kernel<true, false, false>), grid, block, 0, stream, dcfg, nullptr, 1u,
-                           accum, start, count, counter, slots, event_batch, prof, none);
-    } else {
-        hipLaunchKernelGGL((sdf_persistent_kernel<false, false, false>), grid, block, 0, stream, dcfg, nullptr, 1u,
-                           accum, start, count, counter, slots, event_batch, nullptr, none);
-    }
+    const dim3 grid(workgroups);
+    const bool p = prof != nullptr, lowered = has_lowered_tables(hcfg);
+#define FPT_LAUNCH(LEARNED, BULB, MB, LOWERED, PROP)                                                              \
+    launch_profiled<false, LEARNED, BULB, MB, LOWERED>(p, grid, stream, dcfg, nullptr, 1u, accum, start, count,  \
+                                                       counter, slots, event_batch, prof, PROP)
+    if (prop && lowered) FPT_LAUNCH(true, false, false, true, *prop);
+    else if (prop) FPT_LAUNCH(true, false, false, false, *prop);
+    else if (hcfg.sdf_id == FPT_SDF_MANDELBULBER) FPT_LAUNCH(false, false, true, false, none);
+    else if (single_bulb_program(hcfg)) FPT_LAUNCH(false, true, false, false, none);
+    else if (lowered) FPT_LAUNCH(false, false, false, true, none);
+    else FPT_LAUNCH(false, false, false, false, none);
+#undef FPT_LAUNCH
     return hipGetLastError();
 }
 
 hipError_t launch_sdf_batch(const FptRenderConfig *dcfgs, const BatchScene *dscenes, uint32_t nscenes,
                             uint32_t total_slots, float4 *accum, uint32_t *counter, uint32_t workgroups,
-                            int event_batch, uint32_t spp_cap, DeviceProfile *prof, hipStream_t stream) {
+                            int event_batch, uint32_t spp_cap, DeviceProfile *prof, bool lowered,
+                            hipStream_t stream) {
     hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, counter);
     const ProposerDevice none{};
-    if (prof)
-        hipLaunchKernelGGL((sdf_persistent_kernel<true, true, false>), dim3(workgroups), dim3(256), 0, stream, dcfgs,
-                           dscenes, nscenes, accum, 0u, spp_cap, counter, total_slots, event_batch, prof, none);
+    const bool p = prof != nullptr;
+    if (lowered)
+        launch_profiled<true, false, false, false, true>(p, dim3(workgroups), stream, dcfgs, dscenes, nscenes, accum,
+                                                         0u, spp_cap, counter, total_slots, event_batch, prof, none);
     else
-        hipLaunchKernelGGL((sdf_persistent_kernel<false, true, false>), dim3(workgroups), dim3(256), 0, stream, dcfgs,
-                           dscenes, nscenes, accum, 0u, spp_cap, counter, total_slots, event_batch, nullptr, none);
+        launch_profiled<true, false, false, false, false>(p, dim3(workgroups), stream, dcfgs, dscenes, nscenes, accum,
+                                                          0u, spp_cap, counter, total_slots, event_batch, prof, none);
     return hipGetLastError();
 }
 

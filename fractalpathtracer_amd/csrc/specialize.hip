@@ -240,7 +240,7 @@ int specialize(const FptRenderConfig &cfg, std::shared_ptr<SpecializedModule> &o
         const std::string arch_opt = "--offload-arch=" + arch.substr(0, arch.find(':'));
         std::vector<const char *> opts = {arch_opt.c_str()};
         for (const char *o : kOptions) opts.push_back(o);
-        const char *kname = "&fpt::sdf_persistent_kernel<false, false, false, false, false>";
+        const char *kname = "&fpt::sdf_persistent_kernel<false, false, false, false, false, false>";
         hiprtcAddNameExpression(prog, kname);
         const hiprtcResult r = hiprtcCompileProgram(prog, int(opts.size()), opts.data());
         if (r != HIPRTC_SUCCESS) {
@@ -274,7 +274,7 @@ int specialize(const FptRenderConfig &cfg, std::shared_ptr<SpecializedModule> &o
     }
     // The kernel's mangled name does not depend on the topology.
     if (hipModuleGetFunction(&mod->kernel, mod->module,
-                             "_ZN3fpt21sdf_persistent_kernelILb0ELb0ELb0ELb0ELb0EEEvPK15FptRenderConfigPKNS_10BatchSceneEjP15HIP_"
+                             "_ZN3fpt21sdf_persistent_kernelILb0ELb0ELb0ELb0ELb0ELb0EEEvPK15FptRenderConfigPKNS_10BatchSceneEjP15HIP_"
                              "vector_typeIfLj4EEjjPjjiPNS_13DeviceProfileENS_14ProposerDeviceE") != hipSuccess ||
         hipModuleGetFunction(&mod->validate, mod->module, "fpt_topology_validate") != hipSuccess) {
         (void)hipModuleUnload(mod->module);

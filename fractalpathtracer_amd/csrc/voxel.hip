@@ -294,7 +294,7 @@ __device__ bool refine_secant_bisection(f3 o, f3 d, float t0, float t1, const Fp
                                         float &out) {
     if (!isfinite(t0) || !isfinite(t1) || !(t1 > t0)) return false;
     const float tol = 1.0e-6f;
-    float pt = t0, pd = map_sdf(o + d * pt, cfg, w);
+    float pt = t0, pd = map_scene(o + d * pt, cfg, w);
     if (!isfinite(pd)) return false;
     if (fabsf(pd) <= tol) {
         out = pt;
@@ -304,7 +304,7 @@ __device__ bool refine_secant_bisection(f3 o, f3 d, float t0, float t1, const Fp
     bool bracket = false;
     for (uint32_t k = 1u; k <= 4u; ++k) {
         const float ct = mixf(t0, t1, float(k) / 4.0f);
-        const float cd = map_sdf(o + d * ct, cfg, w);
+        const float cd = map_scene(o + d * ct, cfg, w);
         if (!isfinite(cd)) return false;
         if (fabsf(cd) <= tol) {
             out = ct;
@@ -329,7 +329,7 @@ __device__ bool refine_secant_bisection(f3 o, f3 d, float t0, float t1, const Fp
                 if (st > lo_t + guard && st < hi_t - guard) ct = st;
             }
         }
-        const float cd = map_sdf(o + d * ct, cfg, w);
+        const float cd = map_scene(o + d * ct, cfg, w);
         if (!isfinite(cd)) return false;
         if (fabsf(cd) <= tol) {
             out = ct;
@@ -358,7 +358,7 @@ __device__ bool refine_restricted_trace(f3 o, f3 d, float t0, float t1, f3 cente
     const float tol = fmaxf(1.0e-6f, (t1 - t0) * 1.0e-4f);
     const float inv_l = 1.0f / fmaxf(b.lipschitz, 1.0f);
     for (uint32_t k = 0u; k < 12u; ++k) {
-        const float dist = map_sdf(o + d * t, cfg, w);
+        const float dist = map_scene(o + d * t, cfg, w);
         if (!isfinite(dist)) return false;
         if (fabsf(dist) <= tol) {
             out = clampf(t, t0, t1);
@@ -366,7 +366,7 @@ __device__ bool refine_restricted_trace(f3 o, f3 d, float t0, float t1, f3 cente
         }
         const float nt = t + fmaxf(fabsf(dist) * inv_l * 0.99f, tol * 0.5f);
         if (!(nt < t1)) {
-            const float de = map_sdf(o + d * t1, cfg, w);
+            const float de = map_scene(o + d * t1, cfg, w);
             if (isfinite(de) && fabsf(de) <= tol) {
                 out = t1;
                 return true;
@@ -388,7 +388,7 @@ __device__ bool refine_fixed_de(f3 o, f3 d, float t0, float t1, const FptRenderC
     float best_t = t0, best = kInf;
     for (uint32_t k = 0u; k <= 12u; ++k) {
         const float t = t0 + float(k) * step;
-        const float dist = fabsf(map_sdf(o + d * t, cfg, w));
+        const float dist = fabsf(map_scene(o + d * t, cfg, w));
         if (!isfinite(dist)) return false;
         if (dist < best) {
             best = dist;
@@ -398,7 +398,7 @@ __device__ bool refine_fixed_de(f3 o, f3 d, float t0, float t1, const FptRenderC
     float lo = fmaxf(t0, best_t - step), hi = fminf(t1, best_t + step);
     for (uint32_t k = 0u; k < 3u; ++k) {
         const float lt = mixf(lo, hi, 1.0f / 3.0f), rt = mixf(lo, hi, 2.0f / 3.0f);
-        const float ld = fabsf(map_sdf(o + d * lt, cfg, w)), rd = fabsf(map_sdf(o + d * rt, cfg, w));
+        const float ld = fabsf(map_scene(o + d * lt, cfg, w)), rd = fabsf(map_scene(o + d * rt, cfg, w));
         if (!isfinite(ld) || !isfinite(rd)) return false;
         if (ld < best) {
             best = ld;
@@ -488,7 +488,7 @@ __global__ __launch_bounds__(256) void voxel_stage_kernel(const FptRenderConfig 
     const f3 cs = cell_size_of(cfg);
     const f3 c = cell_center(x, y, z, ld3(cfg.voxel_bounds_min), cs);
     Work w = {0, 0};
-    const float d = distance_sdf(c, cfg, w);
+    const float d = distance_scene(c, cfg, w);
     bool o = classify_occupied(d, length(cs * 0.5f), cfg);
     const uint32_t cov = cfg.voxel_coverage_mode;
     if (cov == FPT_VOXEL_COVERAGE_LIPSCHITZ || cov == FPT_VOXEL_COVERAGE_INTERVAL) {  // SM:5388-5398
@@ -784,8 +784,8 @@ __device__ f3 render_voxel_path(UV xy, uint32_t sample, float focus, const FptRe
         Material mat = FIELD ? hit_material(h, cfg, v, w) : table_material(h, v);
         if (mat.emission > 0.001f) light = light + mat.rgb * mat.emission;
         f3 n = !FIELD ? h.normal
-                      : (cfg.voxel_normal_mode == 1u ? normal_at(h.cell_center, cfg, w)
-                                                     : (cfg.voxel_normal_mode == 2u ? normal_at(h.position, cfg, w)
+                      : (cfg.voxel_normal_mode == 1u ? normal_scene(h.cell_center, cfg, w)
+                                                     : (cfg.voxel_normal_mode == 2u ? normal_scene(h.position, cfg, w)
                                                                                     : h.normal));
         if (dot(n, dir) > 0.0f) n = -n;
         f3 on = cfg.voxel_normal_mode == 2u ? h.normal : n;
@@ -958,7 +958,7 @@ __device__ bool bounds_contain_grid(f3 c, f3 h, const FptRenderConfig &cfg) {
         for (int y = 0; y < 3; ++y)
             for (int x = 0; x < 3; ++x) {
                 const f3 u = mk(float(x) - 1.0f, float(y) - 1.0f, float(z) - 1.0f);
-                const float d = distance_sdf(c + mk(u.x * h.x, u.y * h.y, u.z * h.z), cfg, w);
+                const float d = distance_scene(c + mk(u.x * h.x, u.y * h.y, u.z * h.z), cfg, w);
                 if (!isfinite(d) || d < b.minimum - 2.0e-5f || d > b.maximum + 2.0e-5f) return false;
             }
     return true;
@@ -1004,7 +1004,7 @@ __global__ void voxel_dda_contract_kernel(uint32_t *failures_out, const FptRende
     if (CELL_OUTSIDE != 0u || CELL_INSIDE != 1u || CELL_SURFACE != 2u || CELL_UNKNOWN != 3u) f |= 1u << 10;
     auto cls = [&](f3 c, f3 h, const FptRenderConfig &cfg) {
         Work w = {0, 0};
-        return classify_cell(c, h, distance_sdf(c, cfg, w), cfg);
+        return classify_cell(c, h, distance_scene(c, cfg, w), cfg);
     };
     const f3 q = mk(0.25f, 0.25f, 0.25f), z0 = mk(0.f, 0.f, 0.f), t1 = mk(0.1f, 0.1f, 0.1f);
     const ProgramBounds sb = program_bounds(z0, q, pc[0]);
@@ -1273,8 +1273,8 @@ __device__ f3 voxel_diagnostic(UV xy, const FptRenderConfig &cfg, const FptDiagn
         return mk(k, k, k);
     }
     if (!h.hit) return mk(0.f, 0.f, 0.f);
-    f3 n = cfg.voxel_normal_mode == 1u ? normal_at(h.cell_center, cfg, w)
-                                        : (cfg.voxel_normal_mode == 2u ? normal_at(h.position, cfg, w) : h.normal);
+    f3 n = cfg.voxel_normal_mode == 1u ? normal_scene(h.cell_center, cfg, w)
+                                        : (cfg.voxel_normal_mode == 2u ? normal_scene(h.position, cfg, w) : h.normal);
     if (dot(n, dir) > 0.0f) n = -n;
     if (mode == FPT_DIAGNOSTIC_DIFFUSE_NORMAL) {
         const f3 fwd = rotate_camera_roll(mk(0.f, 0.f, 1.f), cfg.camera_yaw_pitch[0], cfg.camera_yaw_pitch[1],

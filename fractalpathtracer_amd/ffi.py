@@ -325,6 +325,8 @@ def _declare(lib):
     lib.fpt_metal_device_name.restype = C.c_int
     lib.fpt_test_voxel_dda.argtypes = [cp, cp, C.c_size_t]
     lib.fpt_test_voxel_dda.restype = C.c_int
+    lib.fpt_test_typed_soa.argtypes = [cp, P(FptRenderConfig), P(FptStitchValidationStats), cp, C.c_size_t]
+    lib.fpt_test_typed_soa.restype = C.c_int
     lib.fpt_hip_trace_primary.argtypes = [P(FptRenderConfig), P(FptRayHit), C.c_size_t, cp,
                                           C.c_size_t]
     lib.fpt_hip_trace_primary.restype = C.c_int

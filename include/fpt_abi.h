@@ -609,9 +609,21 @@ int fpt_metal_device_name(char *name, size_t name_len);
 /* Voxel DDA known-answer contract (metal_bridge.mm:3715-3926). */
 int fpt_test_voxel_dda(const char *metallib_path, char *error, size_t error_len);
 
-/* The three entry points below exist for link compatibility; the Metal
- * features behind them (function stitching, typed SoA lowering, AppKit
- * preview) are not part of this library and they return 1 with a message. */
+/* Typed structure-of-arrays validator (metal_bridge.mm:4116-4203,
+ * Shaders.metal:5106-5154): the config's typed table against its
+ * interpreted program at 2^20 hashed points of the voxel bounds; stats are
+ * filled, and 1 is returned with the failure counts when any point misses
+ * 2e-5 (1 + |d|) in distance or 1e-4 in gradient.  metallib_path is
+ * ignored. */
+int fpt_test_typed_soa(const char *metallib_path,
+                       const struct FptRenderConfig *config,
+                       struct FptStitchValidationStats *stats,
+                       char *error,
+                       size_t error_len);
+
+/* The two entry points below exist for link compatibility; the Metal
+ * features behind them (function stitching, AppKit preview) are not part
+ * of this library and they return 1 with a message. */
 int fpt_test_async_stitch_context(const char *metallib_path,
                                   const char *stitch_metallib_path,
                                   const char *stitch_archive_path,
@@ -619,12 +631,6 @@ int fpt_test_async_stitch_context(const char *metallib_path,
                                   struct FptAsyncJitStats *stats,
                                   char *error,
                                   size_t error_len);
-
-int fpt_test_typed_soa(const char *metallib_path,
-                       const struct FptRenderConfig *config,
-                       struct FptStitchValidationStats *stats,
-                       char *error,
-                       size_t error_len);
 
 int fpt_metal_preview(const char *metallib_path,
                       const char *stitch_metallib_path,

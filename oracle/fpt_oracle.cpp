@@ -16,6 +16,12 @@
 //   program interpreter    closed-form sphere/box/plane/CSG distances of
 //                          scenes/benchmarks/Exact_*.json.
 //   program gradient       FP64 central-difference limit (brute force).
+//   lowered evaluators     (tests/test_lowered_programs.py) the interpreter
+//                          of the same program on every bundled scene each
+//                          lowering accepts (FP64 distances bit-identical);
+//                          a plane/sphere tie for the typed table's
+//                          earlier-instruction rule; union, intersection
+//                          and subtraction of a canonical list in numpy.
 //   march                  invariants: hit => 0.99|d| < eps at the last step;
 //                          analytic hit t for a sphere at the origin.
 //   render                 FP32 twin vs FP64 (agreement), sphere scene
@@ -877,9 +883,173 @@ T de_kifs(V3<T> p, const FptRenderConfig &cfg, T *orbit) {
     return d;
 }
 
+// ---------------------------------------------------------------------------
+// Direct evaluators of lowered programs (SM:2713-2970): typed
+// structure-of-arrays, canonical affine and flat-union tables.
+// ---------------------------------------------------------------------------
+inline uint32_t lowered_cap(uint32_t n) { return std::min<uint32_t>(n, FPT_SDF_FLAT_UNION_MAX_PRIMITIVES); }
+inline uint32_t typed_soa_count(const FptRenderConfig &cfg) {  // SM:2848-2855
+    const FptTypedSoAProgram &t = cfg.sdf_typed_soa;
+    return lowered_cap(t.sphere_count) + lowered_cap(t.box_count) + lowered_cap(t.plane_count);
+}
+
+template <class T>
+V3<T> instance_point(V3<T> s, const float *t) {
+    return V3<T>(dot(V3<T>(T(t[0]), T(t[1]), T(t[2])), s) + T(t[3]),
+                 dot(V3<T>(T(t[4]), T(t[5]), T(t[6])), s) + T(t[7]),
+                 dot(V3<T>(T(t[8]), T(t[9]), T(t[10])), s) + T(t[11]));
+}
+
+template <class T>
+T instance_distance(V3<T> s, const FptPrimitiveInstance &in) {  // SM:2713-2737
+    const V3<T> p = instance_point(s, in.transform);
+    const T div = m_max(T(in.distance_scale), T(1.0e-6f));
+    const V3<T> data(T(in.data[0]), T(in.data[1]), T(in.data[2]));
+    if (in.opcode == FPT_SDF_OP_SPHERE) return (length(p) - data.x) / div;
+    if (in.opcode == FPT_SDF_OP_BOX) {
+        const V3<T> q = vabs(p) - vabs(data);
+        const V3<T> o(m_max(q.x, T(0)), m_max(q.y, T(0)), m_max(q.z, T(0)));
+        return (m_min(m_max(q.x, m_max(q.y, q.z)), T(0)) + length(o)) / div;
+    }
+    if (in.opcode == FPT_SDF_OP_PLANE) return (dot(p, data) + T(in.data[3])) / div;
+    return T(1.0e20);
+}
+
+template <class T>
+T flat_union_distance(V3<T> s, const FptRenderConfig &cfg) {  // SM:2739-2749
+    T d = T(1.0e20);
+    for (uint32_t i = 0, n = lowered_cap(cfg.sdf_flat_union_count); i < n; ++i)
+        d = m_min(d, instance_distance(s, cfg.sdf_flat_union_instances[i]));
+    return d;
+}
+
+template <class T>
+T canonical_distance(V3<T> s, const FptRenderConfig &cfg) {  // SM:2751-2770: _pad0 0 union, 1 and, 2 minus
+    T d = T(1.0e20);
+    for (uint32_t i = 0, n = lowered_cap(cfg.sdf_canonical_count); i < n; ++i) {
+        const FptPrimitiveInstance &in = cfg.sdf_canonical_primitives[i];
+        const T c = instance_distance(s, in);
+        if (i == 0u) d = c;
+        else if (in._pad0 == 1u) d = m_max(d, c);
+        else if (in._pad0 == 2u) d = m_max(d, -c);
+        else d = m_min(d, c);
+    }
+    return d;
+}
+
+template <class T>
+T canonical_surface(V3<T> s, const FptRenderConfig &cfg, V3<T> &grad) {  // SM:2772-2846
+    T d = T(1.0e20);
+    grad = V3<T>();
+    for (uint32_t i = 0, n = lowered_cap(cfg.sdf_canonical_count); i < n; ++i) {
+        const FptPrimitiveInstance &in = cfg.sdf_canonical_primitives[i];
+        const float *t = in.transform;
+        const V3<T> r0 = V3<T>(T(t[0]), T(t[1]), T(t[2]));
+        const V3<T> r1 = V3<T>(T(t[4]), T(t[5]), T(t[6]));
+        const V3<T> r2 = V3<T>(T(t[8]), T(t[9]), T(t[10]));
+        const V3<T> p(dot(r0, s) + T(t[3]), dot(r1, s) + T(t[7]), dot(r2, s) + T(t[11]));
+        const T div = m_max(T(in.distance_scale), T(1.0e-6f));
+        T c = T(1.0e20);
+        V3<T> lg(T(0), T(1), T(0));
+        if (in.opcode == FPT_SDF_OP_SPHERE) {
+            const T r = length(p);
+            c = (r - T(in.data[0])) / div;
+            if (r > T(1.0e-8f)) lg = p / r;
+        } else if (in.opcode == FPT_SDF_OP_BOX) {
+            const V3<T> q = vabs(p) - vabs(V3<T>(T(in.data[0]), T(in.data[1]), T(in.data[2])));
+            const V3<T> o(m_max(q.x, T(0)), m_max(q.y, T(0)), m_max(q.z, T(0)));
+            const T ol = length(o);
+            c = (m_min(m_max(q.x, m_max(q.y, q.z)), T(0)) + ol) / div;
+            if (ol > T(1.0e-8f)) lg = V3<T>(m_sign(p.x), m_sign(p.y), m_sign(p.z)) * o / ol;
+            else if (q.x >= q.y && q.x >= q.z) lg = V3<T>(m_sign(p.x), T(0), T(0));
+            else if (q.y >= q.z) lg = V3<T>(T(0), m_sign(p.y), T(0));
+            else lg = V3<T>(T(0), T(0), m_sign(p.z));
+        } else if (in.opcode == FPT_SDF_OP_PLANE) {
+            lg = V3<T>(T(in.data[0]), T(in.data[1]), T(in.data[2]));
+            c = (dot(p, lg) + T(in.data[3])) / div;
+        }
+        const V3<T> cg = (r0 * lg.x + r1 * lg.y + r2 * lg.z) / div;
+        const bool choose = i == 0u || (in._pad0 == 1u ? c > d : (in._pad0 == 2u ? -c > d : c < d));
+        if (choose) {
+            const bool neg = in._pad0 == 2u && i > 0u;
+            d = neg ? -c : c;
+            grad = neg ? -cg : cg;
+        }
+    }
+    return d;
+}
+
+template <class T>
+T typed_soa_distance(V3<T> s, const FptRenderConfig &cfg) {  // SM:2857-2896
+    const FptTypedSoAProgram &t = cfg.sdf_typed_soa;
+    T d = T(1.0e20);
+    for (uint32_t i = 0, n = lowered_cap(t.sphere_count); i < n; ++i)
+        d = m_min(d, length(s - V3<T>(T(t.sphere_x[i]), T(t.sphere_y[i]), T(t.sphere_z[i]))) - T(t.sphere_radius[i]));
+    for (uint32_t i = 0, n = lowered_cap(t.box_count); i < n; ++i) {
+        const V3<T> q = vabs(s - V3<T>(T(t.box_x[i]), T(t.box_y[i]), T(t.box_z[i]))) -
+                        V3<T>(T(t.box_half_x[i]), T(t.box_half_y[i]), T(t.box_half_z[i]));
+        const V3<T> o(m_max(q.x, T(0)), m_max(q.y, T(0)), m_max(q.z, T(0)));
+        d = m_min(d, m_min(m_max(q.x, m_max(q.y, q.z)), T(0)) + length(o));
+    }
+    for (uint32_t i = 0, n = lowered_cap(t.plane_count); i < n; ++i) {
+        const V3<T> nrm = normalize(V3<T>(T(t.plane_x[i]), T(t.plane_y[i]), T(t.plane_z[i])));
+        const V3<T> c(T(t.plane_center_x[i]), T(t.plane_center_y[i]), T(t.plane_center_z[i]));
+        d = m_min(d, dot(s - c, nrm) + T(t.plane_offset[i]));
+    }
+    return d;
+}
+
+template <class T>
+T typed_soa_surface(V3<T> s, const FptRenderConfig &cfg, V3<T> &grad) {  // SM:2898-2970: ties to the earlier source
+    const FptTypedSoAProgram &t = cfg.sdf_typed_soa;
+    T d = T(1.0e20);
+    grad = V3<T>();
+    uint32_t winner = 0xffffffffu;
+    for (uint32_t i = 0, n = lowered_cap(t.sphere_count); i < n; ++i) {
+        const V3<T> l = s - V3<T>(T(t.sphere_x[i]), T(t.sphere_y[i]), T(t.sphere_z[i]));
+        const T r = length(l), c = r - T(t.sphere_radius[i]);
+        if (c < d || (c == d && t.sphere_source[i] < winner)) {
+            d = c;
+            grad = r > T(1.0e-8f) ? l / r : V3<T>(T(0), T(1), T(0));
+            winner = t.sphere_source[i];
+        }
+    }
+    for (uint32_t i = 0, n = lowered_cap(t.box_count); i < n; ++i) {
+        const V3<T> l = s - V3<T>(T(t.box_x[i]), T(t.box_y[i]), T(t.box_z[i]));
+        const V3<T> q = vabs(l) - V3<T>(T(t.box_half_x[i]), T(t.box_half_y[i]), T(t.box_half_z[i]));
+        const V3<T> o(m_max(q.x, T(0)), m_max(q.y, T(0)), m_max(q.z, T(0)));
+        const T ol = length(o);
+        const T c = m_min(m_max(q.x, m_max(q.y, q.z)), T(0)) + ol;
+        if (c < d || (c == d && t.box_source[i] < winner)) {
+            d = c;
+            if (ol > T(1.0e-8f)) grad = V3<T>(m_sign(l.x), m_sign(l.y), m_sign(l.z)) * o / ol;
+            else if (q.x >= q.y && q.x >= q.z) grad = V3<T>(m_sign(l.x), T(0), T(0));
+            else if (q.y >= q.z) grad = V3<T>(T(0), m_sign(l.y), T(0));
+            else grad = V3<T>(T(0), T(0), m_sign(l.z));
+            winner = t.box_source[i];
+        }
+    }
+    for (uint32_t i = 0, n = lowered_cap(t.plane_count); i < n; ++i) {
+        const V3<T> nrm = normalize(V3<T>(T(t.plane_x[i]), T(t.plane_y[i]), T(t.plane_z[i])));
+        const V3<T> c0(T(t.plane_center_x[i]), T(t.plane_center_y[i]), T(t.plane_center_z[i]));
+        const T c = dot(s - c0, nrm) + T(t.plane_offset[i]);
+        if (c < d || (c == d && t.plane_source[i] < winner)) {
+            d = c;
+            grad = nrm;
+            winner = t.plane_source[i];
+        }
+    }
+    return d;
+}
+
 template <class T>
 T distance_sdf(V3<T> p, const FptRenderConfig &cfg) {  // SM:3564-3610
     if (tl_counters) tl_counters->distance_evals++;
+    if (is_program<T>(cfg)) {  // lowered tables first (SM:3576-3584)
+        if (typed_soa_count(cfg) > 0u) return typed_soa_distance(p, cfg);
+        if (cfg.sdf_canonical_count > 0u) return canonical_distance(p, cfg);
+        if (cfg.sdf_flat_union_count > 0u) return flat_union_distance(p, cfg);
+    }
     if (is_program<T>(cfg)) return program_eval<T>(geometry_program(cfg), p, nullptr);
     switch (cfg.sdf_id) {
         case FPT_SDF_MANDELBULBER: return de_kifs<T>(p, cfg, nullptr);
@@ -1172,8 +1342,10 @@ V3<T> normal_at(V3<T> p, const FptRenderConfig &cfg) {
     if (cfg.sdf_id == FPT_SDF_MANDELBULBER) return normal_mandelbulber(p, cfg);
     T e = m_max(T(cfg.render[2]), T(0.0002f));
     if ((cfg.sdf_normal_mode == 0u || cfg.sdf_normal_mode == 2u) && is_program<T>(cfg)) {
-        V3<T> g;
-        program_surface<T>(geometry_program(cfg), p, g);
+        V3<T> g;  // programSurface (SM:3403-3408): typed SoA, canonical, else interpreted
+        if (typed_soa_count(cfg) > 0u) typed_soa_surface<T>(p, cfg, g);
+        else if (cfg.sdf_canonical_count > 0u) canonical_surface<T>(p, cfg, g);
+        else program_surface<T>(geometry_program(cfg), p, g);
         if (dot(g, g) >= T(1.0e-12f) && vfinite(g)) return normalize(g);
     }
     if (cfg.sdf_normal_mode == 1u) {
