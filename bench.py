@@ -105,11 +105,12 @@ def parse():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=8)
     ap.add_argument("--ref-spp-factor", type=int, default=4)
-    ap.add_argument("--workload", choices=("learned", "typed", "voxel", "fract"), default="learned",
+    ap.add_argument("--workload", choices=("learned", "typed", "voxel", "fract", "lowered"), default="learned",
                     help="learned: the learned-step scene above (the default line); typed: the typed-program "
                          "scene (1280x720, 16 spp); voxel: the 1024^3 sparse-brick Mandelbox (1920x1080, 4 spp); "
                          "fract: the bundled Mandelbulber scene ifs-20.fract (Kaleidoscopic IFS, its own "
-                         "1920x1080, 64 spp)")
+                         "1920x1080, 64 spp); lowered: a 28-primitive translated union lowered to a typed "
+                         "structure-of-arrays table (1280x720, 16 spp), timed beside the interpreter")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy kernel of MIB MiB (the known-size reference for "
                          "HBM byte counters) and exit")
@@ -276,13 +277,18 @@ def one_path_launches(cfg) -> int:
 
 
 def run_workload(args, api, scenes):
-    """One JSON line for the typed-program or the voxel workload, with the
-    same contract keys as the default line (its own roofline)."""
+    """One JSON line for the typed-program, voxel, imported-scene or
+    lowered-program workload, with the same contract keys as the default
+    line (its own roofline)."""
     from fractalpathtracer_amd import ffi
     if args.workload == "typed":
         cfg = scenes.typed_program_scene()
         desc = "typed program: Menger(5) box-folded and rotated 30 deg about Y, union Mandelbox(-1.5, 10) " \
                "offset (2,0,0), 1280x720, 16 spp, 4 bounces, Russian roulette after bounce 2"
+    elif args.workload == "lowered":
+        cfg = scenes.primitive_union_scene()
+        desc = "lowered program: translated union of a ground plane, 15 spheres and 12 boxes as a typed " \
+               "structure-of-arrays table, 1280x720, 16 spp, 3 bounces"
     elif args.workload == "fract":
         cfg, _ = scenes.load_scene(ROOT / "tests" / "golden" / "mandelbulber" / "ifs-20.fract")
         desc = "Mandelbulber ifs-20.fract: Kaleidoscopic IFS (250 iterations), Mandelbulber camera and march, " \
@@ -321,6 +327,30 @@ def run_workload(args, api, scenes):
                             "launch_ms": ms / launches, "flops_per_launch": flops / launches}
         line["config"]["counts"] = {k: prof[k] for k in ("distance_evals", "march_orbit_iterations",
                                                          "normal_evals", "primary_steps", "shadow_steps")}
+    elif args.workload == "lowered":
+        prof, _ = plan.profile(accum)
+        t = cfg.sdf_typed_soa
+        # per distance evaluation of the table: sphere 11 FLOP (offset 3, dot
+        # 5, sqrt, radius, min), box 23 (offset 3, abs 3, extent 3, max/min 3,
+        # clamp 3, length 6, add, min), plane 19 (normalise 9, offset 3,
+        # dot 5, add, min)
+        per_eval = 11 * t.sphere_count + 23 * t.box_count + 19 * t.plane_count
+        flops = prof["distance_evals"] * per_eval
+        achieved = flops / (ms / 1e3) / 1e12
+        line["roofline"] = {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                            "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "kernel": KERNEL,
+                            "flops_per_launch": flops, "flops_per_distance_eval": per_eval, "lower_bound": True}
+        interp = scenes.primitive_union_scene(lowered=False)
+        with api.Plan(interp) as iplan:
+            if args.warmup:
+                api.time_renders(iplan, accum, stream, args.warmup)
+            ims = api.time_renders(iplan, accum, stream, args.steps) / args.steps
+            iprof, _ = iplan.profile(accum)
+        line["config"]["interpreted"] = {"ms_per_step": ims, "value": samples_per_step / (ims / 1e3),
+                                         "distance_evals": iprof["distance_evals"]}
+        line["config"]["speedup_vs_interpreter"] = ims / ms
+        line["config"]["counts"] = {k: prof[k] for k in ("distance_evals", "normal_evals", "primary_steps",
+                                                         "shadow_steps")}
     elif args.workload == "typed":
         prof, _ = plan.profile(accum)
         flops = (prof["march_orbit_iterations"] * TYPED_FLOPS_PER_ORBIT_ITERATION +

@@ -263,6 +263,44 @@ def typed_program_scene(width=1280, height=720, spp=16) -> FptRenderConfig:
     return c
 
 
+def primitive_union_scene(width=1280, height=720, spp=16, lowered=True) -> FptRenderConfig:
+    """Lowered-program workload: a translation-only hard union of 28
+    primitives, a ground plane y = -1, 15 spheres (radius 0.55-0.8) on a
+    5 x 3 grid and 12 boxes (half extents 0.35-0.5) on the 4 x 3 grid between
+    them; 3 bounces, analytic normals.  With `lowered` the program is lowered
+    to a typed structure-of-arrays table by the library's own pass
+    (fpt_scene_lower_typed_soa, scene.rs:1448-1530); without it the
+    interpreter evaluates the same program."""
+    c = default_config()
+    c.width, c.height, c.samples = width, height, spp
+    look_at(c, [0.0, 3.2, -6.5], [0.0, -0.2, 2.5])
+    c.camera_fov, c.camera_dof = 55.0, 0.0
+    _set(c.render, [3.0, 256.0, 0.0005, 1e-4, 100.0, 0.0, 0.0, 0.0])
+    c.sdf_normal_mode = 0
+    _c1_lighting(c)
+    _set(c.world, [3.0, 1.0, 0.0, 0.0, 0.3, 1.0, 0.0])
+    _set(c.world_one_color, [0.35, 0.45, 0.6])
+    placed = [((0.0, 0.0, 0.0), ffi.OP_PLANE, [0.0, 1.0, 0.0, 1.0])]
+    for k in range(15):
+        i, j = k % 5, k // 5
+        placed.append(((-4.0 + 2.0 * i, -0.2, 2.0 * j), ffi.OP_SPHERE, [0.55 + 0.05 * ((i + j) % 6), 0.0, 0.0, 0.0]))
+    for k in range(12):
+        i, j = k % 4, k // 4
+        h = 0.35 + 0.05 * ((i + 2 * j) % 4)
+        placed.append(((-3.0 + 2.0 * i, -1.0 + h, 1.0 + 2.0 * j), ffi.OP_BOX, [h, h, h, 0.0]))
+    ops, at = [], (0.0, 0.0, 0.0)
+    for pos, op, data in placed:  # translations accumulate through the program
+        delta = [p - q for p, q in zip(pos, at)]
+        if any(delta):
+            ops.append((ffi.OP_TRANSLATE, 0, delta + [0.0]))
+        ops.append((op, 0, data))
+        at = pos
+    set_program(c, ops)
+    if lowered and ffi.load_library().fpt_scene_lower_typed_soa(C.byref(c)) != 1:
+        raise RuntimeError("the primitive union did not lower to a typed table")
+    return c
+
+
 def learned_step_scene(width=1920, height=1080, spp=64) -> FptRenderConfig:
     """Learned-step workload scene: Mandelbulb power 8, 12 iterations,
     1920x1080, 64 spp, 4 bounces."""

@@ -140,3 +140,20 @@ def test_typed_soa_validator_refuses_configs_without_a_typed_table(tmp_path):
     m = scenes.mandelbulb_scene(8, 8)
     with pytest.raises(RuntimeError, match="requires a lowered typed program"):
         api.test_typed_soa(m)
+
+
+def test_primitive_union_workload_lowers_to_a_28_entry_table():
+    """The lowered-program workload: the library's pass turns the 55-
+    instruction program into 15 sphere, 12 box and 1 plane entries, whose
+    distances match the interpreter's (the table holds float sums of the
+    translations, the interpreter applies them one by one)."""
+    c = scenes.primitive_union_scene(64, 36, 1)
+    t = c.sdf_typed_soa
+    assert (c.sdf_program_count, t.sphere_count, t.box_count, t.plane_count) == (55, 15, 12, 1)
+    assert _counts(scenes.primitive_union_scene(64, 36, 1, lowered=False)) == (0, 0, 0)
+    i = interpreted(c)
+    pts = np.random.default_rng(2).uniform(-5, 6, (4000, 3))
+    d_l, d_i = bind.map_distance(c, pts, 64), bind.map_distance(i, pts, 64)
+    assert np.all(np.abs(d_l - d_i) <= 1e-6 * (1 + np.abs(d_i)))
+    tr = bind.trace_primary(c, 64)
+    assert np.array_equal(tr[..., 7], bind.trace_primary(i, 64)[..., 7]) and tr[..., 7].all()
