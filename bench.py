@@ -346,9 +346,20 @@ def run_workload(args, api, scenes):
                 api.time_renders(iplan, accum, stream, args.warmup)
             ims = api.time_renders(iplan, accum, stream, args.steps) / args.steps
             iprof, _ = iplan.profile(accum)
+            # the program's topology compiled by hipRTC into straight-line
+            # distance and surface code (validated at 2^20 points first)
+            t0 = time.perf_counter()
+            spec = iplan.specialize()
+            spec_s = time.perf_counter() - t0
+            if args.warmup:
+                api.time_renders(iplan, accum, stream, args.warmup)
+            sms = api.time_renders(iplan, accum, stream, args.steps) / args.steps
         line["config"]["interpreted"] = {"ms_per_step": ims, "value": samples_per_step / (ims / 1e3),
                                          "distance_evals": iprof["distance_evals"]}
         line["config"]["speedup_vs_interpreter"] = ims / ms
+        line["config"]["specialized"] = {"ms_per_step": sms, "value": samples_per_step / (sms / 1e3),
+                                         "specialize_s": spec_s, "cache_status": spec.get("cache_status"),
+                                         "typed_table_speedup_over_specialized": sms / ms}
         line["config"]["counts"] = {k: prof[k] for k in ("distance_evals", "normal_evals", "primary_steps",
                                                          "shadow_steps")}
     elif args.workload == "typed":
