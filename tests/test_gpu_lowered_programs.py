@@ -16,6 +16,8 @@ Bars, as for the other SDF scenes:
                  switch distance renders the exact image;
   validator      fpt_test_typed_soa's failure count matches the oracle's at
                  the same hashed points within 0.2%.
+  voxels         tables built from lowered scenes are bit-exact against the
+                 FP32 oracle.
 """
 import numpy as np
 import pytest
@@ -162,3 +164,25 @@ def test_typed_soa_validator_flags_a_partial_automatic_table(tmp_path):
     assert c.sdf_typed_soa.box_count == 1
     ok, st, msg = api.test_typed_soa(c)
     assert not ok and st["distance_failures"] > 0
+
+
+@pytest.mark.parametrize("option", ["flat_union", "canonical_ir", "typed_soa"])
+def test_voxel_tables_of_a_lowered_scene_are_bit_exact(tmp_path, option):
+    """The voxel builder samples the lowered tables, on the GPU and in the
+    FP32 oracle alike: page table, material ids and FP16 distances of a
+    40^3 sparse 8^3-brick grid are bit-identical, and the voxel render of
+    the lowered scene is finite."""
+    from fractalpathtracer_amd import ffi
+    c = lowered_scene(tmp_path, "Exact_Translation_Union", option)
+    c.renderer_backend = ffi.RENDERER_VOXEL
+    c.voxel_storage = ffi.VOXEL_STORAGE_SPARSE_BRICKS8
+    c.voxel_coverage_mode = 0
+    c.voxel_resolution = 40
+    c.width, c.height, c.samples = 48, 32, 1
+    g = api.voxel_build(c)
+    r = bind.voxel_build(c)
+    assert g["active_cells"] == r["active_cells"] > 0
+    assert np.array_equal(g["page_table"], r["page_table"])
+    assert np.array_equal(g["material"], r["material"])
+    assert np.array_equal(g["distance"], r["distance"])
+    assert np.all(np.isfinite(api.render(c)))
