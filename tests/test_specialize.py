@@ -61,12 +61,15 @@ def _render(cfg, specialize):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("scene", ["typed", "csg", "fold"])
+@pytest.mark.parametrize("scene", ["typed", "csg", "fold", "transforms", "union"])
 def test_specialized_render_is_bit_identical(scene):
-    cfg = {"typed": lambda: scenes.typed_program_scene(96, 64, 3),
-           "csg": lambda: _csg(),
-           "fold": lambda: scenes.apply_scene_json(scenes.default_config(),
-                                                   GOLDEN / "scenes" / "Typed_Program_Fold.json")}[scene]()
+    """"transforms" is the fixture of the reference's topology validator
+    (main.rs:4338-4385, Exact_Transforms.json)."""
+    fixture = {"fold": "Typed_Program_Fold", "transforms": "Exact_Transforms", "union": "Exact_Translation_Union"}
+    if scene in fixture:
+        cfg = scenes.apply_scene_json(scenes.default_config(), GOLDEN / "scenes" / f"{fixture[scene]}.json")
+    else:
+        cfg = {"typed": lambda: scenes.typed_program_scene(96, 64, 3), "csg": lambda: _csg()}[scene]()
     cfg.width, cfg.height, cfg.samples = 96, 64, 3
     ref, _ = _render(cfg, False)
     img, st = _render(cfg, True)
