@@ -2,6 +2,8 @@
 // persistent_kernel.h, shared with the runtime-specialised build).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "persistent_kernel.h"
 
 namespace fpt {
@@ -15,14 +17,18 @@ uint32_t persistent_slots(uint32_t width, uint32_t height) { return slot_count(w
 
 bool single_bulb_program(const FptRenderConfig &c) {
     return FPT_INTERLEAVED_BULB && c.sdf_id == FPT_SDF_PROGRAM && c.sdf_program_count == 1u &&
-           c.sdf_program[0].opcode == FPT_SDF_OP_MANDELBULB;
+           c.sdf_program[0].opcode == FPT_SDF_OP_MANDELBULB && !has_lowered_tables(c);
 }
 
-bool has_lowered_tables(const FptRenderConfig &c) {  // the dispatch condition of SM:3576-3584
+// The dispatch condition of SM:3576-3584, computed exactly as the device's
+// has_lowered(): each typed count capped at the table size before the sum,
+// so a huge count cannot wrap the sum to zero on one side only.
+bool has_lowered_tables(const FptRenderConfig &c) {
     if (c.sdf_id != FPT_SDF_PROGRAM || c.sdf_program_count == 0u) return false;
     const FptTypedSoAProgram &t = c.sdf_typed_soa;
-    return t.sphere_count + t.box_count + t.plane_count > 0u || c.sdf_canonical_count > 0u ||
-           c.sdf_flat_union_count > 0u;
+    const uint32_t cap = uint32_t(FPT_SDF_FLAT_UNION_MAX_PRIMITIVES);
+    const uint32_t typed = std::min(t.sphere_count, cap) + std::min(t.box_count, cap) + std::min(t.plane_count, cap);
+    return typed > 0u || c.sdf_canonical_count > 0u || c.sdf_flat_union_count > 0u;
 }
 
 namespace {
@@ -66,8 +72,10 @@ hipError_t launch_sdf_persistent(const FptRenderConfig *dcfg, const FptRenderCon
     if (prop && lowered) FPT_LAUNCH(true, false, false, true, *prop);
     else if (prop) FPT_LAUNCH(true, false, false, false, *prop);
     else if (hcfg.sdf_id == FPT_SDF_MANDELBULBER) FPT_LAUNCH(false, false, true, false, none);
-    else if (single_bulb_program(hcfg)) FPT_LAUNCH(false, true, false, false, none);
+    // Tables take precedence over the program (SM:3576-3584), so a config
+    // that carries them never runs the single-leaf instance.
     else if (lowered) FPT_LAUNCH(false, false, false, true, none);
+    else if (single_bulb_program(hcfg)) FPT_LAUNCH(false, true, false, false, none);
     else FPT_LAUNCH(false, false, false, false, none);
 #undef FPT_LAUNCH
     return hipGetLastError();

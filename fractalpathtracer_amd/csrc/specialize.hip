@@ -207,8 +207,17 @@ int specialize(const FptRenderConfig &cfg, std::shared_ptr<SpecializedModule> &o
         return 1;
     }
     std::string arch = prop.gcnArchName;  // e.g. "gfx950:sramecc+:xnack-"
+    // The key covers everything the code object depends on: the source, the
+    // options, the embedded headers and the compiler and driver versions (a
+    // cached object from another ROCm release is never loaded).
+    int rtc_major = 0, rtc_minor = 0, driver = 0, runtime = 0;
+    (void)hiprtcVersion(&rtc_major, &rtc_minor);
+    (void)hipDriverGetVersion(&driver);
+    (void)hipRuntimeGetVersion(&runtime);
     std::string opts_key = "--offload-arch=" + arch;
     for (const char *o : kOptions) opts_key += std::string(" ") + o;
+    opts_key += " hiprtc=" + std::to_string(rtc_major) + "." + std::to_string(rtc_minor) +
+                " driver=" + std::to_string(driver) + " runtime=" + std::to_string(runtime);
     uint64_t h = fnv1a(src);
     h = fnv1a(opts_key, h);
     for (size_t i = 0; i < kEmbeddedHeaderCount; ++i) h = fnv1a(kEmbeddedHeaderText[i], h);
@@ -225,17 +234,12 @@ int specialize(const FptRenderConfig &cfg, std::shared_ptr<SpecializedModule> &o
             return 0;
         }
     }
-    std::vector<char> code;
-    const std::string dir = cache_dir();
-    const std::string path = dir.empty() ? std::string() : dir + "/topology-" + hex + ".co";
-    bool from_disk = !path.empty() && read_file(path, code);
-    auto t0 = std::chrono::steady_clock::now();
-    if (!from_disk) {
+    auto compile = [&](std::vector<char> &code) -> bool {
         hiprtcProgram prog;
         if (hiprtcCreateProgram(&prog, src.c_str(), "fpt_topology.hip", int(kEmbeddedHeaderCount), kEmbeddedHeaderText,
                                 kEmbeddedHeaderNames) != HIPRTC_SUCCESS) {
             failure = "hiprtcCreateProgram failed";
-            return 1;
+            return false;
         }
         const std::string arch_opt = "--offload-arch=" + arch.substr(0, arch.find(':'));
         std::vector<const char *> opts = {arch_opt.c_str()};
@@ -250,37 +254,64 @@ int specialize(const FptRenderConfig &cfg, std::shared_ptr<SpecializedModule> &o
             if (n) hiprtcGetProgramLog(prog, &log[0]);
             failure = std::string("hipRTC compilation failed: ") + hiprtcGetErrorString(r) + "\n" + log.substr(0, 2000);
             hiprtcDestroyProgram(&prog);
-            return 1;
+            return false;
         }
         size_t n = 0;
         hiprtcGetCodeSize(prog, &n);
         code.resize(n);
         hiprtcGetCode(prog, code.data());
         hiprtcDestroyProgram(&prog);
+        return true;
+    };
+    // Loads the code object and looks up both kernels (the kernel's mangled
+    // name does not depend on the topology).
+    auto load = [&](const std::vector<char> &code, SpecializedModule &m) -> bool {
+        if (hipModuleLoadData(&m.module, code.data()) != hipSuccess) {
+            m.module = nullptr;
+            failure = "hipModuleLoadData failed for the specialised code object";
+            return false;
+        }
+        if (hipModuleGetFunction(&m.kernel, m.module,
+                                 "_ZN3fpt21sdf_persistent_kernelILb0ELb0ELb0ELb0ELb0ELb0EEEvPK15FptRenderConfigPKNS_"
+                                 "10BatchSceneEjP15HIP_vector_typeIfLj4EEjjPjjiPNS_13DeviceProfileENS_14ProposerDeviceE") !=
+                hipSuccess ||
+            hipModuleGetFunction(&m.validate, m.module, "fpt_topology_validate") != hipSuccess) {
+            (void)hipModuleUnload(m.module);
+            m.module = nullptr;
+            failure = "specialised kernels are missing from the code object";
+            return false;
+        }
+        return true;
+    };
+    std::vector<char> code;
+    const std::string dir = cache_dir();
+    const std::string path = dir.empty() ? std::string() : dir + "/topology-" + hex + ".co";
+    const bool from_disk = !path.empty() && read_file(path, code);
+    auto t0 = std::chrono::steady_clock::now();
+    rep.cache_status = from_disk ? 2u : 1u;
+    if (!from_disk) {
+        if (!compile(code)) return 1;
         if (!path.empty()) {
             make_dirs(dir);
             write_file_atomic(path, code);
         }
-        rep.cache_status = 1u;
-    } else {
-        rep.cache_status = 2u;
     }
     auto t1 = std::chrono::steady_clock::now();
-    rep.compile_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     auto mod = std::make_shared<SpecializedModule>();
-    if (hipModuleLoadData(&mod->module, code.data()) != hipSuccess) {
-        failure = "hipModuleLoadData failed for the specialised code object";
-        return 1;
+    if (!load(code, *mod)) {
+        if (!from_disk) return 1;
+        // A truncated or foreign file in the disk cache: drop it and compile
+        // once.
+        (void)std::remove(path.c_str());
+        t0 = std::chrono::steady_clock::now();
+        if (!compile(code)) return 1;
+        make_dirs(dir);
+        write_file_atomic(path, code);
+        rep.cache_status = 1u;
+        t1 = std::chrono::steady_clock::now();
+        if (!load(code, *mod)) return 1;
     }
-    // The kernel's mangled name does not depend on the topology.
-    if (hipModuleGetFunction(&mod->kernel, mod->module,
-                             "_ZN3fpt21sdf_persistent_kernelILb0ELb0ELb0ELb0ELb0ELb0EEEvPK15FptRenderConfigPKNS_10BatchSceneEjP15HIP_"
-                             "vector_typeIfLj4EEjjPjjiPNS_13DeviceProfileENS_14ProposerDeviceE") != hipSuccess ||
-        hipModuleGetFunction(&mod->validate, mod->module, "fpt_topology_validate") != hipSuccess) {
-        (void)hipModuleUnload(mod->module);
-        failure = "specialised kernels are missing from the code object";
-        return 1;
-    }
+    rep.compile_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     rep.link_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
     mod->code_bytes = code.size();
     rep.code_bytes = code.size();

@@ -1110,6 +1110,13 @@ int voxel_build(const FptRenderConfig &cfg, const FptRenderConfig *dcfg, VoxelSt
     const int shell = cov == FPT_VOXEL_COVERAGE_SHELL ? 1 : 0;
     store.storage = storage;
     const size_t n = size_t(res) * res * res;
+    // The trace addresses cells with 32-bit indices (x + res*(y + res*z) for
+    // dense storage, (page-1)*cells_per_brick + local for bricks).
+    if (storage == FPT_VOXEL_STORAGE_DENSE && n > size_t(0xffffffffu)) {
+        if (error && error_len)
+            std::snprintf(error, error_len, "dense voxel resolution %u exceeds 32-bit cell indices (at most 1625)", res);
+        return 1;
+    }
     unsigned long long *d_cells = nullptr;
     VX_TRY(hipMalloc(&d_cells, sizeof(unsigned long long)), "voxel counter allocation");
     temps.push_back(d_cells);
@@ -1183,6 +1190,13 @@ int voxel_build(const FptRenderConfig &cfg, const FptRenderConfig *dcfg, VoxelSt
     VX_TRY(hipStreamSynchronize(stream), "voxel classification");
     store.active_bricks = pages;
     const size_t cpb = size_t(B) * B * B;
+    if (size_t(pages) * cpb > size_t(0xffffffffu)) {
+        if (error && error_len)
+            std::snprintf(error, error_len, "%u occupied bricks exceed 32-bit cell indices of the brick pool", pages);
+        for (void *p : temps) (void)hipFree(p);
+        voxel_free(store);
+        return 1;
+    }
     const size_t pool_cells = size_t(pages > 0u ? pages : 1u) * cpb;
     if (B == 8u) {
         VX_TRY(hipMalloc(&store.distance, pool_cells * sizeof(__half)), "distance pool allocation");
