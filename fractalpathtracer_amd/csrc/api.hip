@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1065,6 +1066,124 @@ int fpt_hip_render_batch(const FptRenderConfig *configs, uint32_t count, float *
     if (elapsed_ms) *elapsed_ms = ms;
     if (linear_output)
         HIP_TRY(hipMemcpy(linear_output, acc.p, b->pixels * sizeof(float4), hipMemcpyDeviceToHost), "batch readback");
+    return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free batched render: the batch's device state (scene configs,
+// scene table, work counter) lives in one file-scope device array instead of
+// hipMalloc'd buffers.  Layout of the array, 256-byte aligned sections:
+//   [0, C)          FptRenderConfig[FPT_NOALLOC_MAX_SCENES]
+//   [C, C + T)      fpt::BatchScene[FPT_NOALLOC_MAX_SCENES]
+//   [C + T, + 256)  uint32_t work counter
+// Every call on a device shares that device's copy and leaves its contents
+// behind; the counter is cleared by the launch sequence's own fill kernel.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t align256(size_t n) { return (n + 255u) & ~size_t(255u); }
+constexpr size_t kNoAllocScenesOffset = align256(sizeof(FptRenderConfig) * FPT_NOALLOC_MAX_SCENES);
+constexpr size_t kNoAllocCounterOffset = align256(kNoAllocScenesOffset + sizeof(fpt::BatchScene) * FPT_NOALLOC_MAX_SCENES);
+constexpr size_t kNoAllocScratchBytes = kNoAllocCounterOffset + 256u;
+static_assert(kNoAllocScratchBytes <= 2u * 1024u * 1024u, "no-alloc scratch stays under 2 MiB");
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+// Device address of `scratch` on each device, taken once per device with that
+// device current.  The array's name in host code is a host-side stand-in, not
+// a device address: only these pointers reach kernels and copies.
+std::mutex g_scratch_mu;
+std::vector<unsigned char *> g_scratch_ptr;
+
+int noalloc_scratch(unsigned char **out, char *error, size_t error_len) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (size_t(dev) >= g_scratch_ptr.size()) {
+        int n = 0;
+        HIP_TRY(hipGetDeviceCount(&n), "hipGetDeviceCount");
+        g_scratch_ptr.resize(std::max(size_t(n), size_t(dev) + 1u), nullptr);
+    }
+    if (!g_scratch_ptr[size_t(dev)]) {
+        void *ptr = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[size_t(dev)] = static_cast<unsigned char *>(ptr);
+    }
+    *out = g_scratch_ptr[size_t(dev)];
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int fpt_hip_render_batch_noalloc(const FptRenderConfig *configs, uint32_t count, float *device_accum,
+                                 size_t device_accum_len, void *stream, char *error, size_t error_len) {
+    if (!configs || !device_accum) {
+        set_error(error, error_len, "missing scene configs or accumulation buffer");
+        return 1;
+    }
+    if (count == 0u || count > FPT_NOALLOC_MAX_SCENES) {
+        set_error(error, error_len, "invalid scene count %u (the allocation-free batch holds 1..%u scenes)", count,
+                  unsigned(FPT_NOALLOC_MAX_SCENES));
+        return 1;
+    }
+    fpt::BatchScene scenes[FPT_NOALLOC_MAX_SCENES];
+    uint64_t slot = 0, off = 0;
+    bool lowered = false, need_focus = false;
+    for (uint32_t i = 0; i < count; ++i) {
+        const FptRenderConfig &c = configs[i];
+        if (validate_config(&c, error, error_len)) return 1;
+        if (reject_mandelbulber(&c, "batched rendering", error, error_len)) return 1;
+        if (c.renderer_backend != FPT_RENDERER_SDF) {
+            set_error(error, error_len, "scene %u: batched rendering supports the SDF backend only", i);
+            return 1;
+        }
+        // Reading the HDRI file is the caller's step here
+        // (fpt_hip_hydrate_hdri); a hydrated config renders as in a batch.
+        if (c.world[0] == 2.0f && c.hdri_enabled != 0u && c.hdri_width == 0u) {
+            set_error(error, error_len,
+                      "scene %u: HDRI lighting needs a hydrated config (fpt_hip_hydrate_hdri) on the allocation-free path",
+                      i);
+            return 1;
+        }
+        scenes[i].slot_begin = uint32_t(slot);
+        scenes[i].slots = ((c.width + 7u) / 8u) * ((c.height + 7u) / 8u) * 64u;
+        scenes[i].width = c.width;
+        scenes[i].height = c.height;
+        scenes[i].accum_offset = off;
+        slot += scenes[i].slots;
+        off += uint64_t(c.width) * c.height;
+        lowered = lowered || fpt::has_lowered_tables(c);
+        need_focus = need_focus || (c.camera_dof > 0.0f && !(c.focus_distance > 0.0f));
+    }
+    if (slot > 0xffffffffull) {
+        set_error(error, error_len, "batch has too many pixels (%llu slots)", (unsigned long long)slot);
+        return 1;
+    }
+    if (device_accum_len < off * 4u) {
+        set_error(error, error_len, "accumulation buffer is too small (%zu < %llu floats)", device_accum_len,
+                  (unsigned long long)(off * 4u));
+        return 1;
+    }
+    if (check_device(error, error_len)) return 1;
+    unsigned char *base = nullptr;
+    if (noalloc_scratch(&base, error, error_len)) return 1;
+    FptRenderConfig *dcfgs = reinterpret_cast<FptRenderConfig *>(base);
+    fpt::BatchScene *dscenes = reinterpret_cast<fpt::BatchScene *>(base + kNoAllocScenesOffset);
+    uint32_t *counter = reinterpret_cast<uint32_t *>(base + kNoAllocCounterOffset);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // Host sources are pageable, so both copies complete before they return.
+    HIP_TRY(hipMemcpyAsync(dcfgs, configs, sizeof(FptRenderConfig) * count, hipMemcpyHostToDevice, s),
+            "scene config upload");
+    HIP_TRY(hipMemcpyAsync(dscenes, scenes, sizeof(fpt::BatchScene) * count, hipMemcpyHostToDevice, s),
+            "scene table upload");
+    if (need_focus) HIP_TRY(fpt::launch_focus_prepass(dcfgs, count, s), "focus-distance prepass");
+    HIP_TRY(fpt::launch_sdf_batch(dcfgs, dscenes, count, uint32_t(slot), reinterpret_cast<float4 *>(device_accum),
+                                  counter, default_workgroups(), 24, 0u, nullptr, lowered, s),
+            "batch kernel launch");
     return 0;
 }
 

@@ -70,6 +70,9 @@ VOXEL_LEAF_REFINEMENT_SECANT_BISECTION = 1
 VOXEL_LEAF_REFINEMENT_RESTRICTED_TRACE = 2
 VOXEL_LEAF_REFINEMENT_FIXED_DE = 3
 
+# Scene capacity of fpt_hip_render_batch_noalloc (fpt_abi.h).
+NOALLOC_MAX_SCENES = 64
+
 u32 = C.c_uint32
 f32 = C.c_float
 
@@ -288,6 +291,7 @@ EXPORTED_SYMBOLS = (
     "fpt_hip_plan_destroy", "fpt_hip_last_error", "fpt_hip_program_surface", "fpt_hip_voxel_build",
     "fpt_hip_voxel_trace_cells", "fpt_hip_plan_tune", "fpt_hip_batch_create", "fpt_hip_batch_pixels",
     "fpt_hip_batch_tune", "fpt_hip_batch_render", "fpt_hip_batch_destroy", "fpt_hip_render_batch",
+    "fpt_hip_render_batch_noalloc",
     "fpt_hip_autotune", "fpt_hip_device_alloc", "fpt_hip_device_free", "fpt_hip_copy", "fpt_hip_fill_f32",
     "fpt_hip_stream_create", "fpt_hip_stream_destroy", "fpt_hip_synchronize", "fpt_hip_time_renders",
     "fpt_hip_proposer_samples", "fpt_hip_proposer_eval", "fpt_hip_plan_set_proposer", "fpt_hip_plan_profile",
@@ -372,6 +376,9 @@ def _declare(lib):
     lib.fpt_hip_render_batch.argtypes = [P(FptRenderConfig), u32, P(f32), C.c_size_t, P(C.c_double), cp,
                                          C.c_size_t]
     lib.fpt_hip_render_batch.restype = C.c_int
+    lib.fpt_hip_render_batch_noalloc.argtypes = [P(FptRenderConfig), u32, C.c_void_p, C.c_size_t, C.c_void_p, cp,
+                                                 C.c_size_t]
+    lib.fpt_hip_render_batch_noalloc.restype = C.c_int
     lib.fpt_hip_autotune.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, u32, P(FptTuneResult), cp,
                                      C.c_size_t]
     lib.fpt_hip_autotune.restype = C.c_int

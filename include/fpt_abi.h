@@ -1107,6 +1107,33 @@ int fpt_hip_plan_voxel_stats(FptHipPlan *plan,
                              char *error,
                              size_t error_len);
 
+/*
+ * Allocation-free batched render.  No counterpart in the reference: this is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It renders
+ * `count` SDF scenes exactly as fpt_hip_batch_create + fpt_hip_batch_render
+ * do (same kernels, bit-identical images, same output layout), enqueued on
+ * `stream` without synchronising, but it calls no hipMalloc, hipFree or other
+ * allocating routine: the batch's device state (scene configs, scene table,
+ * work counter) lives in a fixed device array inside the library, under 2 MiB,
+ * one copy per GPU, held until the process ends.
+ *   - count must be 1..FPT_NOALLOC_MAX_SCENES; device_accum_len (floats) must
+ *     cover 4 * sum W_i*H_i.
+ *   - HDRI-lit configs must already be hydrated (fpt_hip_hydrate_hdri): this
+ *     entry point reads no files.
+ *   - Every call on one GPU shares that array and keeps the last call's
+ *     contents: calls that overlap on one GPU (other threads, or streams that
+ *     run concurrently) race.  Serialise them, e.g. on one stream.
+ */
+#define FPT_NOALLOC_MAX_SCENES 64u
+int fpt_hip_render_batch_noalloc(const struct FptRenderConfig *configs,
+                                 uint32_t count,
+                                 float *device_accum,
+                                 size_t device_accum_len,
+                                 void *stream,
+                                 char *error,
+                                 size_t error_len);
+
 /* Last error string of the calling thread (empty when none). */
 const char *fpt_hip_last_error(void);
 
