@@ -560,7 +560,7 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
                     } else {
                         dr = normalize(refr);
                         side = -side;
-                        color = color * vmix(mk(1.f, 1.f, 1.f), mat.rgb, 0.35f) * (1.0f - fres * 0.5f);
+                        color = color * (vmix(mk(1.f, 1.f, 1.f), mat.rgb, 0.35f) * (1.0f - fres * 0.5f));
                     }
                 }
                 f3 rp = hit_p + n * (0.001f * signf(dot(dr, n)));

@@ -114,7 +114,7 @@ __device__ f3 render_path(UV xy, uint32_t sample_idx, const FptRenderConfig &cfg
             } else {
                 dr = normalize(refr);
                 side = -side;
-                color = color * vmix(mk(1.f, 1.f, 1.f), mat.rgb, 0.35f) * (1.0f - fres * 0.5f);
+                color = color * (vmix(mk(1.f, 1.f, 1.f), mat.rgb, 0.35f) * (1.0f - fres * 0.5f));
             }
         }
         rp = rp + n * (0.001f * signf(dot(dr, n)));
@@ -505,7 +505,7 @@ __device__ f3 bounce_contribution_diagnostic(UV xy, const FptRenderConfig &cfg, 
                     dr = normalize(refl);
                 } else {
                     dr = normalize(refr);
-                    thr = thr * vmix(mk(1.f, 1.f, 1.f), mat.rgb, 0.35f) * (1.0f - fres * 0.5f);
+                    thr = thr * (vmix(mk(1.f, 1.f, 1.f), mat.rgb, 0.35f) * (1.0f - fres * 0.5f));
                 }
             }
             rp = rp + n * (0.001f * signf(dot(dr, n)));

@@ -818,7 +818,7 @@ __device__ f3 render_voxel_path(UV xy, uint32_t sample, float focus, const FptRe
                 dir = normalize(refl);
             } else {
                 dir = normalize(refr);
-                color = color * vmix(mk(1.f, 1.f, 1.f), mat.rgb, 0.35f) * (1.0f - fres * 0.5f);
+                color = color * (vmix(mk(1.f, 1.f, 1.f), mat.rgb, 0.35f) * (1.0f - fres * 0.5f));
                 transmitted = true;
             }
         }

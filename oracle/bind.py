@@ -60,6 +60,8 @@ def lib():
         L.orc_hash13.argtypes = [C.c_float] * 3
         L.orc_hash13.restype = C.c_float
         L.orc_last_counters.argtypes = [P(C.c_uint64)]
+        L.orc_random_vector.argtypes = [C.c_int, dp, P(C.c_float), C.c_size_t, dp]
+        L.orc_scatter.argtypes = [C.c_int, dp, dp]
         _voxel_decls(L)
         _lib = L
     return _lib
@@ -327,3 +329,26 @@ def hash13_bits(x: float, y: float, z: float) -> int:
 
 def hash13(x: float, y: float, z: float) -> float:
     return float(lib().orc_hash13(x, y, z))
+
+
+def random_vector(normals, keys, precision: int = 64) -> np.ndarray:
+    """randomVector (Shaders.metal:557-568): (n, 3) directions about normals
+    (n, 3) with float32 keys (n, 3) = {uv.x, uv.y, seed}."""
+    nv = np.ascontiguousarray(normals, np.float64).reshape(-1, 3)
+    k = np.ascontiguousarray(keys, np.float32).reshape(-1, 3)
+    if len(k) != len(nv):
+        raise ValueError("one key per normal")
+    out = np.zeros((len(nv), 3), np.float64)
+    if lib().orc_random_vector(precision, _dp(nv), k.ctypes.data_as(C.POINTER(C.c_float)), len(nv), _dp(out)):
+        raise RuntimeError("orc_random_vector failed")
+    return out
+
+
+def scatter(dr, n, rgb=(1.0, 1.0, 1.0), roughness=1.0, specular=0.0, translucency=0.0, ior=1.5, side=1.0,
+            r1=0.5, r2=0.5, diffuse=(0.0, 0.0, 0.0), precision: int = 64) -> dict:
+    """One scatter event of renderPath (Shaders.metal:5982-6013)."""
+    inp = np.array([*dr, *n, *rgb, roughness, specular, translucency, ior, side, r1, r2, *diffuse], np.float64)
+    out = np.zeros(10, np.float64)
+    if lib().orc_scatter(precision, _dp(inp), _dp(out)):
+        raise RuntimeError("orc_scatter failed")
+    return dict(dr=out[0:3].copy(), n=out[3:6].copy(), factor=out[6:9].copy(), side=float(out[9]))

@@ -1484,6 +1484,60 @@ V3<T> sun_contribution(V3<T> rp, UV xy, float seed, const Material<T> &mat, V3<T
     return V3<T>();
 }
 
+// One scatter event of renderPath (SM:5982-6013): the material's branch
+// choice (r1 against translucency), then
+//   reflective: dr = normalize(mix(reflect(dr, n), diffuse, roughness)),
+//               Schlick Fresnel f0 + (1 - f0)(1 - cos)^5 times specular
+//               against r2 picks the mirror direction (factor 1), otherwise
+//               the colour is multiplied by the albedo;
+//   refractive: eta = 1/ior entering (side 1), ior leaving (normal flipped);
+//               total internal reflection (|refract|^2 < 1e-6 or not finite)
+//               or r2 < Fresnel reflects, otherwise the ray refracts, the
+//               side flips and the colour takes mix(1, rgb, 0.35)(1 - F/2).
+// `diffuse` is randomVector's draw (used on the reflective branch only).
+// Returns the new direction, the normal the bounce offset uses, the colour
+// factor and the new side.
+template <class T>
+struct Scatter {
+    V3<T> dr, n, factor;
+    T side;
+};
+
+template <class T>
+Scatter<T> scatter_event(V3<T> dr, V3<T> n, const Material<T> &mat, T side, T r1, T r2, V3<T> diffuse) {
+    Scatter<T> out;
+    out.factor = V3<T>(T(1), T(1), T(1));
+    out.side = side;
+    const T f0 = std::pow((mat.ior - T(1)) / (mat.ior + T(1)), T(2));
+    if (r1 > mat.translucency) {
+        const V3<T> metal = vreflect(dr, n);
+        const T ct = m_clamp(dot(n, -dr), T(0), T(1));
+        const T om = T(1) - ct;
+        const T fres = f0 + (T(1) - f0) * (om * om * om * om * om);
+        out.dr = normalize(vmix(metal, diffuse, mat.roughness));
+        if (r2 < fres * mat.specular) out.dr = metal;
+        else out.factor = mat.rgb;
+        out.n = n;
+        return out;
+    }
+    const T eta = side == T(1) ? T(1) / mat.ior : mat.ior;
+    if (side != T(1)) n = -n;
+    const T ct = m_clamp(dot(n, -dr), T(0), T(1));
+    const T om = T(1) - ct;
+    const T fres = f0 + (T(1) - f0) * (om * om * om * om * om);
+    const V3<T> refr = vrefract(dr, n, eta);
+    const V3<T> refl = vreflect(dr, n);
+    if (dot(refr, refr) < T(0.000001f) || !std::isfinite(refr.x) || r2 < fres) {
+        out.dr = normalize(refl);
+    } else {
+        out.dr = normalize(refr);
+        out.side = -side;
+        out.factor = vmix(V3<T>(T(1), T(1), T(1)), mat.rgb, T(0.35f)) * (T(1) - fres * T(0.5f));
+    }
+    out.n = n;
+    return out;
+}
+
 // renderPath (SM:5926-6026).
 template <class T>
 V3<T> render_path(UV xy, uint32_t sample_idx, const FptRenderConfig &cfg) {
@@ -1527,36 +1581,14 @@ V3<T> render_path(UV xy, uint32_t sample_idx, const FptRenderConfig &cfg) {
         if (cfg.sun[0] == 1.0f) light = light + sun_contribution(rp, xy, frame, mat, n, cfg);
         T r1 = T(hash13(xy.x, xy.y, mul_add_f32(frame, 1.37f, float(i))));
         T r2 = T(hash13(xy.x, xy.y, mul_add_f32(frame, 7.91f, float(i))));
-        if (r1 > mat.translucency) {
-            V3<T> metal = vreflect(dr, n);
-            V3<T> diffuse = random_vector(n, xy, mul_add_f32(frame, 13.37f, float(i)));
-            T f0 = std::pow((mat.ior - T(1)) / (mat.ior + T(1)), T(2));
-            T ct = m_clamp(dot(n, -dr), T(0), T(1));
-            T om = T(1) - ct;
-            T fres = f0 + (T(1) - f0) * (om * om * om * om * om);
-            dr = normalize(vmix(metal, diffuse, mat.roughness));
-            if (r2 < fres * mat.specular) {
-                dr = metal;
-            } else {
-                color = color * mat.rgb;
-            }
-        } else {
-            T eta = side == T(1) ? T(1) / mat.ior : mat.ior;
-            if (side != T(1)) n = -n;
-            T f0 = std::pow((mat.ior - T(1)) / (mat.ior + T(1)), T(2));
-            T ct = m_clamp(dot(n, -dr), T(0), T(1));
-            T om = T(1) - ct;
-            T fres = f0 + (T(1) - f0) * (om * om * om * om * om);
-            V3<T> refr = vrefract(dr, n, eta);
-            V3<T> refl = vreflect(dr, n);
-            if (dot(refr, refr) < T(0.000001f) || !std::isfinite(refr.x) || r2 < fres) {
-                dr = normalize(refl);
-            } else {
-                dr = normalize(refr);
-                side = -side;
-                color = color * vmix(V3<T>(T(1), T(1), T(1)), mat.rgb, T(0.35f)) * (T(1) - fres * T(0.5f));
-            }
-        }
+        // randomVector is drawn only on the reflective branch, as in SM:5986.
+        const V3<T> diffuse =
+            r1 > mat.translucency ? random_vector(n, xy, mul_add_f32(frame, 13.37f, float(i))) : V3<T>();
+        const Scatter<T> sc = scatter_event(dr, n, mat, side, r1, r2, diffuse);
+        dr = sc.dr;
+        n = sc.n;
+        side = sc.side;
+        color = color * sc.factor;
         rp = rp + n * (T(0.001f) * m_sign(dot(dr, n)));
         if (cfg.sdf_russian_roulette != 0u && i + 1 < bounces && float(i + 1) >= cfg.sdf_rr_start) {
             T surv = m_clamp(m_max(color.x, m_max(color.y, color.z)),
@@ -1921,6 +1953,51 @@ int orc_diagnostic(const FptRenderConfig *cfg, uint32_t mode, uint32_t target_bo
             }
         }
     }
+    return 0;
+}
+
+// randomVector (SM:557-568) about normals[3i..3i+2] with keys
+// keys[3i..3i+2] = {uv.x, uv.y, seed} (float32): out[3i..3i+2].
+int orc_random_vector(int precision, const double *normals, const float *keys, size_t n, double *out) {
+    if (!normals || !keys || !out) return 1;
+    for (size_t i = 0; i < n; ++i) {
+        const double *nv = normals + 3 * i;
+        const UV xy{keys[3 * i], keys[3 * i + 1]};
+        if (precision == 64) {
+            const V3<double> d = random_vector(V3<double>(nv[0], nv[1], nv[2]), xy, keys[3 * i + 2]);
+            out[3 * i] = d.x; out[3 * i + 1] = d.y; out[3 * i + 2] = d.z;
+        } else {
+            const V3<float> d =
+                random_vector(V3<float>(float(nv[0]), float(nv[1]), float(nv[2])), xy, keys[3 * i + 2]);
+            out[3 * i] = d.x; out[3 * i + 1] = d.y; out[3 * i + 2] = d.z;
+        }
+    }
+    return 0;
+}
+
+// One scatter event (scatter_event): in = {dr[3], n[3], rgb[3], roughness,
+// specular, translucency, ior, side, r1, r2, diffuse[3]} (19 doubles),
+// out = {dr[3], n[3], factor[3], side} (10 doubles).
+int orc_scatter(int precision, const double *in, double *out) {
+    if (!in || !out) return 1;
+    auto run = [&](auto zero) {
+        using T = decltype(zero);
+        Material<T> m;
+        m.rgb = V3<T>(T(in[6]), T(in[7]), T(in[8]));
+        m.roughness = T(in[9]);
+        m.specular = T(in[10]);
+        m.translucency = T(in[11]);
+        m.ior = T(in[12]);
+        const Scatter<T> sc = scatter_event(V3<T>(T(in[0]), T(in[1]), T(in[2])), V3<T>(T(in[3]), T(in[4]), T(in[5])),
+                                            m, T(in[13]), T(in[14]), T(in[15]),
+                                            V3<T>(T(in[16]), T(in[17]), T(in[18])));
+        const double v[10] = {double(sc.dr.x), double(sc.dr.y), double(sc.dr.z), double(sc.n.x), double(sc.n.y),
+                              double(sc.n.z), double(sc.factor.x), double(sc.factor.y), double(sc.factor.z),
+                              double(sc.side)};
+        std::memcpy(out, v, sizeof(v));
+    };
+    if (precision == 64) run(0.0);
+    else run(0.0f);
     return 0;
 }
 
