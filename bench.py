@@ -87,6 +87,13 @@ VOXEL_PAGE_ENTRY_BYTES, VOXEL_CELL_BYTES = 4, 1
 # scaled like the default line's.
 VOXEL_TRAFFIC_FETCH_KIB, VOXEL_TRAFFIC_WRITE_KIB = 580290.8125, 671285.78125
 VOXEL_TRAFFIC_WORKLOAD = (1920, 1080, 4, 1024)
+# Batched workload: orbit iterations per leaf family.  Mandelbulb of a
+# general power in the triplex form (as above, 29), Mandelbox (box fold 12,
+# |z|^2 5, sphere fold 6, scale + offset 6, derivative 2 = 31), Menger (abs
+# 3, sort 6, scale + offset 6, cut 2 = 17); per distance evaluation outside
+# the orbit 17 (as above), plus 6 for the Menger scenes' rotation.
+BATCH_FLOPS_PER_ORBIT_ITERATION = (29, 31, 17)
+BATCH_FLOPS_PER_DISTANCE_EVAL = (17, 17, 23)
 HBM_PEAK_GBS = 8000.0      # MI355X published HBM3E bandwidth
 FP32_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
 BF16_PEAK_TFLOPS = 2500.0  # MI355X published dense BF16 matrix peak
@@ -105,12 +112,15 @@ def parse():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=8)
     ap.add_argument("--ref-spp-factor", type=int, default=4)
-    ap.add_argument("--workload", choices=("learned", "typed", "voxel", "fract", "lowered"), default="learned",
+    ap.add_argument("--workload", choices=("learned", "typed", "voxel", "fract", "lowered", "batch"),
+                    default="learned",
                     help="learned: the learned-step scene above (the default line); typed: the typed-program "
                          "scene (1280x720, 16 spp); voxel: the 1024^3 sparse-brick Mandelbox (1920x1080, 4 spp); "
                          "fract: the bundled Mandelbulber scene ifs-20.fract (Kaleidoscopic IFS, its own "
                          "1920x1080, 64 spp); lowered: a 28-primitive translated union lowered to a typed "
-                         "structure-of-arrays table (1280x720, 16 spp), timed beside the interpreter")
+                         "structure-of-arrays table (1280x720, 16 spp), timed beside the interpreter; batch: 64 "
+                         "uneven scenes (Mandelbulb / Mandelbox / Menger, 64^2-512^2 tiles, 1-16 spp, 2 bounces) "
+                         "in one graph-replayed persistent launch with the autotuned shape")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy kernel of MIB MiB (the known-size reference for "
                          "HBM byte counters) and exit")
@@ -171,6 +181,9 @@ def main():
         src.fill(1.0)
         api.device_copy(dst, src)
         print(json.dumps({"copy_kernel_bytes_read": n * 4, "copy_kernel_bytes_written": n * 4}), flush=True)
+        return
+    if args.workload == "batch":
+        run_batch(args, api, scenes)
         return
     if args.workload != "learned":
         run_workload(args, api, scenes)
@@ -274,6 +287,73 @@ def one_path_launches(cfg) -> int:
         return 1
     chunk = min(max(cfg.sdf_chunk_samples or 8, 1), 64)
     return -(-spp // chunk)
+
+
+def counted_flops(prof: dict, per_iteration: float, per_distance_eval: float) -> float:
+    """Algorithmic FLOPs of a counting pass: orbit iterations, distance
+    evaluations, and each analytic normal as NORMAL_DE_FACTOR evaluations of
+    the mean orbit."""
+    iters = prof.get("march_orbit_iterations", 0)
+    de = prof.get("distance_evals", 0)
+    avg_iter = iters / max(de, 1)
+    return (iters * per_iteration + de * per_distance_eval +
+            prof.get("normal_evals", 0) * NORMAL_DE_FACTOR * (avg_iter * per_iteration + per_distance_eval))
+
+
+def run_batch(args, api, scenes):
+    """C3: 64 uneven scenes in one persistent launch, captured in a HIP graph
+    and replayed, at the autotuned launch shape."""
+    cfgs = scenes.batch_scenes(64, seed=0)
+    samples_per_step = sum(c.width * c.height * c.samples for c in cfgs)
+    stream = api.Stream()
+    batch = api.Batch(cfgs)
+    accum = api.DeviceBuffer(batch.pixels * 4)
+    tune = batch.autotune(accum.address, stream.address, rounds=2)
+    if args.warmup:
+        api.time_renders(batch, accum, stream, args.warmup)
+    ms = api.time_renders(batch, accum, stream, args.steps) / args.steps
+    # One launch per scene (each scene its own graph-free plan), for scale.
+    solo_ms, flops, counts = 0.0, 0.0, {"distance_evals": 0, "march_orbit_iterations": 0, "normal_evals": 0}
+    for i, c in enumerate(cfgs):
+        buf = api.DeviceBuffer(c.width * c.height * 4)
+        with api.Plan(c) as plan:
+            solo_ms += api.time_renders(plan, buf, stream, 1)
+            prof, _ = plan.profile(buf)
+        fam = i % 3
+        flops += counted_flops(prof, BATCH_FLOPS_PER_ORBIT_ITERATION[fam], BATCH_FLOPS_PER_DISTANCE_EVAL[fam])
+        for k in counts:
+            counts[k] += prof[k]
+    achieved = flops / (ms / 1e3) / 1e12
+    line = {"metric": "path samples/s, batch workload", "value": samples_per_step / (ms / 1e3),
+            "unit": "path samples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "batch: 64 scenes (family i mod 3: Mandelbulb power U[2,10], Mandelbox scale "
+                                   "U[-2.5,2.5], Menger U{3..6} iterations), tiles U{64..512}^2, spp U{1..16}, "
+                                   "2 bounces, seed 0; one graph-replayed persistent launch",
+                       "scenes": len(cfgs), "pixels": batch.pixels, "samples_per_step": samples_per_step,
+                       "autotuned": {"workgroups": tune.workgroups, "event_batch": tune.event_batch,
+                                     "best_ms": tune.best_ms, "default_ms": tune.default_ms},
+                       "one_launch_per_scene_ms": solo_ms, "speedup_vs_one_launch_per_scene": solo_ms / ms,
+                       "counts": counts},
+            "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "kernel": KERNEL,
+                         "launches_per_step": 1, "launch_ms": ms, "flops_per_launch": flops}}
+    if not args.no_cpu_baseline:
+        from oracle import bind
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        t = time.perf_counter()
+        n = 0
+        for c in cfgs:
+            y0 = c.height // 2 - 1
+            bind.render(c, 64, threads=threads, rows=(y0, y0 + 2))
+            n += 2 * c.width * c.samples
+        dt = time.perf_counter() - t
+        line["cpu_baseline"] = {"value": n / dt, "unit": "path samples/s", "cores": threads, "kind": "port",
+                                "sample": f"FP64 C++ oracle (OpenMP), the two centre rows of each of the 64 "
+                                          f"scenes at its own spp ({n} path samples, {dt:.1f} s)"}
+    batch.close()
+    print(json.dumps(line), flush=True)
 
 
 def run_workload(args, api, scenes):

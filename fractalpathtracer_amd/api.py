@@ -430,6 +430,18 @@ class Plan:
         ffi.check(self.lib.fpt_hip_plan_specialize(self.handle, int(enable), C.byref(st), err, len(err)), err)
         return {k: getattr(st, k) for k, _ in ffi.FptSpecializationStats._fields_}
 
+    def specialized_surface(self, points) -> np.ndarray:
+        """fpt_hip_plan_specialized_surface: (N, 4) {distance, gx, gy, gz} of
+        the attached topology-specialised code."""
+        p = np.zeros((len(points), 4), np.float32)
+        p[:, :3] = np.asarray(points, np.float32).reshape(-1, 3)
+        out = np.zeros((len(p), 4), np.float32)
+        err = ffi.error_buffer()
+        ffi.check(self.lib.fpt_hip_plan_specialized_surface(self.handle, p.ctypes.data_as(C.POINTER(C.c_float)),
+                                                            len(p), out.ctypes.data_as(C.POINTER(C.c_float)), err,
+                                                            len(err)), err)
+        return out
+
     def profile(self, buffer: "DeviceBuffer") -> tuple[dict, dict]:
         """Counting pass of the full frame: (profile counters, learned-step
         counters)."""

@@ -769,6 +769,35 @@ int fpt_hip_plan_specialize(FptHipPlan *plan, uint32_t enable, FptSpecialization
     return rc;
 }
 
+int fpt_hip_plan_specialized_surface(FptHipPlan *plan, const float *points_xyzw, size_t point_count,
+                                     float *out_dgrad, char *error, size_t error_len) {
+    if (!plan || !points_xyzw || !out_dgrad) {
+        set_error(error, error_len, "missing plan, points or output buffer");
+        return 1;
+    }
+    if (!plan->spec) {
+        set_error(error, error_len, "the plan has no topology-specialised module (fpt_hip_plan_specialize)");
+        return 1;
+    }
+    if (point_count == 0) return 0;
+    if (point_count > 0xffffffffull) {
+        set_error(error, error_len, "too many points (%zu)", point_count);
+        return 1;
+    }
+    Stream st;
+    HIP_TRY(hipStreamCreate(&st.s), "hipStreamCreate");
+    DevBuf dp, dout;
+    HIP_TRY(dp.alloc(point_count * 16), "point buffer allocation");
+    HIP_TRY(dout.alloc(point_count * 16), "output buffer allocation");
+    HIP_TRY(hipMemcpyAsync(dp.p, points_xyzw, point_count * 16, hipMemcpyHostToDevice, st.s), "point upload");
+    HIP_TRY(fpt::sample_specialized(*plan->spec, plan->dcfg, dp.as<float4>(), uint32_t(point_count),
+                                    dout.as<float4>(), st.s),
+            "specialised surface kernel launch");
+    HIP_TRY(hipMemcpyAsync(out_dgrad, dout.p, point_count * 16, hipMemcpyDeviceToHost, st.s), "readback");
+    HIP_TRY(hipStreamSynchronize(st.s), "specialised surface sampling");
+    return 0;
+}
+
 int fpt_hip_topology_source(const FptRenderConfig *config, char *out, size_t out_len, size_t *needed, char *error,
                             size_t error_len) {
     if (!config) {

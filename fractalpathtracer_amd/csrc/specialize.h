@@ -14,6 +14,7 @@ struct SpecializedModule {
     hipModule_t module = nullptr;
     hipFunction_t kernel = nullptr;    // sdf_persistent_kernel<false, false, false>
     hipFunction_t validate = nullptr;  // fpt_topology_validate
+    hipFunction_t sample = nullptr;    // fpt_topology_sample
     size_t code_bytes = 0;
 };
 
@@ -33,6 +34,9 @@ int specialize(const FptRenderConfig &cfg, std::shared_ptr<SpecializedModule> &o
                std::string &failure);
 hipError_t validate_specialized(const SpecializedModule &m, const FptRenderConfig *dcfg, uint32_t points,
                                 SpecializeReport &rep, hipStream_t stream);
+// Specialised distance + gradient at n device points (float4 xyz_) into out.
+hipError_t sample_specialized(const SpecializedModule &m, const FptRenderConfig *dcfg, const float4 *pts, uint32_t n,
+                              float4 *out, hipStream_t stream);
 hipError_t launch_specialized(const SpecializedModule &m, const FptRenderConfig *dcfg, float4 *accum, uint32_t start,
                               uint32_t count, uint32_t *counter, uint32_t slots, uint32_t workgroups, int event_batch,
                               hipStream_t stream);

@@ -189,6 +189,19 @@ extern "C" __global__ __launch_bounds__(256) void fpt_topology_validate(const Fp
     if (isfinite(ge)) atomicMax(&counts[3], __float_as_uint(ge));
 }
 
+// Specialised distance and analytic gradient at caller points (the test hook
+// that compares this code with the CPU oracle): out[i] = {d, grad}.
+extern "C" __global__ __launch_bounds__(256) void fpt_topology_sample(const FptRenderConfig *__restrict__ cfgp,
+                                                                   const float4 *pts, uint32_t n, float4 *out) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const float4 q = pts[i];
+    const f3 p = mk(q.x, q.y, q.z);
+    f3 g;
+    topology_surface(p, *cfgp, g);
+    out[i] = make_float4(topology_distance(p, *cfgp, nullptr), g.x, g.y, g.z);
+}
+
 }  // namespace fpt
 )SRC";
     return src;
@@ -275,7 +288,8 @@ int specialize(const FptRenderConfig &cfg, std::shared_ptr<SpecializedModule> &o
                                  "_ZN3fpt21sdf_persistent_kernelILb0ELb0ELb0ELb0ELb0ELb0EEEvPK15FptRenderConfigPKNS_"
                                  "10BatchSceneEjP15HIP_vector_typeIfLj4EEjjPjjiPNS_13DeviceProfileENS_14ProposerDeviceE") !=
                 hipSuccess ||
-            hipModuleGetFunction(&m.validate, m.module, "fpt_topology_validate") != hipSuccess) {
+            hipModuleGetFunction(&m.validate, m.module, "fpt_topology_validate") != hipSuccess ||
+            hipModuleGetFunction(&m.sample, m.module, "fpt_topology_sample") != hipSuccess) {
             (void)hipModuleUnload(m.module);
             m.module = nullptr;
             failure = "specialised kernels are missing from the code object";
@@ -344,6 +358,13 @@ hipError_t validate_specialized(const SpecializedModule &m, const FptRenderConfi
     std::memcpy(&rep.max_distance_error, &host[2], 4);
     std::memcpy(&rep.max_gradient_error, &host[3], 4);
     return e;
+}
+
+hipError_t sample_specialized(const SpecializedModule &m, const FptRenderConfig *dcfg, const float4 *pts, uint32_t n,
+                              float4 *out, hipStream_t stream) {
+    if (n == 0u) return hipSuccess;
+    void *args[] = {const_cast<FptRenderConfig **>(&dcfg), const_cast<float4 **>(&pts), &n, &out};
+    return hipModuleLaunchKernel(m.sample, (n + 255u) / 256u, 1, 1, 256, 1, 1, 0, stream, args, nullptr);
 }
 
 hipError_t launch_specialized(const SpecializedModule &m, const FptRenderConfig *dcfg, float4 *accum, uint32_t start,

@@ -296,7 +296,8 @@ EXPORTED_SYMBOLS = (
     "fpt_hip_stream_create", "fpt_hip_stream_destroy", "fpt_hip_synchronize", "fpt_hip_time_renders",
     "fpt_hip_proposer_samples", "fpt_hip_proposer_eval", "fpt_hip_plan_set_proposer", "fpt_hip_plan_profile",
     "fpt_hip_plan_voxel_stats",
-    "fpt_hip_diagnostic_render", "fpt_hip_plan_specialize", "fpt_hip_topology_source", "fpt_hip_hydrate_hdri",
+    "fpt_hip_diagnostic_render", "fpt_hip_plan_specialize", "fpt_hip_plan_specialized_surface",
+    "fpt_hip_topology_source", "fpt_hip_hydrate_hdri",
     "fpt_scene_default_config", "fpt_scene_load", "fpt_scene_load_fpt_settings", "fpt_scene_apply_program_json",
     "fpt_scene_optimize_program", "fpt_scene_split_program", "fpt_scene_compile_canonical",
     "fpt_scene_lower_flat_union", "fpt_scene_lower_typed_soa",
@@ -415,6 +416,8 @@ def _declare(lib):
     lib.fpt_hip_diagnostic_render.argtypes = [P(FptRenderConfig), P(FptDiagnosticConfig), P(C.c_uint8), C.c_size_t,
                                               P(C.c_double), cp, C.c_size_t]
     lib.fpt_hip_diagnostic_render.restype = C.c_int
+    lib.fpt_hip_plan_specialized_surface.argtypes = [C.c_void_p, P(f32), C.c_size_t, P(f32), cp, C.c_size_t]
+    lib.fpt_hip_plan_specialized_surface.restype = C.c_int
     lib.fpt_hip_plan_specialize.argtypes = [C.c_void_p, u32, P(FptSpecializationStats), cp, C.c_size_t]
     lib.fpt_hip_plan_specialize.restype = C.c_int
     lib.fpt_hip_topology_source.argtypes = [P(FptRenderConfig), cp, C.c_size_t, P(C.c_size_t), cp, C.c_size_t]

@@ -983,6 +983,17 @@ int fpt_hip_plan_specialize(FptHipPlan *plan,
                             char *error,
                             size_t error_len);
 
+/* The specialised code itself at caller points (host memory): distance and
+ * analytic gradient of the plan's attached module, out_dgrad[4*i..4*i+3] =
+ * {distance, d/dx, d/dy, d/dz} as fpt_hip_program_surface reports them for
+ * the interpreter.  Fails when no module is attached. */
+int fpt_hip_plan_specialized_surface(FptHipPlan *plan,
+                                     const float *points_xyzw,
+                                     size_t point_count,
+                                     float *out_dgrad,
+                                     char *error,
+                                     size_t error_len);
+
 /* The generated source for `config`'s topology (NUL-terminated; *needed =
  * bytes including the NUL; out may be NULL to query the size). */
 int fpt_hip_topology_source(const struct FptRenderConfig *config,

@@ -67,3 +67,33 @@ def test_autotune_picks_a_valid_shape_and_keeps_the_image():
         b.render_into(buf.address, st.address)
         st.synchronize()
         assert np.array_equal(buf.numpy(), before)
+
+
+def test_batch_workload_at_its_configuration():
+    """The batched workload itself: 64 scenes (seed 0), 64^2-512^2 tiles,
+    1-16 spp, 2 bounces, graph-replayed.  Every scene equals its solo
+    render bit for bit, and one scene of each family meets the image bar
+    against the FP64 oracle on its two centre rows."""
+    cfgs = scenes.batch_scenes(64, seed=0)
+    sizes = [(c.width, c.height, c.samples) for c in cfgs]
+    assert min(min(w, h) for w, h, _ in sizes) >= 64 and max(max(w, h) for w, h, _ in sizes) <= 512
+    assert {s for _, _, s in sizes} <= set(range(1, 17))
+    st = api.Stream()
+    with api.Batch(cfgs) as b:
+        buf = api.DeviceBuffer(b.pixels * 4)
+        b.render_into(buf.address, st.address)
+        st.synchronize()
+        imgs = b.split(buf.numpy())
+    for i, c in enumerate(cfgs):
+        alone = api.render(c)
+        assert np.array_equal(imgs[i], alone), f"scene {i} ({c.width}x{c.height}, {c.samples} spp)"
+    for i in (0, 1, 2):
+        c = cfgs[i]
+        y0 = c.height // 2 - 1
+        ref = bind.render(c, 64, rows=(y0, y0 + 2))[y0:y0 + 2, :, :3]
+        twin = bind.render(c, 32, rows=(y0, y0 + 2))[y0:y0 + 2, :, :3]
+        band = imgs[i][y0:y0 + 2, :, :3]
+        rmse = float(np.sqrt(np.mean((band - ref) ** 2)))
+        rmse_twin = float(np.sqrt(np.mean((twin - ref) ** 2)))
+        print(f"scene {i} ({c.width}x{c.height}, {c.samples} spp): rmse gpu {rmse:.3e} twin {rmse_twin:.3e}")
+        assert rmse <= 1.25 * rmse_twin + 2e-3
