@@ -55,7 +55,7 @@ def render_full(cfg: FptRenderConfig, output_png: str | None = None, profile: bo
         pdict = {k: getattr(prof, k) for k, _ in ffi.FptSdfProfileStats._fields_[:13]}
     return RenderResult(out.reshape(cfg.height, cfg.width, 4), elapsed.value, build_ms.value, pdict,
                         dict(memory_bytes=vbytes.value, active_bricks=vbricks.value,
-                             active_cells=vcells.value), stitch)
+                             active_cells=vcells.value, rejected_bricks=vrej.value), stitch)
 
 
 def render(cfg: FptRenderConfig) -> np.ndarray:
@@ -128,6 +128,7 @@ def voxel_build(cfg: FptRenderConfig) -> dict:
                                       len(err)), err)
     return dict(storage=info.storage, brick_size=B, bricks_per_axis=G, active_bricks=pages,
                 active_cells=info.active_cells, memory_bytes=info.memory_bytes, build_ms=info.build_ms,
+                rejected_bricks=info.rejected_bricks,
                 page_table=pt[:G ** 3], records=rec[:nrec], distance=dist[:pages * cpb],
                 material=mat[:pages * cpb], palette=pal.reshape(256, 8))
 

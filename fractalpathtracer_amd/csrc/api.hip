@@ -1394,6 +1394,7 @@ int fpt_metal_render(const char *metallib_path, const char *stitch_metallib_path
     if (config->renderer_backend == FPT_RENDERER_VOXEL) {
         if (voxel_memory_bytes) *voxel_memory_bytes = plan->voxels.memory_bytes;
         if (voxel_active_bricks) *voxel_active_bricks = plan->voxels.active_bricks;
+        if (voxel_rejected_bricks) *voxel_rejected_bricks = plan->voxels.rejected_bricks;
         if (voxel_active_cells) *voxel_active_cells = plan->voxels.active_cells;
     }
     Stream st;
@@ -1558,8 +1559,8 @@ int fpt_hip_voxel_build(const FptRenderConfig *config, uint32_t *page_table, siz
         return 1;
     }
     double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    FptVoxelBuildInfo inf = {vs.storage, vs.brick_size, vs.bricks_per_axis, vs.active_bricks, vs.active_cells,
-                             vs.memory_bytes, ms};
+    FptVoxelBuildInfo inf = {vs.storage, vs.brick_size,   vs.bricks_per_axis, vs.active_bricks, vs.active_cells,
+                             vs.memory_bytes, ms,         vs.rejected_bricks, 0u};
     if (info) *info = inf;
     const size_t G = vs.bricks_per_axis;
     const size_t res = config->voxel_resolution;

@@ -31,6 +31,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -541,6 +542,45 @@ __global__ __launch_bounds__(B * B * B) void voxel_brick_flag_kernel(const uint8
     if (x < res && y < res && z < res && final_occupied(occ, x, y, z, res, shell != 0)) any = 1;
     __syncthreads();
     if (threadIdx.x == 0) flags[brick] = any ? 1u : 0u;
+}
+
+// Certified whole-brick rejection of the direct build (SM:5704-5726): one
+// thread per brick classifies the brick's box with the program's certified
+// bounds; a brick certified outside the surface, or inside it when the
+// interior is not filled, keeps no page.  Its cells are then never built, so
+// every cell of it is empty whatever the per-cell coverage said.  Brick box:
+// first = b*B, end = min(first + B, res), centre bmin + (first + (end -
+// first) / 2) * cell, half extent (end - first) * cell / 2, in the
+// reference's float order.
+template <int B>
+__global__ __launch_bounds__(256) void voxel_brick_reject_kernel(const FptRenderConfig *__restrict__ cfgp,
+                                                                 uint32_t *__restrict__ flags, uint32_t grid, int res,
+                                                                 unsigned long long *__restrict__ rejected) {
+    const FptRenderConfig &cfg = *cfgp;
+    const uint32_t nb = grid * grid * grid;
+    const uint32_t brick = blockIdx.x * 256u + threadIdx.x;
+    bool reject = false;
+    if (brick < nb) {
+        const uint32_t b3[3] = {brick % grid, (brick / grid) % grid, brick / (grid * grid)};
+        float c[3], h[3];
+        const f3 cs = cell_size_of(cfg);
+        const float csa[3] = {cs.x, cs.y, cs.z};
+        for (int a = 0; a < 3; ++a) {
+            const uint32_t first = b3[a] * uint32_t(B);
+            const uint32_t end = first + uint32_t(B) < uint32_t(res) ? first + uint32_t(B) : uint32_t(res);
+            const float ext = float(end - first);
+            c[a] = cfg.voxel_bounds_min[a] + (float(first) + ext * 0.5f) * csa[a];
+            h[a] = ext * csa[a] * 0.5f;
+        }
+        const f3 center = mk(c[0], c[1], c[2]);
+        Work w = {0, 0};
+        const CellClass k = classify_cell(center, mk(h[0], h[1], h[2]), distance_scene(center, cfg, w), cfg);
+        reject = k.certified && (k.cell_class == CELL_OUTSIDE ||
+                                 (cfg.voxel_fill_interior == 0u && k.cell_class == CELL_INSIDE));
+        if (reject) flags[brick] = 0u;
+    }
+    const unsigned long long bal = __ballot(reject);
+    if ((threadIdx.x & 63u) == 0u && bal) atomicAdd(rejected, (unsigned long long)__popcll(bal));
 }
 
 // Exclusive scan of brick flags -> page table (page + 1, 0 empty), one
@@ -1102,6 +1142,10 @@ int voxel_build(const FptRenderConfig &cfg, const FptRenderConfig *dcfg, VoxelSt
         if (error && error_len) std::snprintf(error, error_len, "voxel storage %u is not supported", storage);
         return 1;
     }
+    if (cfg.voxel_build_mode == FPT_VOXEL_BUILD_DIRECT && storage == FPT_VOXEL_STORAGE_DENSE) {  // BR:4720-4724
+        if (error && error_len) std::snprintf(error, error_len, "direct voxel build requires a sparse voxel storage mode");
+        return 1;
+    }
     const uint32_t cov = cfg.voxel_coverage_mode;
     if (cov > FPT_VOXEL_COVERAGE_SHELL) {
         if (error && error_len) std::snprintf(error, error_len, "voxel coverage mode %u is not supported", cov);
@@ -1146,6 +1190,10 @@ int voxel_build(const FptRenderConfig &cfg, const FptRenderConfig *dcfg, VoxelSt
     const uint32_t B = storage == FPT_VOXEL_STORAGE_SPARSE_BRICKS8 ? 8u : 4u;
     const uint32_t G = (res + B - 1u) / B;
     const uint32_t nb = G * G * G;
+    // The direct build (metal_bridge.mm:1818-1870) may reject whole bricks
+    // whose certified program bounds exclude the surface (SM:5704-5726).
+    const bool reject_bricks = cfg.voxel_build_mode == FPT_VOXEL_BUILD_DIRECT && cfg.voxel_brick_rejection != 0u &&
+                               cfg.sdf_id == FPT_SDF_PROGRAM && cfg.sdf_program_count > 0u;
     store.brick_size = B;
     store.bricks_per_axis = G;
     __half *sdist = nullptr;
@@ -1178,6 +1226,19 @@ int voxel_build(const FptRenderConfig &cfg, const FptRenderConfig *dcfg, VoxelSt
     else
         hipLaunchKernelGGL(voxel_brick_flag_kernel<4>, dim3(nb), dim3(64), 0, stream, occ, flags, G, int(res), shell);
     VX_TRY(hipGetLastError(), "brick flag kernel");
+    unsigned long long *d_rejected = nullptr;
+    if (reject_bricks) {
+        VX_TRY(hipMalloc(&d_rejected, sizeof(unsigned long long)), "rejection counter allocation");
+        temps.push_back(d_rejected);
+        VX_TRY(hipMemsetAsync(d_rejected, 0, sizeof(unsigned long long), stream), "rejection counter clear");
+        if (B == 8u)
+            hipLaunchKernelGGL(voxel_brick_reject_kernel<8>, dim3((nb + 255u) / 256u), dim3(256), 0, stream, dcfg,
+                               flags, G, int(res), d_rejected);
+        else
+            hipLaunchKernelGGL(voxel_brick_reject_kernel<4>, dim3((nb + 255u) / 256u), dim3(256), 0, stream, dcfg,
+                               flags, G, int(res), d_rejected);
+        VX_TRY(hipGetLastError(), "brick rejection kernel");
+    }
     hipLaunchKernelGGL(scan_local_kernel, dim3(nblocks), dim3(1024), 0, stream, flags, local, sums, nb);
     VX_TRY(hipGetLastError(), "scan kernel");
     hipLaunchKernelGGL(scan_blocks_kernel, dim3(1), dim3(1024), 0, stream, sums, nblocks, total);
@@ -1186,8 +1247,36 @@ int voxel_build(const FptRenderConfig &cfg, const FptRenderConfig *dcfg, VoxelSt
                        static_cast<uint32_t *>(store.page_table), nb);
     VX_TRY(hipGetLastError(), "page table kernel");
     uint32_t pages = 0;
+    unsigned long long rejected = 0;
     VX_TRY(hipMemcpyAsync(&pages, total, 4, hipMemcpyDeviceToHost, stream), "page count readback");
+    if (reject_bricks)
+        VX_TRY(hipMemcpyAsync(&rejected, d_rejected, sizeof(rejected), hipMemcpyDeviceToHost, stream),
+               "rejection count readback");
     VX_TRY(hipStreamSynchronize(stream), "voxel classification");
+    if (reject_bricks && B == 4u) {
+        // The reference's direct build holds at most page_capacity 4^3 bricks
+        // (480 MiB at resolutions >= 512, metal_bridge.mm:1824-1834) and
+        // falls back to the staging build, which rejects nothing, when the
+        // kept bricks do not fit (metal_bridge.mm:5411-5440).
+        const uint64_t brick_bytes = 64u * sizeof(VoxelRecord);
+        const uint64_t page_bytes = uint64_t(nb) * sizeof(uint32_t);
+        const uint64_t budget = res >= 512u ? 480ull * 1024u * 1024u : page_bytes + uint64_t(nb) * brick_bytes;
+        const uint64_t available = budget > page_bytes ? budget - page_bytes : brick_bytes;
+        const uint64_t capacity = std::max<uint64_t>(1u, std::min<uint64_t>(nb, available / brick_bytes));
+        if (pages > capacity) {
+            rejected = 0;
+            hipLaunchKernelGGL(voxel_brick_flag_kernel<4>, dim3(nb), dim3(64), 0, stream, occ, flags, G, int(res),
+                               shell);
+            hipLaunchKernelGGL(scan_local_kernel, dim3(nblocks), dim3(1024), 0, stream, flags, local, sums, nb);
+            hipLaunchKernelGGL(scan_blocks_kernel, dim3(1), dim3(1024), 0, stream, sums, nblocks, total);
+            hipLaunchKernelGGL(page_table_kernel, dim3(nblocks), dim3(1024), 0, stream, flags, local, sums,
+                               static_cast<uint32_t *>(store.page_table), nb);
+            VX_TRY(hipGetLastError(), "staging fallback kernels");
+            VX_TRY(hipMemcpyAsync(&pages, total, 4, hipMemcpyDeviceToHost, stream), "page count readback");
+            VX_TRY(hipStreamSynchronize(stream), "staging fallback");
+        }
+    }
+    store.rejected_bricks = uint32_t(rejected);
     store.active_bricks = pages;
     const size_t cpb = size_t(B) * B * B;
     if (size_t(pages) * cpb > size_t(0xffffffffu)) {
@@ -1400,6 +1489,88 @@ void contract_programs(FptRenderConfig *pc) {
 }
 }  // namespace
 
+// The reference's direct-build parity check (metal_bridge.mm:3836-3923): the
+// contract's unit sphere on a 32^3 grid over [-2, 2]^3, 4^3 bricks, interval
+// coverage, built by staging and by the direct build with certified brick
+// rejection.  Interval coverage keeps no cell in a brick whose bounds exclude
+// the surface, so the two builds must hold the same bricks and cells, cell
+// for cell through their page tables.
+static int direct_build_parity(const FptRenderConfig &sphere, char *error, size_t error_len) {
+    FptRenderConfig c = sphere;
+    c.voxel_resolution = 32u;
+    c.voxel_storage = FPT_VOXEL_STORAGE_SPARSE_BRICKS;
+    c.voxel_coverage_mode = FPT_VOXEL_COVERAGE_INTERVAL;
+    c.voxel_brick_rejection = 1u;
+    for (int a = 0; a < 3; ++a) {
+        c.voxel_bounds_min[a] = -2.0f;
+        c.voxel_bounds_max[a] = 2.0f;
+    }
+    FptRenderConfig cfgs[2] = {c, c};
+    cfgs[0].voxel_build_mode = FPT_VOXEL_BUILD_STAGING;
+    cfgs[1].voxel_build_mode = FPT_VOXEL_BUILD_DIRECT;
+    struct Tables {
+        std::vector<uint32_t> pages;
+        std::vector<fpt::VoxelRecord> cells;
+        uint32_t bricks = 0, rejected = 0;
+        uint64_t active = 0;
+    } t[2];
+    FptRenderConfig *d = nullptr;
+    if (hipMalloc(&d, sizeof(FptRenderConfig)) != hipSuccess) {
+        if (error && error_len) std::snprintf(error, error_len, "failed to allocate direct-build parity resources");
+        return 1;
+    }
+    for (int k = 0; k < 2; ++k) {
+        fpt::VoxelStore vs;
+        hipError_t e = hipMemcpy(d, &cfgs[k], sizeof(FptRenderConfig), hipMemcpyHostToDevice);
+        if (e != hipSuccess || fpt::voxel_build(cfgs[k], d, vs, nullptr, error, error_len)) {
+            fpt::voxel_free(vs);
+            (void)hipFree(d);
+            if (e != hipSuccess && error && error_len)
+                std::snprintf(error, error_len, "direct-build parity upload failed: %s", hipGetErrorString(e));
+            return 1;
+        }
+        t[k].pages.resize(size_t(vs.bricks_per_axis) * vs.bricks_per_axis * vs.bricks_per_axis);
+        t[k].cells.resize(size_t(vs.active_bricks) * 64u);
+        t[k].bricks = vs.active_bricks;
+        t[k].rejected = vs.rejected_bricks;
+        t[k].active = vs.active_cells;
+        e = hipMemcpy(t[k].pages.data(), vs.page_table, t[k].pages.size() * 4u, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && !t[k].cells.empty())
+            e = hipMemcpy(t[k].cells.data(), vs.pool, t[k].cells.size() * sizeof(fpt::VoxelRecord),
+                          hipMemcpyDeviceToHost);
+        fpt::voxel_free(vs);
+        if (e != hipSuccess) {
+            (void)hipFree(d);
+            if (error && error_len)
+                std::snprintf(error, error_len, "direct-build parity readback failed: %s", hipGetErrorString(e));
+            return 1;
+        }
+    }
+    (void)hipFree(d);
+    if (t[0].bricks != t[1].bricks || t[0].active != t[1].active || t[1].rejected == 0u) {
+        if (error && error_len) std::snprintf(error, error_len, "direct-build occupancy totals differ from staging");
+        return 1;
+    }
+    const fpt::VoxelRecord empty = {0u, 0u, 0.0f};
+    for (uint32_t z = 0; z < 32u; ++z)
+        for (uint32_t y = 0; y < 32u; ++y)
+            for (uint32_t x = 0; x < 32u; ++x) {
+                const uint32_t page_index = x / 4u + (y / 4u) * 8u + (z / 4u) * 64u;
+                const uint32_t local = x % 4u + (y % 4u) * 4u + (z % 4u) * 16u;
+                const fpt::VoxelRecord *cell[2];
+                for (int k = 0; k < 2; ++k) {
+                    const uint32_t page = t[k].pages[page_index];
+                    cell[k] = page == 0u ? &empty : &t[k].cells[size_t(page - 1u) * 64u + local];
+                }
+                if (std::memcmp(cell[0], cell[1], sizeof(fpt::VoxelRecord)) != 0) {
+                    if (error && error_len)
+                        std::snprintf(error, error_len, "direct-build cell mismatch at (%u,%u,%u)", x, y, z);
+                    return 1;
+                }
+            }
+    return 0;
+}
+
 // Runs the known-answer checks of the reference's contract (Shaders.metal:
 // 5156-5386) that this library implements: bits 0-29.  Bits 30 (Cage-fractal
 // bounds) and 31 (derivative bounds) belong to features this library does not
@@ -1429,5 +1600,5 @@ extern "C" int fpt_test_voxel_dda(const char *metallib_path, char *error, size_t
         if (error && error_len) std::snprintf(error, error_len, "voxel DDA contract failures: 0x%08x", f);
         return 1;
     }
-    return 0;
+    return direct_build_parity(pc[0], error, error_len);
 }

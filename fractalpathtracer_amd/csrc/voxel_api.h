@@ -20,6 +20,7 @@ struct VoxelStore {
     uint32_t bricks_per_axis = 0;
     uint32_t brick_size = 0;      // cells per brick edge (4 or 8); 0 dense
     uint32_t active_bricks = 0;
+    uint32_t rejected_bricks = 0; // bricks dropped by certified whole-brick rejection
     uint64_t active_cells = 0;
     uint64_t memory_bytes = 0;
 };

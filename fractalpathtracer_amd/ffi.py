@@ -237,7 +237,8 @@ class FptRayHit(C.Structure):
 
 class FptVoxelBuildInfo(C.Structure):
     _fields_ = [("storage", u32), ("brick_size", u32), ("bricks_per_axis", u32), ("active_bricks", u32),
-                ("active_cells", C.c_uint64), ("memory_bytes", C.c_uint64), ("build_ms", C.c_double)]
+                ("active_cells", C.c_uint64), ("memory_bytes", C.c_uint64), ("build_ms", C.c_double),
+                ("rejected_bricks", u32), ("_pad0", u32)]
 
 
 class FptTuneResult(C.Structure):
@@ -274,7 +275,7 @@ EXPECTED_SIZES = {
     FptSdfInstruction: 32, FptPrimitiveInstance: 80, FptAffineTransform: 64,
     FptIndexedPrimitive: 32, FptTypedSoAProgram: 5136, FptStitchPipelineStats: 72,
     FptRenderConfig: 30448, FptSdfProfileStats: 288, FptDiagnosticConfig: 24,
-    FptMandelbulberFieldSample: 16, FptAsyncJitStats: 40, FptRayHit: 40, FptVoxelBuildInfo: 40,
+    FptMandelbulberFieldSample: 16, FptAsyncJitStats: 40, FptRayHit: 40, FptVoxelBuildInfo: 48,
     FptTuneResult: 24, FptProposerSettings: 16, FptProposerStats: 32, FptSpecializationStats: 80,
     FptSceneOptions: 56, FptVoxelTraceStats: 32,
 }

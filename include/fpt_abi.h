@@ -788,6 +788,11 @@ int fpt_hip_plan_tune(FptHipPlan *plan,
  * built on the GPU exactly as fpt_metal_render builds them
  * (Shaders.metal:5388-5424, metal_bridge.mm:1718-1816; pages are numbered in
  * linear brick order bx + G*(by + G*bz), the reference's staging order).
+ * With voxel_build_mode DIRECT and voxel_brick_rejection set, bricks whose
+ * certified program bounds exclude the surface keep no page, as the
+ * reference's direct build drops them (Shaders.metal:5704-5726); the tables
+ * then equal the reference's cell for cell, with pages in staging order
+ * where the reference's direct build numbers them in completion order.
  * info is always filled.  Each output buffer is optional (NULL skips it); a
  * buffer that is too small fails the call and the message names the size.
  *   page_table  u32[G^3]                      (sparse layouts)
@@ -805,8 +810,10 @@ struct FptVoxelBuildInfo {
     uint64_t active_cells;
     uint64_t memory_bytes;
     double build_ms;
+    uint32_t rejected_bricks;   /* direct build with brick rejection (SM:5704-5726) */
+    uint32_t _pad0;
 };
-FPT_ABI_ASSERT(sizeof(struct FptVoxelBuildInfo) == 40, "FptVoxelBuildInfo");
+FPT_ABI_ASSERT(sizeof(struct FptVoxelBuildInfo) == 48, "FptVoxelBuildInfo");
 
 int fpt_hip_voxel_build(const struct FptRenderConfig *config,
                         uint32_t *page_table, size_t page_table_len,

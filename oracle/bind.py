@@ -92,7 +92,8 @@ def voxel_build(cfg, threads: int = 0) -> dict:
         raise RuntimeError("orc_voxel_build failed")
     storage, B, G, pages, cells = (int(info[i]) for i in range(5))
     res = cfg.voxel_resolution
-    out = dict(storage=storage, brick_size=B, bricks_per_axis=G, active_bricks=pages, active_cells=cells)
+    out = dict(storage=storage, brick_size=B, bricks_per_axis=G, active_bricks=pages, active_cells=cells,
+               rejected_bricks=int(info[5]))
     cpb = B ** 3
     pt = np.zeros(G ** 3, np.uint32)
     nrec = res ** 3 if storage == 0 else (pages * cpb if storage == 1 else 0)

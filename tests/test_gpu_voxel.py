@@ -191,3 +191,62 @@ def test_trace_memory_counts_match_oracle(storage):
     assert float(np.sqrt(np.mean((img[..., :3] - ref[..., :3]) ** 2))) <= 2e-2
     for key in ("visited_cells", "page_reads", "cell_reads"):
         assert k[key] > 0 and abs(st[key] - k[key]) <= 0.01 * k[key], (key, st[key], k[key])
+
+
+@pytest.mark.parametrize("storage", [ffi.VOXEL_STORAGE_SPARSE_BRICKS, ffi.VOXEL_STORAGE_SPARSE_BRICKS8])
+@pytest.mark.parametrize("coverage,fill", [(ffi.VOXEL_COVERAGE_LEGACY, 0), (ffi.VOXEL_COVERAGE_LEGACY, 1),
+                                           (ffi.VOXEL_COVERAGE_INTERVAL, 0), (ffi.VOXEL_COVERAGE_LIPSCHITZ, 0)])
+def test_brick_rejection_tables_are_bit_exact(storage, coverage, fill):
+    """The direct build with certified whole-brick rejection
+    (Shaders.metal:5704-5726): tables and rejected-brick counts equal the FP32
+    oracle's, on the box-minus-sphere program with a wide legacy band (where
+    rejection drops occupied cells)."""
+    cfg = _program_cfg(storage, coverage, res=44)
+    cfg.voxel_surface_band = 3.0
+    cfg.voxel_fill_interior = fill
+    cfg.voxel_build_mode, cfg.voxel_brick_rejection = 1, 1
+    g = api.voxel_build(cfg)
+    c = bind.voxel_build(cfg)
+    print("coverage", coverage, "fill", fill, "bricks", g["active_bricks"], "rejected", g["rejected_bricks"])
+    assert g["rejected_bricks"] == c["rejected_bricks"] > 0
+    for k in ("active_bricks", "active_cells"):
+        assert g[k] == c[k], k
+    assert np.array_equal(g["page_table"], c["page_table"])
+    assert g["records"].tobytes() == c["records"].tobytes()
+    assert np.array_equal(g["material"], c["material"])
+    assert np.array_equal(g["distance"], c["distance"])
+
+
+def test_direct_build_refuses_dense_storage():
+    cfg = _program_cfg(ffi.VOXEL_STORAGE_DENSE, ffi.VOXEL_COVERAGE_LEGACY)
+    cfg.voxel_build_mode = 1
+    with pytest.raises(ffi.FptError, match="direct voxel build requires a sparse voxel storage mode"):
+        api.voxel_build(cfg)
+
+
+def test_metal_render_reports_rejected_bricks():
+    cfg = _program_cfg(ffi.VOXEL_STORAGE_SPARSE_BRICKS, ffi.VOXEL_COVERAGE_LEGACY, res=44)
+    cfg.renderer_backend = ffi.RENDERER_VOXEL
+    cfg.voxel_surface_band = 3.0
+    cfg.width, cfg.height, cfg.samples = 32, 24, 1
+    cfg.voxel_build_mode, cfg.voxel_brick_rejection = 1, 1
+    res = api.render_full(cfg)
+    assert res.voxel["rejected_bricks"] == bind.voxel_build(cfg)["rejected_bricks"] > 0
+
+
+@pytest.mark.parametrize("storage", [ffi.VOXEL_STORAGE_SPARSE_BRICKS, ffi.VOXEL_STORAGE_SPARSE_BRICKS8])
+@pytest.mark.parametrize("fill", [0, 1])
+def test_brick_rejection_on_the_sphere_is_bit_exact(storage, fill):
+    """The sphere scene has bricks certified inside, so the fill_interior
+    rule decides their fate (rejected when the interior is not filled)."""
+    from tests.test_voxel_oracle import _sphere_rejection_cfg
+    cfg = _sphere_rejection_cfg(storage, ffi.VOXEL_COVERAGE_LEGACY, res=36, band=3.0, fill=fill)
+    cfg.voxel_build_mode, cfg.voxel_brick_rejection = 1, 1
+    g = api.voxel_build(cfg)
+    c = bind.voxel_build(cfg)
+    print("fill", fill, "bricks", g["active_bricks"], "rejected", g["rejected_bricks"])
+    assert g["rejected_bricks"] == c["rejected_bricks"] > 0
+    assert (g["active_bricks"], g["active_cells"]) == (c["active_bricks"], c["active_cells"])
+    assert np.array_equal(g["page_table"], c["page_table"])
+    assert g["records"].tobytes() == c["records"].tobytes()
+    assert np.array_equal(g["material"], c["material"])

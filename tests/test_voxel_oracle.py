@@ -149,3 +149,119 @@ def test_memory_work_counts_follow_the_visit_lists(storage, B):
         payload += int(np.count_nonzero(v["page_table"][bid]))
     assert visited > 100 and (storage == ffi.VOXEL_STORAGE_DENSE or 0 < payload < visited)
     assert got == {"visited_cells": visited, "page_reads": pages, "cell_reads": payload}
+
+
+# --- certified whole-brick rejection (Shaders.metal:5704-5726) --------------------
+
+def _sphere_rejection_cfg(storage, coverage, res=32, band=0.5, fill=0, radius=1.0):
+    """The reference's direct-build parity scene (metal_bridge.mm:3836-3923):
+    a sphere program on [-2, 2]^3."""
+    c = scenes.default_config()
+    scenes.set_program(c, [(ffi.OP_SPHERE, 0, [radius])])
+    c.voxel_resolution = res
+    c.voxel_storage = storage
+    c.voxel_coverage_mode = coverage
+    c.voxel_surface_band = band
+    c.voxel_fill_interior = fill
+    for i in range(3):
+        c.voxel_bounds_min[i], c.voxel_bounds_max[i] = -2.0, 2.0
+    return c
+
+
+def _cells_through_pages(v, res):
+    """Occupied flag of every grid cell, looked up through the page table."""
+    B, G = v["brick_size"], v["bricks_per_axis"]
+    occ = np.zeros((res, res, res), bool)
+    for page_index in np.nonzero(v["page_table"])[0]:
+        page = int(v["page_table"][page_index]) - 1
+        bx, by, bz = page_index % G, (page_index // G) % G, page_index // (G * G)
+        if B == 8:
+            cells = v["material"][page * 512:(page + 1) * 512] != 0
+        else:
+            cells = (v["records"]["packed_color"][page * 64:(page + 1) * 64] >> 31).astype(bool)
+        blk = cells.reshape(B, B, B)  # (z, y, x)
+        z0, y0, x0 = bz * B, by * B, bx * B
+        sz, sy, sx = min(B, res - z0), min(B, res - y0), min(B, res - x0)
+        occ[z0:z0 + sz, y0:y0 + sy, x0:x0 + sx] = blk[:sz, :sy, :sx]
+    return occ
+
+
+@pytest.mark.parametrize("storage", [ffi.VOXEL_STORAGE_SPARSE_BRICKS, ffi.VOXEL_STORAGE_SPARSE_BRICKS8])
+def test_direct_build_with_interval_coverage_equals_staging(storage):
+    """The reference's own parity check: with interval coverage no cell of a
+    brick certified off the surface is occupied, so rejection changes
+    nothing but the count (metal_bridge.mm:3836-3923)."""
+    cfg = _sphere_rejection_cfg(storage, ffi.VOXEL_COVERAGE_INTERVAL)
+    staging = bind.voxel_build(cfg)
+    cfg.voxel_build_mode, cfg.voxel_brick_rejection = 1, 1
+    direct = bind.voxel_build(cfg)
+    assert staging["rejected_bricks"] == 0 and direct["rejected_bricks"] > 0
+    assert direct["active_bricks"] == staging["active_bricks"]
+    assert direct["active_cells"] == staging["active_cells"]
+    assert np.array_equal(_cells_through_pages(direct, 32), _cells_through_pages(staging, 32))
+
+
+def _brick_distance_range(G, B, res, lo=-2.0, hi=2.0, radius=1.0):
+    """Closed form of the sphere distance over each brick's box: min and max
+    of |p| - r, (G^3,) arrays in linear brick order."""
+    cell = (hi - lo) / res
+    idx = np.arange(G)
+    first = idx * B
+    end = np.minimum(first + B, res)
+    a0, a1 = lo + first * cell, lo + end * cell
+    near = np.where(a1 < 0, a1, np.where(a0 > 0, a0, 0.0))          # per axis
+    far = np.maximum(np.abs(a0), np.abs(a1))
+    nz, ny, nx = np.meshgrid(near, near, near, indexing="ij")
+    fz, fy, fx = np.meshgrid(far, far, far, indexing="ij")
+    dmin = np.sqrt(nx ** 2 + ny ** 2 + nz ** 2) - radius
+    dmax = np.sqrt(fx ** 2 + fy ** 2 + fz ** 2) - radius
+    return dmin.reshape(-1), dmax.reshape(-1)
+
+
+@pytest.mark.parametrize("storage,B", [(ffi.VOXEL_STORAGE_SPARSE_BRICKS, 4), (ffi.VOXEL_STORAGE_SPARSE_BRICKS8, 8)])
+@pytest.mark.parametrize("fill", [0, 1])
+def test_rejection_drops_exactly_the_bricks_off_the_surface(storage, B, fill):
+    """Legacy coverage with a wide band marks cells near, but not on, the
+    surface.  Against the sphere's closed-form distance range over each brick
+    box: a brick clearly outside (min > 1e-3), or clearly inside (max <
+    -1e-3) with the interior not filled, keeps no page and is counted; a
+    brick clearly straddling the surface keeps its cells; a brick inside
+    with fill_interior keeps its filled cells."""
+    res = 36
+    cfg = _sphere_rejection_cfg(storage, ffi.VOXEL_COVERAGE_LEGACY, res=res, band=3.0, fill=fill)
+    staging = bind.voxel_build(cfg)
+    cfg.voxel_build_mode, cfg.voxel_brick_rejection = 1, 1
+    direct = bind.voxel_build(cfg)
+    G = -(-res // B)
+    dmin, dmax = _brick_distance_range(G, B, res)
+    outside, inside = dmin > 1e-3, dmax < -1e-3
+    surface = (dmin < -1e-3) & (dmax > 1e-3)
+    clear_reject = outside | (inside & (fill == 0))
+    had = staging["page_table"] > 0
+    assert np.all(direct["page_table"][clear_reject] == 0)
+    assert np.all((direct["page_table"] > 0)[surface & had])
+    if fill:
+        assert np.all((direct["page_table"] > 0)[inside & had])
+    ambiguous = ~(outside | inside | surface)
+    assert clear_reject.sum() <= direct["rejected_bricks"] <= clear_reject.sum() + ambiguous.sum()
+    # rejection dropped occupied cells here, unlike interval coverage
+    assert (had & clear_reject).any() and direct["active_cells"] < staging["active_cells"]
+    # kept bricks hold exactly the staging build's cells
+    occ_s, occ_d = _cells_through_pages(staging, res), _cells_through_pages(direct, res)
+    keep = np.zeros((G * B,) * 3, bool)
+    keep.reshape(G, B, G, B, G, B)[...] = (direct["page_table"] > 0).reshape(G, 1, G, 1, G, 1)
+    keep = keep[:res, :res, :res]
+    assert np.array_equal(occ_d, occ_s & keep)
+
+
+def test_direct_build_without_rejection_or_programs_equals_staging():
+    cfg = _sphere_rejection_cfg(ffi.VOXEL_STORAGE_SPARSE_BRICKS, ffi.VOXEL_COVERAGE_LEGACY, band=3.0)
+    staging = bind.voxel_build(cfg)
+    cfg.voxel_build_mode = 1
+    direct = bind.voxel_build(cfg)
+    assert direct["rejected_bricks"] == 0 and np.array_equal(direct["page_table"], staging["page_table"])
+    mb = _small(ffi.VOXEL_STORAGE_SPARSE_BRICKS)   # a built-in fractal: no certified bounds
+    a = bind.voxel_build(mb)
+    mb.voxel_build_mode, mb.voxel_brick_rejection = 1, 1
+    b = bind.voxel_build(mb)
+    assert b["rejected_bricks"] == 0 and np.array_equal(a["page_table"], b["page_table"])
