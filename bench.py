@@ -47,15 +47,19 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "path samples/s at 1080p: learned-step vs exact-DE march at equal image RMSE"
 
-# Algorithmic operation counts (FLOPs; each transcendental counts 1):
+# Algorithmic operation counts.  Arithmetic FLOPs (add/sub/mul/min/max/cmp 1,
+# FMA 2, div 1, sqrt 1) are the roofline's numerator; transcendentals are
+# tallied apart, against the quarter-rate special-function peak.
 #  Mandelbulb orbit iteration in the reference's triplex form
 #  (orbit.rs:766-778 + add_c + |z| + bailout test): 22 arithmetic + 7
 #  transcendentals (asin, atan2, pow, 2 cos, 2 sin).
-FLOPS_PER_ORBIT_ITERATION = 29
+FLOPS_PER_ORBIT_ITERATION = 22
+SPECIAL_PER_ORBIT_ITERATION = 7
 #  Per distance evaluation outside the orbit loop: Logarithmic finalizer
-#  (log + 3), error-protection mix (3), march step (abs, scale, 3-vector
+#  (3 + log), error-protection mix (3), march step (abs, scale, 3-vector
 #  advance = 8, LOD/threshold/max tests 3).
-FLOPS_PER_DISTANCE_EVAL = 17
+FLOPS_PER_DISTANCE_EVAL = 16
+SPECIAL_PER_DISTANCE_EVAL = 1
 #  Analytic normal = forward-mode gradient: 4 lanes of value+3 tangents,
 #  counted as 4 distance evaluations of the same orbit.
 NORMAL_DE_FACTOR = 4
@@ -87,15 +91,20 @@ VOXEL_PAGE_ENTRY_BYTES, VOXEL_CELL_BYTES = 4, 1
 # scaled like the default line's.
 VOXEL_TRAFFIC_FETCH_KIB, VOXEL_TRAFFIC_WRITE_KIB = 580290.8125, 671285.78125
 VOXEL_TRAFFIC_WORKLOAD = (1920, 1080, 4, 1024)
-# Batched workload: orbit iterations per leaf family.  Mandelbulb of a
-# general power in the triplex form (as above, 29), Mandelbox (box fold 12,
-# |z|^2 5, sphere fold 6, scale + offset 6, derivative 2 = 31), Menger (abs
-# 3, sort 6, scale + offset 6, cut 2 = 17); per distance evaluation outside
-# the orbit 17 (as above), plus 6 for the Menger scenes' rotation.
-BATCH_FLOPS_PER_ORBIT_ITERATION = (29, 31, 17)
-BATCH_FLOPS_PER_DISTANCE_EVAL = (17, 17, 23)
+# Batched workload: arithmetic FLOPs per orbit iteration per leaf family.
+# Mandelbulb of a general power in the triplex form (as above, 22 + 7
+# transcendentals), Mandelbox (box fold 12, |z|^2 5, sphere fold 6, scale +
+# offset 6, derivative 2 = 31), Menger (abs 3, sort 6, scale + offset 6, cut
+# 2 = 17); per distance evaluation outside the orbit 16 + a logarithm for the
+# Mandelbulb, 17 for the Mandelbox finaliser, plus 6 for the Menger scenes'
+# rotation.
+BATCH_FLOPS_PER_ORBIT_ITERATION = (22, 31, 17)
+BATCH_FLOPS_PER_DISTANCE_EVAL = (16, 17, 23)
 HBM_PEAK_GBS = 8000.0      # MI355X published HBM3E bandwidth
 FP32_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
+# Transcendentals issue at a quarter of the unpacked FP32 rate: 256 CUs x
+# 16 per clock x 2.4 GHz.
+SPECIAL_PEAK_TOPS = FP32_PEAK_TFLOPS / 16.0
 BF16_PEAK_TFLOPS = 2500.0  # MI355X published dense BF16 matrix peak
 KERNEL = "sdf_persistent_kernel"
 
@@ -142,17 +151,38 @@ def oracle_band(cfg, rows: int, ref_factor: int):
     samples = rows * cfg.width * (s1 - s0)
     base = {"value": samples / dt, "unit": "path samples/s", "cores": threads, "kind": "port",
             "sample": f"FP64 C++ oracle (OpenMP), rows {y0}-{y1 - 1} of the {cfg.width}x{cfg.height} frame, "
-                      f"samples {s0}-{s1 - 1} per pixel ({samples} path samples, {dt:.1f} s)"}
+                      f"samples {s0}-{s1 - 1} per pixel ({samples} path samples, {dt:.1f} s)",
+            "cpu_model": cpu_model(), "single_core": single_core_rate(cfg, y0)}
     return base, same, conv, (y0, y1)
 
 
 def frame_flops(prof: dict) -> float:
-    iters = prof.get("march_orbit_iterations", 0)
-    de = prof.get("distance_evals", 0)
-    normals = prof.get("normal_evals", 0)
-    avg_iter = iters / max(de, 1)
-    return (iters * FLOPS_PER_ORBIT_ITERATION + de * FLOPS_PER_DISTANCE_EVAL +
-            normals * NORMAL_DE_FACTOR * (avg_iter * FLOPS_PER_ORBIT_ITERATION + FLOPS_PER_DISTANCE_EVAL))
+    return counted_flops(prof, FLOPS_PER_ORBIT_ITERATION, FLOPS_PER_DISTANCE_EVAL)
+
+
+def frame_special_ops(prof: dict) -> float:
+    return counted_flops(prof, SPECIAL_PER_ORBIT_ITERATION, SPECIAL_PER_DISTANCE_EVAL)
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def single_core_rate(cfg, y0: int) -> dict:
+    """The FP64 oracle on one host core: one row, 16 samples per pixel."""
+    from oracle import bind
+    t = time.perf_counter()
+    bind.render_samples(cfg, (cfg.samples, cfg.samples + 16), rows=(y0, y0 + 1), threads=1)
+    dt = time.perf_counter() - t
+    n = cfg.width * 16
+    return {"value": n / dt, "cores": 1, "sample": f"row {y0}, samples {cfg.samples}-{cfg.samples + 15} "
+                                                    f"({n} path samples, {dt:.1f} s)"}
 
 
 def run_method(api, plan, cfg, stream, accum, steps, warmup):
@@ -229,6 +259,7 @@ def main():
         traffic = 1024 * (TRAFFIC_FETCH_KIB * TRAFFIC_FETCH_SCALE + TRAFFIC_WRITE_KIB * TRAFFIC_WRITE_SCALE)
     launch_ms = m["ms_per_step"] / launches
     flops_launch = frame_flops(m["prof"]) / launches
+    special_launch = frame_special_ops(m["prof"]) / launches
     achieved = flops_launch / (launch_ms / 1e3) / 1e12
     summary = {}
     for k, mm in methods.items():
@@ -267,7 +298,11 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": KERNEL, "launches_per_step": launches, "launch_ms": launch_ms,
-                     "flops_per_launch": flops_launch},
+                     "flops_per_launch": flops_launch,
+                     "special_ops": {"per_launch": special_launch,
+                                     "achieved_tops": special_launch / (launch_ms / 1e3) / 1e12,
+                                     "peak_tops": SPECIAL_PEAK_TOPS,
+                                     "frac": special_launch / (launch_ms / 1e3) / 1e12 / SPECIAL_PEAK_TOPS}},
     }
     if base is not None:
         line["cpu_baseline"] = base
@@ -351,7 +386,8 @@ def run_batch(args, api, scenes):
         dt = time.perf_counter() - t
         line["cpu_baseline"] = {"value": n / dt, "unit": "path samples/s", "cores": threads, "kind": "port",
                                 "sample": f"FP64 C++ oracle (OpenMP), the two centre rows of each of the 64 "
-                                          f"scenes at its own spp ({n} path samples, {dt:.1f} s)"}
+                                          f"scenes at its own spp ({n} path samples, {dt:.1f} s)",
+                                "cpu_model": cpu_model()}
     batch.close()
     print(json.dumps(line), flush=True)
 

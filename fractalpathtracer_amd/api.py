@@ -454,6 +454,14 @@ class Plan:
         return ({k: getattr(prof, k) for k, _ in ffi.FptSdfProfileStats._fields_[:13]},
                 {k: getattr(lst, k) for k, _ in ffi.FptProposerStats._fields_})
 
+    def lane_stats(self, buffer: "DeviceBuffer") -> dict:
+        """Lane occupancy of the persistent kernel over the full frame
+        (fpt_hip_plan_lane_stats)."""
+        st = ffi.FptLaneStats()
+        err = ffi.error_buffer()
+        ffi.check(self.lib.fpt_hip_plan_lane_stats(self.handle, buffer.ptr, C.byref(st), err, len(err)), err)
+        return {k: getattr(st, k) for k, _ in ffi.FptLaneStats._fields_[:-1]}
+
     def voxel_stats(self, buffer: "DeviceBuffer") -> dict:
         """Counting pass of a voxel plan's full frame: cells visited, page-table
         entries and cell payloads read by every traversal."""

@@ -442,6 +442,10 @@ struct FptHipPlan {
     uint32_t workgroups = 0;     // persistent grid; 0 = 2 per compute unit
     uint32_t event_batch = 24;   // waiting lanes before per-hit work runs
     uint32_t *counter = nullptr; // persistent pixel counter
+    // Hand-out order of the persistent kernel's 8x8 tiles: the most work
+    // first (longest-processing-time order from a probe of primary rays), so
+    // the frame's tail is made of cheap tiles and lanes finish together.
+    uint32_t *tile_order = nullptr;
     fpt::ProposerHost proposer;  // learned stepping (fpt_hip_plan_set_proposer)
     // Topology-specialised persistent kernel (fpt_hip_plan_specialize).
     std::shared_ptr<fpt::SpecializedModule> spec;
@@ -470,10 +474,32 @@ hipError_t launch_sdf(FptHipPlan *plan, float4 *acc, uint32_t start, uint32_t ch
         hipError_t e = fpt::launch_zero_counter(plan->counter, s);
         if (e != hipSuccess) return e;
         return fpt::launch_specialized(*plan->spec, plan->dcfg, acc, start, chunk, plan->counter,
-                                       fpt::persistent_slots(c.width, c.height), wg, int(plan->event_batch), s);
+                                       fpt::persistent_slots(c.width, c.height), wg, int(plan->event_batch),
+                                       plan->tile_order, s);
     }
     return fpt::launch_sdf_persistent(plan->dcfg, c, acc, start, chunk, plan->counter, wg, int(plan->event_batch),
-                                      prof, plan->proposer.enabled ? &plan->proposer.dev : nullptr, s);
+                                      prof, plan->proposer.enabled ? &plan->proposer.dev : nullptr, plan->tile_order,
+                                      s);
+}
+
+// The persistent kernel's tile order: tiles sorted by the probe's work
+// estimate, most first (ties in row-major order).  Any order gives the same
+// image, since every pixel is rendered by one lane alone.
+int plan_tile_order(FptHipPlan *plan, char *error, size_t error_len) {
+    const FptRenderConfig &c = plan->host;
+    const uint32_t tiles_x = (c.width + 7u) / 8u, ntiles = tiles_x * ((c.height + 7u) / 8u);
+    DevBuf cost;
+    HIP_TRY(cost.alloc(size_t(ntiles) * sizeof(float)), "tile cost buffer");
+    HIP_TRY(fpt::launch_tile_cost(plan->dcfg, c, cost.as<float>(), nullptr), "tile cost kernel");
+    std::vector<float> h(ntiles);
+    HIP_TRY(hipMemcpy(h.data(), cost.p, size_t(ntiles) * sizeof(float), hipMemcpyDeviceToHost), "tile cost readback");
+    std::vector<uint32_t> order(ntiles);
+    for (uint32_t i = 0; i < ntiles; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return h[a] > h[b]; });
+    HIP_TRY(hipMalloc(&plan->tile_order, size_t(ntiles) * sizeof(uint32_t)), "tile order allocation");
+    HIP_TRY(hipMemcpy(plan->tile_order, order.data(), size_t(ntiles) * sizeof(uint32_t), hipMemcpyHostToDevice),
+            "tile order upload");
+    return 0;
 }
 
 // Attach a validated topology-specialised kernel to `plan`
@@ -575,6 +601,10 @@ int fpt_hip_plan_create(const FptRenderConfig *config, FptHipPlan **plan, char *
             return 1;
         }
     }
+    if (config->renderer_backend == FPT_RENDERER_SDF && plan_tile_order(p, error, error_len)) {
+        fpt_hip_plan_destroy(p);
+        return 1;
+    }
     if (config->renderer_backend == FPT_RENDERER_VOXEL) {
         auto t0 = std::chrono::steady_clock::now();
         if (fpt::voxel_build(*config, p->dcfg, p->voxels, nullptr, error, error_len)) {
@@ -640,6 +670,7 @@ void fpt_hip_plan_destroy(FptHipPlan *plan) {
     if (!plan) return;
     if (plan->dcfg) (void)hipFree(plan->dcfg);
     if (plan->counter) (void)hipFree(plan->counter);
+    if (plan->tile_order) (void)hipFree(plan->tile_order);
     fpt::voxel_free(plan->voxels);
     fpt::proposer_release(plan->proposer);
     plan->spec.reset();
@@ -707,18 +738,15 @@ int fpt_hip_plan_voxel_stats(FptHipPlan *plan, float *device_accum, FptVoxelTrac
     return 0;
 }
 
-int fpt_hip_plan_profile(FptHipPlan *plan, float *device_accum, FptSdfProfileStats *stats,
-                         FptProposerStats *learned, char *error, size_t error_len) {
-    if (!plan || !device_accum || !stats) {
-        set_error(error, error_len, "missing plan, accumulation buffer or stats");
-        return 1;
-    }
+}  // extern "C"
+
+namespace {
+// The untimed counting pass of a plan's full frame (the PROFILE kernels).
+int profile_pass(FptHipPlan *plan, float *device_accum, fpt::DeviceProfile &hp, char *error, size_t error_len) {
     if (plan->host.renderer_backend != FPT_RENDERER_SDF) {
         set_error(error, error_len, "the counting pass applies to the SDF backend");
         return 1;
     }
-    std::memset(stats, 0, sizeof(*stats));
-    if (learned) std::memset(learned, 0, sizeof(*learned));
     Stream st;
     HIP_TRY(hipStreamCreate(&st.s), "hipStreamCreate");
     DevBuf prof;
@@ -728,9 +756,24 @@ int fpt_hip_plan_profile(FptHipPlan *plan, float *device_accum, FptSdfProfileSta
     float4 *acc = reinterpret_cast<float4 *>(device_accum);
     for (uint32_t s0 = 0; s0 < spp; s0 += chunk)
         HIP_TRY(launch_sdf(plan, acc, s0, chunk, prof.as<fpt::DeviceProfile>(), st.s), "profile kernel launch");
-    fpt::DeviceProfile hp;
     HIP_TRY(hipMemcpyAsync(&hp, prof.p, sizeof(hp), hipMemcpyDeviceToHost, st.s), "profile readback");
     HIP_TRY(hipStreamSynchronize(st.s), "profile pass");
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int fpt_hip_plan_profile(FptHipPlan *plan, float *device_accum, FptSdfProfileStats *stats,
+                         FptProposerStats *learned, char *error, size_t error_len) {
+    if (!plan || !device_accum || !stats) {
+        set_error(error, error_len, "missing plan, accumulation buffer or stats");
+        return 1;
+    }
+    std::memset(stats, 0, sizeof(*stats));
+    if (learned) std::memset(learned, 0, sizeof(*learned));
+    fpt::DeviceProfile hp;
+    if (profile_pass(plan, device_accum, hp, error, error_len)) return 1;
     stats->primary_steps = hp.primary_steps;
     stats->secondary_steps = hp.secondary_steps;
     stats->shadow_steps = hp.shadow_steps;
@@ -745,6 +788,24 @@ int fpt_hip_plan_profile(FptHipPlan *plan, float *device_accum, FptSdfProfileSta
         learned->far_steps = hp.far_steps;
         learned->backtracks = hp.backtracks;
     }
+    return 0;
+}
+
+int fpt_hip_plan_lane_stats(FptHipPlan *plan, float *device_accum, FptLaneStats *stats, char *error,
+                            size_t error_len) {
+    if (!plan || !device_accum || !stats) {
+        set_error(error, error_len, "missing plan, accumulation buffer or stats");
+        return 1;
+    }
+    if (plan->kernel != 1u) {
+        set_error(error, error_len, "lane statistics are kept by the persistent kernel (kernel 1)");
+        return 1;
+    }
+    std::memset(stats, 0, sizeof(*stats));
+    fpt::DeviceProfile hp;
+    if (profile_pass(plan, device_accum, hp, error, error_len)) return 1;
+    static_assert(sizeof(FptLaneStats) == sizeof(hp.lane), "lane counters");
+    std::memcpy(stats, hp.lane, sizeof(hp.lane));
     return 0;
 }
 
@@ -1248,7 +1309,8 @@ int fpt_hip_autotune(FptHipPlan *plan, FptHipBatch *batch, float *device_accum, 
     auto launch = [&](uint32_t wg, uint32_t eb) -> hipError_t {
         if (plan)
             return fpt::launch_sdf_persistent(plan->dcfg, plan->host, acc, 0u, 1u, plan->counter, wg, int(eb), nullptr,
-                                              plan->proposer.enabled ? &plan->proposer.dev : nullptr, s);
+                                              plan->proposer.enabled ? &plan->proposer.dev : nullptr, plan->tile_order,
+                                              s);
         return fpt::launch_sdf_batch(batch->dcfgs, batch->dscenes, uint32_t(batch->host.size()), batch->total_slots, acc,
                                      batch->counter, wg, int(eb), 1u, nullptr, batch->lowered, s);
     };

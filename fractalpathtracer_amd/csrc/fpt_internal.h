@@ -30,6 +30,8 @@ struct DeviceProfile {
     unsigned long long proposals;
     unsigned long long far_steps;
     unsigned long long backtracks;
+    // Lane occupancy of the persistent kernel (FptLaneStats).
+    unsigned long long lane[12];
 };
 
 struct ProposerDevice;
@@ -75,6 +77,9 @@ uint16_t half_bits_round_up(float v);
 hipError_t launch_focus_prepass(FptRenderConfig *dcfgs, uint32_t n, hipStream_t stream);
 hipError_t launch_trace_primary(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, FptRayHit *hits,
                                 hipStream_t stream);
+// Work estimate per 8x8 tile (tiles row-major) into cost[ceil(W/8)*ceil(H/8)].
+hipError_t launch_tile_cost(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, float *cost,
+                            hipStream_t stream);
 hipError_t launch_field_sample(const FptRenderConfig *dcfg, const float4 *pts, size_t n,
                                FptMandelbulberFieldSample *out, hipStream_t stream);
 hipError_t launch_surface_sample(const FptRenderConfig *dcfg, const float4 *pts, size_t n, float4 *out,
@@ -83,10 +88,12 @@ hipError_t launch_surface_sample(const FptRenderConfig *dcfg, const float4 *pts,
 // max gradient error bits}, zeroed by the caller.
 hipError_t launch_typed_soa_validation(const FptRenderConfig *dcfg, uint32_t n, uint32_t *counts,
                                        hipStream_t stream);
+// tile_order (optional, device): the order in which the 8x8 tiles are handed
+// out (a permutation of the row-major tile indices).
 hipError_t launch_sdf_persistent(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, float4 *accum,
                                  uint32_t start, uint32_t count, uint32_t *counter, uint32_t workgroups,
                                  int event_batch, DeviceProfile *prof, const ProposerDevice *prop,
-                                 hipStream_t stream);
+                                 const uint32_t *tile_order, hipStream_t stream);
 hipError_t launch_sdf_diagnostic(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg,
                                  const FptDiagnosticConfig &dg, uchar4 *out, hipStream_t stream);
 hipError_t launch_present(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, const float4 *accum,

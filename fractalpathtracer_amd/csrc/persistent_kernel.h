@@ -77,6 +77,22 @@ struct LearnedCounters {
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 
+// Lane-occupancy counters of the counting pass (FptLaneStats): the first
+// active lane of the region adds the region's count, so summing the lanes'
+// counters at the end gives one count per wave pass.
+enum : int {
+    LS_TRIPS = 0, LS_MARCH_PASSES, LS_MARCH_LANES, LS_FINISH_PASSES, LS_FINISH_LANES, LS_EVENT_PASSES,
+    LS_EVENT_LANES, LS_NORMAL_PASSES, LS_NORMAL_LANES, LS_IDLE_LANES, LS_PROPOSER_LANES, LS_COUNT = 12
+};
+__device__ __forceinline__ void lane_stat(unsigned long long *ls, int passes, int lanes, bool member) {
+    const unsigned long long m = __ballot(member);
+    const unsigned long long active = __ballot(true);
+    if (m && int(lane_id()) == __ffsll((long long)active) - 1) {
+        if (passes >= 0) ls[passes] += 1u;
+        ls[lanes] += __popcll(m);
+    }
+}
+
 // Wave-level fetch of work slots: one atomic per wave, consecutive indices to
 // the requesting lanes in lane order.  Must be reached by every lane.
 __device__ __forceinline__ uint32_t fetch_slot(bool want, uint32_t *counter) {
@@ -91,10 +107,11 @@ __device__ __forceinline__ uint32_t fetch_slot(bool want, uint32_t *counter) {
     return base + uint32_t(__popcll(below));
 }
 
-// Slot index within a scene -> (x, y): 8x8 tiles, tiles row-major.
-__device__ __forceinline__ void pixel_of(uint32_t k, uint32_t w, uint32_t &x, uint32_t &y) {
+// Slot index within a scene -> (x, y): 8x8 tiles, handed out in `order`
+// (a permutation of the row-major tiles) when given, else row-major.
+__device__ __forceinline__ void pixel_of(uint32_t k, uint32_t w, const uint32_t *order, uint32_t &x, uint32_t &y) {
     const uint32_t tiles_x = (w + 7u) >> 3;
-    const uint32_t tile = k >> 6, in = k & 63u;
+    const uint32_t tile = order ? order[k >> 6] : k >> 6, in = k & 63u;
     x = (tile % tiles_x) * 8u + (in & 7u);
     y = (tile / tiles_x) * 8u + (in >> 3);
 }
@@ -139,13 +156,15 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
                                                              float4 *__restrict__ accum, uint32_t start,
                                                              uint32_t count, uint32_t *__restrict__ counter,
                                                              uint32_t total_slots, int event_batch,
-                                                             DeviceProfile *prof, ProposerDevice prop) {
+                                                             DeviceProfile *prof, ProposerDevice prop,
+                                                             const uint32_t *__restrict__ tile_order) {
     const FptRenderConfig *cp = cfgs;  // the lane's scene (fixed when !MULTI)
     float4 *out = accum;               // the lane's scene accumulation buffer
 
     Work w = {0, 0};
     PersistentCounters pc = {0, 0, 0, 0, 0};
     LearnedCounters lc = {0, 0, 0, 0};
+    unsigned long long ls[LS_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     int pixels_done = 0;
 
     // Per-scene constants, refreshed when a lane takes a pixel.
@@ -295,7 +314,7 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
             load_scene();
         }
         uint32_t x, y;
-        pixel_of(local, W, x, y);
+        pixel_of(local, W, MULTI ? nullptr : tile_order, x, y);
         if (x >= W || y >= H) {
             retry = true;
             return;
@@ -316,6 +335,10 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
     while (__any(retry)) take_slot(retry);
 
     while (true) {
+        if (PROFILE) {
+            if (lane_id() == 0u) ls[LS_TRIPS] += 1u;
+            lane_stat(ls, -1, LS_IDLE_LANES, phase == PH_IDLE);
+        }
         float q = 0.f;
         bool use_q = false;
         if (LEARNED) {
@@ -325,6 +348,7 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
                 if (PROFILE) {
                     lc.proposer_calls += lane_id() == 0u;
                     lc.proposals += use_q;
+                    lane_stat(ls, -1, LS_PROPOSER_LANES, use_q);
                 }
             }
         }
@@ -346,6 +370,7 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
             while (true) {
                 const bool run = act && !fin;
                 if (!__any(run)) break;
+                if (PROFILE) lane_stat(ls, LS_MARCH_PASSES, LS_MARCH_LANES, run);
                 if (run) {
                     const v3<float> old = bz;
                     if (bip == 8) bulb_step<float, 8>(bz, br, bdr, bpower, 8);
@@ -364,6 +389,7 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
                 if (__popcll(__ballot(fin)) >= FPT_ORBIT_BATCH) break;
             }
             const FptRenderConfig &cfg = *cp;
+            if (PROFILE) lane_stat(ls, LS_FINISH_PASSES, LS_FINISH_LANES, fin);
             if (fin) {
                 borb = false;
                 w.orbit_iterations += boi;
@@ -384,6 +410,7 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
             }
         } else if (MB && phase == PH_MARCH) {
             // marchMandelbulber's loop (SM:3711-3733), one step per trip.
+            if (PROFILE) lane_stat(ls, LS_MARCH_PASSES, LS_MARCH_LANES, true);
             const FptRenderConfig &cfg = *cp;
             bool stop = it >= max_iter;
             if (!stop) {
@@ -408,6 +435,7 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
             }
             if (stop) phase = PH_EVENT;
         } else if (!BULB && phase == PH_MARCH) {
+            if (PROFILE) lane_stat(ls, LS_MARCH_PASSES, LS_MARCH_LANES, true);
             const FptRenderConfig &cfg = *cp;
             if (LEARNED && use_q && q > prop.switch_distance) {
                 // Far step on the proposal.
@@ -448,6 +476,7 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
         if (!(__popcll(ev) >= event_batch || (ev && !mar))) continue;
 
         bool need_slot = false;
+        if (PROFILE) lane_stat(ls, LS_EVENT_PASSES, LS_EVENT_LANES, phase == PH_EVENT);
         if (phase == PH_EVENT) {
             const FptRenderConfig &cfg = *cp;
             const float frame = float(s);
@@ -500,6 +529,7 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
                     path_done = true;
                 } else {
                     hit_p = p;
+                    if (PROFILE) lane_stat(ls, LS_NORMAL_PASSES, LS_NORMAL_LANES, true);
                     mat = MB ? material_any<SCENE_MB>(p, cfg, w) : user_material(p, cfg, w);
                     if (mat.emission > 0.001f) light = light + mat.rgb * mat.emission;
                     hit_n = MB ? normal_mandelbulber(p, cfg, w) : (LOWERED ? normal_scene(p, cfg, w) : normal_at(p, cfg, w));
@@ -621,6 +651,7 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
         wadd(&prof->normal_evals, (unsigned long long)pc.normal_evals);
         wadd(&prof->bounces, (unsigned long long)pc.bounces);
         wadd(&prof->pixels, (unsigned long long)pixels_done);
+        for (int k = 0; k < LS_COUNT; ++k) wadd(&prof->lane[k], ls[k]);
         if (LEARNED) {
             wadd(&prof->proposer_calls, (unsigned long long)lc.proposer_calls);
             wadd(&prof->proposals, (unsigned long long)lc.proposals);

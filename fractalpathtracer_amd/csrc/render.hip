@@ -223,6 +223,34 @@ __global__ __launch_bounds__(256) void trace_primary_kernel(const FptRenderConfi
     hits[size_t(y) * cfg.width + x] = h;
 }
 
+// Work estimate of each 8x8 tile for the persistent kernel's hand-out order:
+// the un-jittered primary rays of four pixels of the tile, each charged its
+// march steps plus 256 when it reaches the surface (a surface hit starts the
+// bounce and shadow marches that dominate a path's cost).
+__global__ __launch_bounds__(256) void tile_cost_kernel(const FptRenderConfig *__restrict__ cfgp, uint32_t tiles_x,
+                                                        uint32_t ntiles, float *__restrict__ cost) {
+    const FptRenderConfig &cfg = *cfgp;
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= ntiles) return;
+    const uint32_t x0 = (t % tiles_x) * 8u, y0 = (t / tiles_x) * 8u;
+    const f3 cam = ld3(cfg.camera_position);
+    const float far = cfg.render[4] * 0.99f;
+    float c = 0.0f;
+    for (uint32_t k = 0; k < 4u; ++k) {
+        const uint32_t x = min(x0 + ((k & 1u) ? 6u : 1u), cfg.width - 1u);
+        const uint32_t y = min(y0 + ((k & 2u) ? 6u : 1u), cfg.height - 1u);
+        const UV uv = screen_uv_any(x, y, cfg);
+        f3 dr = mk(0.f, 0.f, 0.f);
+        if (!camera_ray_any(uv.x, uv.y, cfg, dr)) continue;
+        Work w = {0, 0};
+        int steps = 0;
+        const f3 rp = march_any(dr, cam, int(cfg.render[1]), cfg.render[3], 0.0002f, cfg, w, steps);
+        const f3 tv = rp - cam;
+        c += float(steps) + (dot(tv, tv) <= far * far ? 256.0f : 0.0f);
+    }
+    cost[t] = c;
+}
+
 // Distance field samples {distance, radius, derivative, iterations}: the
 // Mandelbulber native KIFS field for sdf_id MANDELBULBER, otherwise the
 // scene distance with the orbit statistics of the program's first fractal
@@ -639,6 +667,14 @@ hipError_t launch_sdf_accumulate(const FptRenderConfig *dcfg, const FptRenderCon
 hipError_t launch_trace_primary(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, FptRayHit *hits,
                                 hipStream_t stream) {
     hipLaunchKernelGGL(trace_primary_kernel, tile_grid(hcfg.width, hcfg.height), dim3(256), 0, stream, dcfg, hits);
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_cost(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, float *cost,
+                            hipStream_t stream) {
+    const uint32_t tiles_x = (hcfg.width + 7u) / 8u, ntiles = tiles_x * ((hcfg.height + 7u) / 8u);
+    hipLaunchKernelGGL(tile_cost_kernel, dim3((ntiles + 255u) / 256u), dim3(256), 0, stream, dcfg, tiles_x, ntiles,
+                       cost);
     return hipGetLastError();
 }
 

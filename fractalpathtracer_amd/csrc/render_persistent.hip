@@ -36,23 +36,26 @@ namespace {
 template <bool PROFILE, bool MULTI, bool LEARNED, bool BULB, bool MB, bool LOWERED>
 void launch_instance(dim3 grid, hipStream_t stream, const FptRenderConfig *cfgs, const BatchScene *scenes,
                      uint32_t nscenes, float4 *accum, uint32_t start, uint32_t count, uint32_t *counter,
-                     uint32_t slots, int event_batch, DeviceProfile *prof, const ProposerDevice &prop) {
+                     uint32_t slots, int event_batch, DeviceProfile *prof, const ProposerDevice &prop,
+                     const uint32_t *order) {
     hipLaunchKernelGGL((sdf_persistent_kernel<PROFILE, MULTI, LEARNED, BULB, MB, LOWERED>), grid, dim3(256), 0,
                        stream, cfgs, scenes, nscenes, accum, start, count, counter, slots, event_batch,
-                       PROFILE ? prof : nullptr, prop);
+                       PROFILE ? prof : nullptr, prop, order);
 }
 
 template <bool MULTI, bool LEARNED, bool BULB, bool MB, bool LOWERED>
 void launch_profiled(bool profiled, dim3 grid, hipStream_t stream, const FptRenderConfig *cfgs,
                      const BatchScene *scenes, uint32_t nscenes, float4 *accum, uint32_t start, uint32_t count,
                      uint32_t *counter, uint32_t slots, int event_batch, DeviceProfile *prof,
-                     const ProposerDevice &prop) {
+                     const ProposerDevice &prop, const uint32_t *order) {
     if (profiled)
         launch_instance<true, MULTI, LEARNED, BULB, MB, LOWERED>(grid, stream, cfgs, scenes, nscenes, accum, start,
-                                                                  count, counter, slots, event_batch, prof, prop);
+                                                                  count, counter, slots, event_batch, prof, prop,
+                                                                  order);
     else
         launch_instance<false, MULTI, LEARNED, BULB, MB, LOWERED>(grid, stream, cfgs, scenes, nscenes, accum, start,
-                                                                   count, counter, slots, event_batch, prof, prop);
+                                                                   count, counter, slots, event_batch, prof, prop,
+                                                                   order);
 }
 
 }  // namespace
@@ -60,7 +63,7 @@ void launch_profiled(bool profiled, dim3 grid, hipStream_t stream, const FptRend
 hipError_t launch_sdf_persistent(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, float4 *accum,
                                  uint32_t start, uint32_t count, uint32_t *counter, uint32_t workgroups,
                                  int event_batch, DeviceProfile *prof, const ProposerDevice *prop,
-                                 hipStream_t stream) {
+                                 const uint32_t *tile_order, hipStream_t stream) {
     const uint32_t slots = slot_count(hcfg.width, hcfg.height);
     hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, counter);
     const ProposerDevice none{};
@@ -68,7 +71,7 @@ hipError_t launch_sdf_persistent(const FptRenderConfig *dcfg, const FptRenderCon
     const bool p = prof != nullptr, lowered = has_lowered_tables(hcfg);
 #define FPT_LAUNCH(LEARNED, BULB, MB, LOWERED, PROP)                                                              \
     launch_profiled<false, LEARNED, BULB, MB, LOWERED>(p, grid, stream, dcfg, nullptr, 1u, accum, start, count,  \
-                                                       counter, slots, event_batch, prof, PROP)
+                                                       counter, slots, event_batch, prof, PROP, tile_order)
     if (prop && lowered) FPT_LAUNCH(true, false, false, true, *prop);
     else if (prop) FPT_LAUNCH(true, false, false, false, *prop);
     else if (hcfg.sdf_id == FPT_SDF_MANDELBULBER) FPT_LAUNCH(false, false, true, false, none);
@@ -90,10 +93,12 @@ hipError_t launch_sdf_batch(const FptRenderConfig *dcfgs, const BatchScene *dsce
     const bool p = prof != nullptr;
     if (lowered)
         launch_profiled<true, false, false, false, true>(p, dim3(workgroups), stream, dcfgs, dscenes, nscenes, accum,
-                                                         0u, spp_cap, counter, total_slots, event_batch, prof, none);
+                                                         0u, spp_cap, counter, total_slots, event_batch, prof, none,
+                                                         nullptr);
     else
         launch_profiled<true, false, false, false, false>(p, dim3(workgroups), stream, dcfgs, dscenes, nscenes, accum,
-                                                          0u, spp_cap, counter, total_slots, event_batch, prof, none);
+                                                          0u, spp_cap, counter, total_slots, event_batch, prof, none,
+                                                          nullptr);
     return hipGetLastError();
 }
 

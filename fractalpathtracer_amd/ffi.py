@@ -241,6 +241,12 @@ class FptVoxelBuildInfo(C.Structure):
                 ("rejected_bricks", u32), ("_pad0", u32)]
 
 
+class FptLaneStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("loop_trips", "march_passes", "march_lanes", "finish_passes",
+                                          "finish_lanes", "event_passes", "event_lanes", "normal_passes",
+                                          "normal_lanes", "idle_lanes", "proposer_lanes", "_reserved")]
+
+
 class FptTuneResult(C.Structure):
     _fields_ = [("workgroups", u32), ("event_batch", u32), ("candidates", u32), ("rounds", u32),
                 ("best_ms", f32), ("default_ms", f32)]
@@ -277,7 +283,7 @@ EXPECTED_SIZES = {
     FptRenderConfig: 30448, FptSdfProfileStats: 288, FptDiagnosticConfig: 24,
     FptMandelbulberFieldSample: 16, FptAsyncJitStats: 40, FptRayHit: 40, FptVoxelBuildInfo: 48,
     FptTuneResult: 24, FptProposerSettings: 16, FptProposerStats: 32, FptSpecializationStats: 80,
-    FptSceneOptions: 56, FptVoxelTraceStats: 32,
+    FptSceneOptions: 56, FptVoxelTraceStats: 32, FptLaneStats: 96,
 }
 
 PKG_DIR = Path(__file__).resolve().parent
@@ -296,7 +302,7 @@ EXPORTED_SYMBOLS = (
     "fpt_hip_autotune", "fpt_hip_device_alloc", "fpt_hip_device_free", "fpt_hip_copy", "fpt_hip_fill_f32",
     "fpt_hip_stream_create", "fpt_hip_stream_destroy", "fpt_hip_synchronize", "fpt_hip_time_renders",
     "fpt_hip_proposer_samples", "fpt_hip_proposer_eval", "fpt_hip_plan_set_proposer", "fpt_hip_plan_profile",
-    "fpt_hip_plan_voxel_stats",
+    "fpt_hip_plan_voxel_stats", "fpt_hip_plan_lane_stats",
     "fpt_hip_diagnostic_render", "fpt_hip_plan_specialize", "fpt_hip_plan_specialized_surface",
     "fpt_hip_topology_source", "fpt_hip_hydrate_hdri",
     "fpt_scene_default_config", "fpt_scene_load", "fpt_scene_load_fpt_settings", "fpt_scene_apply_program_json",
@@ -419,6 +425,8 @@ def _declare(lib):
     lib.fpt_hip_diagnostic_render.restype = C.c_int
     lib.fpt_hip_plan_specialized_surface.argtypes = [C.c_void_p, P(f32), C.c_size_t, P(f32), cp, C.c_size_t]
     lib.fpt_hip_plan_specialized_surface.restype = C.c_int
+    lib.fpt_hip_plan_lane_stats.argtypes = [C.c_void_p, C.c_void_p, P(FptLaneStats), cp, C.c_size_t]
+    lib.fpt_hip_plan_lane_stats.restype = C.c_int
     lib.fpt_hip_plan_specialize.argtypes = [C.c_void_p, u32, P(FptSpecializationStats), cp, C.c_size_t]
     lib.fpt_hip_plan_specialize.restype = C.c_int
     lib.fpt_hip_topology_source.argtypes = [P(FptRenderConfig), cp, C.c_size_t, P(C.c_size_t), cp, C.c_size_t]

@@ -1102,6 +1102,40 @@ int fpt_hip_plan_profile(FptHipPlan *plan,
                          char *error,
                          size_t error_len);
 
+/*
+ * Lane occupancy of the persistent SDF kernel over a plan's full frame (an
+ * untimed counting pass, same samples as a render; no counterpart in the
+ * reference).  Counts are summed over waves: a "pass" is one execution of a
+ * code region by a wave, "lanes" the lanes active in it, so lanes / (64
+ * passes) is that region's lane utilisation.
+ *   loop_trips          trips of the main state-machine loop
+ *   march_passes/lanes  march steps (one distance evaluation, or one orbit
+ *                       iteration in the interleaved single-Mandelbulb kernel)
+ *   finish_passes/lanes distance finalisations + march steps of the
+ *                       interleaved kernel (0 otherwise)
+ *   event_passes/lanes  per-hit blocks (hit, shadow end, scatter)
+ *   normal_passes/lanes analytic normals inside the per-hit blocks
+ *   idle_lanes          lanes without work, summed over loop trips
+ *   proposer_lanes      learned-step proposals used (lanes of MLP calls)
+ */
+struct FptLaneStats {
+    uint64_t loop_trips;
+    uint64_t march_passes, march_lanes;
+    uint64_t finish_passes, finish_lanes;
+    uint64_t event_passes, event_lanes;
+    uint64_t normal_passes, normal_lanes;
+    uint64_t idle_lanes;
+    uint64_t proposer_lanes;
+    uint64_t _reserved;
+};
+FPT_ABI_ASSERT(sizeof(struct FptLaneStats) == 96, "FptLaneStats");
+
+int fpt_hip_plan_lane_stats(FptHipPlan *plan,
+                            float *device_accum,
+                            struct FptLaneStats *stats,
+                            char *error,
+                            size_t error_len);
+
 /* Memory work of the voxel trace over a plan's full frame: every traversal
  * (camera, bounce and shadow rays) of every sample.  No counterpart in the
  * reference (its profile categories, Shaders.metal:7650-7704, are SDF-only);
