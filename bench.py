@@ -148,12 +148,13 @@ def oracle_band(cfg, rows: int, ref_factor: int):
     t = time.perf_counter()
     conv = bind.render_samples(cfg, (s0, s1), rows=(y0, y1), threads=threads)[y0:y1]
     dt = time.perf_counter() - t
+    counts = bind.counters()
     samples = rows * cfg.width * (s1 - s0)
     base = {"value": samples / dt, "unit": "path samples/s", "cores": threads, "kind": "port",
             "sample": f"FP64 C++ oracle (OpenMP), rows {y0}-{y1 - 1} of the {cfg.width}x{cfg.height} frame, "
                       f"samples {s0}-{s1 - 1} per pixel ({samples} path samples, {dt:.1f} s)",
             "cpu_model": cpu_model(), "single_core": single_core_rate(cfg, y0)}
-    return base, same, conv, (y0, y1)
+    return base, same, conv, (y0, y1), counts
 
 
 def frame_flops(prof: dict) -> float:
@@ -241,9 +242,10 @@ def main():
 
     base = None
     gate = None
+    band_counts = None
     chosen = "exact"
     if not args.no_cpu_baseline:
-        base, same, conv, (y0, y1) = oracle_band(cfg, args.cpu_rows, args.ref_spp_factor)
+        base, same, conv, (y0, y1), band_counts = oracle_band(cfg, args.cpu_rows, args.ref_spp_factor)
         for m in methods.values():
             band = m["img"][y0:y1, :, :3]
             m["rmse_same"] = float(np.sqrt(np.mean((band - same[..., :3]) ** 2)))
@@ -270,10 +272,20 @@ def main():
              "orbit_iterations": mm["prof"]["march_orbit_iterations"]}
         if k == "learned":
             L = mm["learned"]
-            mlp_flops = L["proposer_calls"] * 64 * FLOPS_PER_PROPOSAL
+            # Useful work: one network evaluation per proposal a lane used;
+            # issued work: every wave-wide call computes 64 points.
+            secs = mm["ms_per_step"] / 1e3
             s.update(L)
-            s["mfma_tflops"] = mlp_flops / (mm["ms_per_step"] / 1e3) / 1e12
+            s["mfma_tflops"] = L["proposals"] * FLOPS_PER_PROPOSAL / secs / 1e12
             s["mfma_frac_of_bf16_peak"] = s["mfma_tflops"] / BF16_PEAK_TFLOPS
+            s["mfma_tflops_issued"] = L["proposer_calls"] * 64 * FLOPS_PER_PROPOSAL / secs / 1e12
+            s["proposals_per_call"] = L["proposals"] / max(L["proposer_calls"], 1)
+            if band_counts:
+                # Even a free proposer only removes the exact far-field
+                # steps (|d| > 0.02): the bound on its speed-up.
+                share = band_counts["far_orbit_iterations"] / max(band_counts["orbit_iterations"], 1)
+                s["learned_ceiling"] = {"far_field_orbit_iteration_share": share, "speedup_bound": 1.0 / (1.0 - share),
+                                        "source": "FP64 oracle work counters on the converged band"}
             s["speedup_vs_exact"] = mm["value"] / methods["exact"]["value"]
             s["training"] = {k2: mm["train"].get(k2) for k2 in
                              ("samples", "steps", "sample_seconds", "train_seconds", "mae_far")}
@@ -489,7 +501,7 @@ def run_workload(args, api, scenes):
         line["config"]["counts"] = {k: prof[k] for k in ("distance_evals", "march_orbit_iterations",
                                                          "normal_evals", "primary_steps", "shadow_steps")}
         if not args.no_cpu_baseline:
-            base, _, _, _ = oracle_band(cfg, args.cpu_rows, 1)
+            base, _, _, _, _ = oracle_band(cfg, args.cpu_rows, 1)
             line["cpu_baseline"] = base
     else:
         st = plan.voxel_stats(accum)

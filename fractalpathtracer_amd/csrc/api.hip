@@ -958,6 +958,7 @@ struct FptHipBatch {
     FptRenderConfig *dcfgs = nullptr;
     fpt::BatchScene *dscenes = nullptr;
     uint32_t *counter = nullptr;
+    uint32_t *tile_order = nullptr;  // longest-first order of the batch's tiles
     uint32_t total_slots = 0;
     size_t pixels = 0;
     bool lowered = false;  // some scene carries lowered program tables
@@ -1053,6 +1054,29 @@ int fpt_hip_batch_create(const FptRenderConfig *configs, uint32_t count, FptHipB
         fpt_hip_batch_destroy(b);
         return 1;
     }
+    // Longest-first hand-out of the batch's tiles, over all scenes.
+    {
+        const uint32_t ntiles = b->total_slots / 64u;
+        DevBuf cost;
+        e = cost.alloc(size_t(ntiles) * sizeof(float));
+        for (uint32_t i = 0; i < count && e == hipSuccess; ++i)
+            e = fpt::launch_tile_cost(b->dcfgs + i, b->host[i], cost.as<float>() + b->scenes[i].slot_begin / 64u,
+                                      nullptr);
+        std::vector<float> h(ntiles);
+        if (e == hipSuccess)
+            e = hipMemcpy(h.data(), cost.p, size_t(ntiles) * sizeof(float), hipMemcpyDeviceToHost);
+        std::vector<uint32_t> order(ntiles);
+        for (uint32_t i = 0; i < ntiles; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return h[x] > h[y]; });
+        if (e == hipSuccess) e = hipMalloc(&b->tile_order, size_t(ntiles) * sizeof(uint32_t));
+        if (e == hipSuccess)
+            e = hipMemcpy(b->tile_order, order.data(), size_t(ntiles) * sizeof(uint32_t), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            set_error(error, error_len, "batch tile order failed: %s", hipGetErrorString(e));
+            fpt_hip_batch_destroy(b);
+            return 1;
+        }
+    }
     *out = b;
     return 0;
 }
@@ -1085,7 +1109,8 @@ int fpt_hip_batch_render(FptHipBatch *batch, float *device_accum, void *stream, 
     float4 *acc = reinterpret_cast<float4 *>(device_accum);
     if (!batch->use_graph) {
         HIP_TRY(fpt::launch_sdf_batch(batch->dcfgs, batch->dscenes, uint32_t(batch->host.size()), batch->total_slots, acc,
-                                      batch->counter, wg, int(batch->event_batch), 0u, nullptr, batch->lowered, s),
+                                      batch->counter, wg, int(batch->event_batch), 0u, nullptr, batch->lowered,
+                                      batch->tile_order, s),
                 "batch kernel launch");
         return 0;
     }
@@ -1097,7 +1122,8 @@ int fpt_hip_batch_render(FptHipBatch *batch, float *device_accum, void *stream, 
         hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess)
             e = fpt::launch_sdf_batch(batch->dcfgs, batch->dscenes, uint32_t(batch->host.size()), batch->total_slots,
-                                      acc, batch->counter, wg, int(batch->event_batch), 0u, nullptr, batch->lowered, cap);
+                                      acc, batch->counter, wg, int(batch->event_batch), 0u, nullptr, batch->lowered,
+                                      batch->tile_order, cap);
         hipGraph_t g = nullptr;
         hipError_t e2 = hipStreamEndCapture(cap, &g);
         (void)hipStreamDestroy(cap);
@@ -1123,6 +1149,7 @@ void fpt_hip_batch_destroy(FptHipBatch *batch) {
     if (batch->dcfgs) (void)hipFree(batch->dcfgs);
     if (batch->dscenes) (void)hipFree(batch->dscenes);
     if (batch->counter) (void)hipFree(batch->counter);
+    if (batch->tile_order) (void)hipFree(batch->tile_order);
     delete batch;
 }
 
@@ -1272,7 +1299,7 @@ int fpt_hip_render_batch_noalloc(const FptRenderConfig *configs, uint32_t count,
             "scene table upload");
     if (need_focus) HIP_TRY(fpt::launch_focus_prepass(dcfgs, count, s), "focus-distance prepass");
     HIP_TRY(fpt::launch_sdf_batch(dcfgs, dscenes, count, uint32_t(slot), reinterpret_cast<float4 *>(device_accum),
-                                  counter, default_workgroups(), 24, 0u, nullptr, lowered, s),
+                                  counter, default_workgroups(), 24, 0u, nullptr, lowered, nullptr, s),
             "batch kernel launch");
     return 0;
 }
@@ -1312,7 +1339,7 @@ int fpt_hip_autotune(FptHipPlan *plan, FptHipBatch *batch, float *device_accum, 
                                               plan->proposer.enabled ? &plan->proposer.dev : nullptr, plan->tile_order,
                                               s);
         return fpt::launch_sdf_batch(batch->dcfgs, batch->dscenes, uint32_t(batch->host.size()), batch->total_slots, acc,
-                                     batch->counter, wg, int(eb), 1u, nullptr, batch->lowered, s);
+                                     batch->counter, wg, int(eb), 1u, nullptr, batch->lowered, batch->tile_order, s);
     };
     HIP_TRY(launch(cand[0].first, cand[0].second), "autotune warm-up");
     for (uint32_t r = 0; r < rounds; ++r) {

@@ -56,10 +56,12 @@ bool single_bulb_program(const FptRenderConfig &c);
 // The scene's program carries flat-union, canonical or typed-SoA tables: the
 // kernels' instances with the table evaluators apply.
 bool has_lowered_tables(const FptRenderConfig &c);
+// tile_order (optional, device): hand-out order of the batch's global 8x8
+// tiles (scene after scene, row-major within a scene).
 hipError_t launch_sdf_batch(const FptRenderConfig *dcfgs, const BatchScene *dscenes, uint32_t nscenes,
                             uint32_t total_slots, float4 *accum, uint32_t *counter, uint32_t workgroups,
                             int event_batch, uint32_t spp_cap, DeviceProfile *prof, bool lowered,
-                            hipStream_t stream);
+                            const uint32_t *tile_order, hipStream_t stream);
 
 hipError_t launch_sdf_accumulate(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, float4 *accum,
                                  uint32_t start, uint32_t count, DeviceProfile *prof, hipStream_t stream);

@@ -126,7 +126,11 @@ __host__ __device__ __forceinline__ uint32_t slot_count(uint32_t w, uint32_t h) 
 #define FPT_PERSISTENT_WAVES 2
 #endif
 #ifndef FPT_LEARNED_WAVES
-#define FPT_LEARNED_WAVES 1
+#define FPT_LEARNED_WAVES 2
+#endif
+// Lanes that must want a learned proposal before the network runs for them.
+#ifndef FPT_PROPOSER_BATCH
+#define FPT_PROPOSER_BATCH 32
 #endif
 // Interleaved-orbit variant for a single Mandelbulb leaf (on/off), and its
 // finished lanes per finalisation batch.
@@ -298,11 +302,14 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
         }
         uint32_t local = k;
         if (MULTI) {
+            // Slot in the hand-out order: tile_order permutes the batch's global
+            // 8x8 tiles (scene after scene, row-major within a scene).
+            const uint32_t g = tile_order ? tile_order[k >> 6] * 64u + (k & 63u) : k;
             // Binary search of the scene owning slot k.
             uint32_t lo = 0, hi = nscenes - 1u;
             while (lo < hi) {
                 uint32_t mid = (lo + hi + 1u) >> 1;
-                if (scenes[mid].slot_begin <= k) lo = mid;
+                if (scenes[mid].slot_begin <= g) lo = mid;
                 else hi = mid - 1u;
             }
             const BatchScene sc = scenes[lo];
@@ -310,7 +317,7 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
             out = accum + sc.accum_offset;
             W = sc.width;
             H = sc.height;
-            local = k - sc.slot_begin;
+            local = g - sc.slot_begin;
             load_scene();
         }
         uint32_t x, y;
@@ -340,16 +347,25 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
             lane_stat(ls, -1, LS_IDLE_LANES, phase == PH_IDLE);
         }
         float q = 0.f;
-        bool use_q = false;
+        bool use_q = false, wait_q = false;
         if (LEARNED) {
-            use_q = phase == PH_MARCH && lfar && proposer_in_box(prop, p);
-            if (__any(use_q)) {  // wave-uniform: the MLP needs every lane
+            // The network is wave-collective (every lane runs it), so it is
+            // evaluated for a batch: once FPT_PROPOSER_BATCH lanes want a
+            // proposal, or when no marching lane has an exact step to take
+            // meanwhile.  Until then the lanes that want one wait.
+            const bool want = phase == PH_MARCH && lfar && proposer_in_box(prop, p);
+            const unsigned long long wm = __ballot(want);
+            const unsigned long long exact_m = __ballot(phase == PH_MARCH && !want);
+            if (wm && (__popcll(wm) >= FPT_PROPOSER_BATCH || !exact_m)) {
+                use_q = want;
                 q = proposer_eval(prop, p);
                 if (PROFILE) {
                     lc.proposer_calls += lane_id() == 0u;
                     lc.proposals += use_q;
                     lane_stat(ls, -1, LS_PROPOSER_LANES, use_q);
                 }
+            } else {
+                wait_q = want;
             }
         }
         if (BULB) {
@@ -434,7 +450,7 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
                 }
             }
             if (stop) phase = PH_EVENT;
-        } else if (!BULB && phase == PH_MARCH) {
+        } else if (!BULB && phase == PH_MARCH && !(LEARNED && wait_q)) {
             if (PROFILE) lane_stat(ls, LS_MARCH_PASSES, LS_MARCH_LANES, true);
             const FptRenderConfig &cfg = *cp;
             if (LEARNED && use_q && q > prop.switch_distance) {

@@ -87,18 +87,18 @@ hipError_t launch_sdf_persistent(const FptRenderConfig *dcfg, const FptRenderCon
 hipError_t launch_sdf_batch(const FptRenderConfig *dcfgs, const BatchScene *dscenes, uint32_t nscenes,
                             uint32_t total_slots, float4 *accum, uint32_t *counter, uint32_t workgroups,
                             int event_batch, uint32_t spp_cap, DeviceProfile *prof, bool lowered,
-                            hipStream_t stream) {
+                            const uint32_t *tile_order, hipStream_t stream) {
     hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, counter);
     const ProposerDevice none{};
     const bool p = prof != nullptr;
     if (lowered)
         launch_profiled<true, false, false, false, true>(p, dim3(workgroups), stream, dcfgs, dscenes, nscenes, accum,
                                                          0u, spp_cap, counter, total_slots, event_batch, prof, none,
-                                                         nullptr);
+                                                         tile_order);
     else
         launch_profiled<true, false, false, false, false>(p, dim3(workgroups), stream, dcfgs, dscenes, nscenes, accum,
                                                           0u, spp_cap, counter, total_slots, event_batch, prof, none,
-                                                          nullptr);
+                                                          tile_order);
     return hipGetLastError();
 }
 
