@@ -70,7 +70,7 @@ FLOPS_PER_PROPOSAL = 2 * (128 * 64 + 3 * 64 * 64 + 64)
 # passes (profiles/sdf_persistent_mandelbulb_1080p_64spp.md): FETCH_SIZE and
 # WRITE_SIZE KiB per dispatch, scaled by what the same counters read for a
 # 1 GiB copy kernel (x2.000 and x1.000).
-TRAFFIC_FETCH_KIB, TRAFFIC_WRITE_KIB = 1606.125, 116832.9375
+TRAFFIC_FETCH_KIB, TRAFFIC_WRITE_KIB = 2090.938, 113180.438
 TRAFFIC_FETCH_SCALE, TRAFFIC_WRITE_SCALE = 2.0, 1.0
 TRAFFIC_WORKLOAD = (1920, 1080, 64)
 
@@ -165,6 +165,13 @@ def frame_special_ops(prof: dict) -> float:
     return counted_flops(prof, SPECIAL_PER_ORBIT_ITERATION, SPECIAL_PER_DISTANCE_EVAL)
 
 
+def tune_dict(t) -> dict:
+    """The autotuner's choice (fpt_hip_autotune) as JSON."""
+    return {"workgroups": t.workgroups, "event_batch": t.event_batch,
+            "tile_order": "longest-first" if t.tile_order else "row-major", "best_ms": t.best_ms,
+            "default_ms": t.default_ms, "candidates": t.candidates, "rounds": t.rounds}
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -226,6 +233,7 @@ def main():
     stream = api.Stream()
     accum = api.DeviceBuffer(cfg.width * cfg.height * 4)
     plan = api.Plan(cfg)
+    tune = plan.autotune(accum.address, stream.address, rounds=2)
     methods = {}
     ms, img, prof, _ = run_method(api, plan, cfg, stream, accum, args.steps, args.warmup)
     methods["exact"] = {"ms_per_step": ms, "value": samples_per_step / (ms / 1e3), "img": img, "prof": prof}
@@ -306,7 +314,7 @@ def main():
         "config": {"workload": f"Mandelbulb p8/12it/bailout2, {cfg.width}x{cfg.height}, {cfg.samples} spp, "
                                f"4 bounces", "method": chosen, "rmse_gate": gate,
                    "width": cfg.width, "height": cfg.height, "spp": cfg.samples, "bounces": 4,
-                   "methods": summary},
+                   "autotuned": tune_dict(tune), "methods": summary},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": KERNEL, "launches_per_step": launches, "launch_ms": launch_ms,
@@ -379,8 +387,7 @@ def run_batch(args, api, scenes):
                                    "U[-2.5,2.5], Menger U{3..6} iterations), tiles U{64..512}^2, spp U{1..16}, "
                                    "2 bounces, seed 0; one graph-replayed persistent launch",
                        "scenes": len(cfgs), "pixels": batch.pixels, "samples_per_step": samples_per_step,
-                       "autotuned": {"workgroups": tune.workgroups, "event_batch": tune.event_batch,
-                                     "best_ms": tune.best_ms, "default_ms": tune.default_ms},
+                       "autotuned": tune_dict(tune),
                        "one_launch_per_scene_ms": solo_ms, "speedup_vs_one_launch_per_scene": solo_ms / ms,
                        "counts": counts},
             "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -431,6 +438,9 @@ def run_workload(args, api, scenes):
     t0 = time.perf_counter()
     plan = api.Plan(cfg)
     build_s = time.perf_counter() - t0
+    tune = None
+    if args.workload in ("typed", "lowered"):  # the persistent-kernel workloads
+        tune = plan.autotune(accum.address, stream.address, rounds=2)
     if args.warmup:
         api.time_renders(plan, accum, stream, args.warmup)
     ms = api.time_renders(plan, accum, stream, args.steps) / args.steps
@@ -440,6 +450,8 @@ def run_workload(args, api, scenes):
             "dtype": "f32", "data": "synthetic",
             "config": {"workload": desc, "width": cfg.width, "height": cfg.height, "spp": cfg.samples,
                        "plan_create_s": build_s}}
+    if tune is not None:
+        line["config"]["autotuned"] = tune_dict(tune)
     if args.workload == "fract":
         prof, _ = plan.profile(accum)
         # per KIFS iteration: 3 abs folds, 3 plane reflections (dot 5, test 1,
@@ -470,6 +482,9 @@ def run_workload(args, api, scenes):
                             "flops_per_launch": flops, "flops_per_distance_eval": per_eval, "lower_bound": True}
         interp = scenes.primitive_union_scene(lowered=False)
         with api.Plan(interp) as iplan:
+            # the table plan's autotuned launch shape, for a like-for-like comparison
+            iplan.tune(1, tune.workgroups, tune.event_batch)
+            iplan.set_tile_order(tune.tile_order)
             if args.warmup:
                 api.time_renders(iplan, accum, stream, args.warmup)
             ims = api.time_renders(iplan, accum, stream, args.steps) / args.steps
@@ -518,6 +533,30 @@ def run_workload(args, api, scenes):
                             "kernel": "voxel_accumulate_table_kernel",
                             "launches_per_step": launches, "bytes_per_step": nbytes}
         line["config"]["counts"] = st
+        if not args.no_cpu_baseline:
+            # The FP32 oracle's voxel path tracer over the GPU's own tables
+            # (read back), on a band of rows: the same traversal and shading;
+            # pixels differ only where a path's direction, drawn with libm's
+            # trigonometry on the CPU, rounds differently and takes another
+            # cell.
+            from oracle import bind
+            threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+            img = accum.numpy().reshape(cfg.height, cfg.width, 4)
+            tables = api.voxel_build(cfg)
+            y0 = cfg.height // 2 - args.cpu_rows * 4
+            y1 = y0 + args.cpu_rows * 8
+            t = time.perf_counter()
+            band = bind.voxel_render_tables(cfg, tables, threads=threads, rows=(y0, y1))[y0:y1]
+            dt = time.perf_counter() - t
+            n = (y1 - y0) * cfg.width * cfg.samples
+            line["cpu_baseline"] = {"value": n / dt, "unit": "path samples/s", "cores": threads, "kind": "port",
+                                    "sample": f"FP32 C++ oracle voxel path tracer (OpenMP) over the GPU-built "
+                                              f"tables read back, rows {y0}-{y1 - 1} at {cfg.samples} spp "
+                                              f"({n} path samples, {dt:.1f} s)",
+                                    "cpu_model": cpu_model(),
+                                    "band_equal_pixel_fraction": float(np.mean(np.all(
+                                        band[..., :3].astype(np.float32) == img[y0:y1, :, :3], axis=-1))),
+                                    "band_max_abs_diff": float(np.abs(band[..., :3] - img[y0:y1, :, :3]).max())}
     plan.close()
     print(json.dumps(line), flush=True)
 

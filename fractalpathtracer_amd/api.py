@@ -397,6 +397,11 @@ class Plan:
         err = ffi.error_buffer()
         ffi.check(self.lib.fpt_hip_plan_tune(self.handle, kernel, workgroups, event_batch, err, len(err)), err)
 
+    def set_tile_order(self, mode: int) -> None:
+        """1 longest-first (default), 0 row-major (fpt_hip_set_tile_order)."""
+        err = ffi.error_buffer()
+        ffi.check(self.lib.fpt_hip_set_tile_order(self.handle, None, mode, err, len(err)), err)
+
     def autotune(self, device_ptr: int, stream_ptr: int = 0, rounds: int = 3) -> ffi.FptTuneResult:
         res = ffi.FptTuneResult()
         err = ffi.error_buffer()
@@ -506,6 +511,11 @@ class Batch:
         err = ffi.error_buffer()
         ffi.check(self.lib.fpt_hip_batch_tune(self.handle, workgroups, event_batch, int(use_graph), err, len(err)),
                   err)
+
+    def set_tile_order(self, mode: int) -> None:
+        """1 longest-first (default), 0 row-major (fpt_hip_set_tile_order)."""
+        err = ffi.error_buffer()
+        ffi.check(self.lib.fpt_hip_set_tile_order(None, self.handle, mode, err, len(err)), err)
 
     def autotune(self, device_ptr: int, stream_ptr: int = 0, rounds: int = 3) -> ffi.FptTuneResult:
         res = ffi.FptTuneResult()

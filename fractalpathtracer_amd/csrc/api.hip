@@ -446,6 +446,7 @@ struct FptHipPlan {
     // first (longest-processing-time order from a probe of primary rays), so
     // the frame's tail is made of cheap tiles and lanes finish together.
     uint32_t *tile_order = nullptr;
+    uint32_t tile_order_mode = 1;  // 1 longest-first, 0 row-major (fpt_hip_set_tile_order)
     fpt::ProposerHost proposer;  // learned stepping (fpt_hip_plan_set_proposer)
     // Topology-specialised persistent kernel (fpt_hip_plan_specialize).
     std::shared_ptr<fpt::SpecializedModule> spec;
@@ -475,16 +476,25 @@ hipError_t launch_sdf(FptHipPlan *plan, float4 *acc, uint32_t start, uint32_t ch
         if (e != hipSuccess) return e;
         return fpt::launch_specialized(*plan->spec, plan->dcfg, acc, start, chunk, plan->counter,
                                        fpt::persistent_slots(c.width, c.height), wg, int(plan->event_batch),
-                                       plan->tile_order, s);
+                                       plan->tile_order_mode ? plan->tile_order : nullptr, s);
     }
     return fpt::launch_sdf_persistent(plan->dcfg, c, acc, start, chunk, plan->counter, wg, int(plan->event_batch),
-                                      prof, plan->proposer.enabled ? &plan->proposer.dev : nullptr, plan->tile_order,
-                                      s);
+                                      prof, plan->proposer.enabled ? &plan->proposer.dev : nullptr,
+                                      plan->tile_order_mode ? plan->tile_order : nullptr, s);
 }
 
-// The persistent kernel's tile order: tiles sorted by the probe's work
-// estimate, most first (ties in row-major order).  Any order gives the same
+// Hand-out order of tiles from their probe estimates (tile_cost_kernel):
+// most work first, ties in row-major order.  Cheap tiles then form the
+// frame's tail, so lanes run out of work together.  Any order gives the same
 // image, since every pixel is rendered by one lane alone.
+std::vector<uint32_t> tile_order_from_costs(const std::vector<float> &cost) {
+    std::vector<uint32_t> order(cost.size());
+    for (uint32_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
+    return order;
+}
+
+// The persistent kernel's tile order (tile_order_from_costs).
 int plan_tile_order(FptHipPlan *plan, char *error, size_t error_len) {
     const FptRenderConfig &c = plan->host;
     const uint32_t tiles_x = (c.width + 7u) / 8u, ntiles = tiles_x * ((c.height + 7u) / 8u);
@@ -493,9 +503,7 @@ int plan_tile_order(FptHipPlan *plan, char *error, size_t error_len) {
     HIP_TRY(fpt::launch_tile_cost(plan->dcfg, c, cost.as<float>(), nullptr), "tile cost kernel");
     std::vector<float> h(ntiles);
     HIP_TRY(hipMemcpy(h.data(), cost.p, size_t(ntiles) * sizeof(float), hipMemcpyDeviceToHost), "tile cost readback");
-    std::vector<uint32_t> order(ntiles);
-    for (uint32_t i = 0; i < ntiles; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return h[a] > h[b]; });
+    const std::vector<uint32_t> order = tile_order_from_costs(h);
     HIP_TRY(hipMalloc(&plan->tile_order, size_t(ntiles) * sizeof(uint32_t)), "tile order allocation");
     HIP_TRY(hipMemcpy(plan->tile_order, order.data(), size_t(ntiles) * sizeof(uint32_t), hipMemcpyHostToDevice),
             "tile order upload");
@@ -959,6 +967,7 @@ struct FptHipBatch {
     fpt::BatchScene *dscenes = nullptr;
     uint32_t *counter = nullptr;
     uint32_t *tile_order = nullptr;  // longest-first order of the batch's tiles
+    uint32_t tile_order_mode = 1;    // 1 longest-first, 0 row-major
     uint32_t total_slots = 0;
     size_t pixels = 0;
     bool lowered = false;  // some scene carries lowered program tables
@@ -970,7 +979,7 @@ struct FptHipBatch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     float *graph_accum = nullptr;
-    uint32_t graph_wg = 0, graph_eb = 0;
+    uint32_t graph_wg = 0, graph_eb = 0, graph_order = 0;
 };
 
 namespace {
@@ -1065,9 +1074,7 @@ int fpt_hip_batch_create(const FptRenderConfig *configs, uint32_t count, FptHipB
         std::vector<float> h(ntiles);
         if (e == hipSuccess)
             e = hipMemcpy(h.data(), cost.p, size_t(ntiles) * sizeof(float), hipMemcpyDeviceToHost);
-        std::vector<uint32_t> order(ntiles);
-        for (uint32_t i = 0; i < ntiles; ++i) order[i] = i;
-        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return h[x] > h[y]; });
+        const std::vector<uint32_t> order = tile_order_from_costs(h);
         if (e == hipSuccess) e = hipMalloc(&b->tile_order, size_t(ntiles) * sizeof(uint32_t));
         if (e == hipSuccess)
             e = hipMemcpy(b->tile_order, order.data(), size_t(ntiles) * sizeof(uint32_t), hipMemcpyHostToDevice);
@@ -1110,12 +1117,12 @@ int fpt_hip_batch_render(FptHipBatch *batch, float *device_accum, void *stream, 
     if (!batch->use_graph) {
         HIP_TRY(fpt::launch_sdf_batch(batch->dcfgs, batch->dscenes, uint32_t(batch->host.size()), batch->total_slots, acc,
                                       batch->counter, wg, int(batch->event_batch), 0u, nullptr, batch->lowered,
-                                      batch->tile_order, s),
+                                      batch->tile_order_mode ? batch->tile_order : nullptr, s),
                 "batch kernel launch");
         return 0;
     }
     if (!batch->exec || batch->graph_accum != device_accum || batch->graph_wg != wg ||
-        batch->graph_eb != batch->event_batch) {
+        batch->graph_eb != batch->event_batch || batch->graph_order != batch->tile_order_mode) {
         batch_drop_graph(batch);
         hipStream_t cap = nullptr;
         HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "capture stream");
@@ -1123,7 +1130,7 @@ int fpt_hip_batch_render(FptHipBatch *batch, float *device_accum, void *stream, 
         if (e == hipSuccess)
             e = fpt::launch_sdf_batch(batch->dcfgs, batch->dscenes, uint32_t(batch->host.size()), batch->total_slots,
                                       acc, batch->counter, wg, int(batch->event_batch), 0u, nullptr, batch->lowered,
-                                      batch->tile_order, cap);
+                                      batch->tile_order_mode ? batch->tile_order : nullptr, cap);
         hipGraph_t g = nullptr;
         hipError_t e2 = hipStreamEndCapture(cap, &g);
         (void)hipStreamDestroy(cap);
@@ -1138,6 +1145,7 @@ int fpt_hip_batch_render(FptHipBatch *batch, float *device_accum, void *stream, 
         batch->graph_accum = device_accum;
         batch->graph_wg = wg;
         batch->graph_eb = batch->event_batch;
+        batch->graph_order = batch->tile_order_mode;
     }
     HIP_TRY(hipGraphLaunch(batch->exec, s), "batch graph launch");
     return 0;
@@ -1323,9 +1331,13 @@ int fpt_hip_autotune(FptHipPlan *plan, FptHipBatch *batch, float *device_accum, 
     if (plan && reject_mandelbulber(&plan->host, "the persistent-kernel autotuner", error, error_len)) return 1;
     hipStream_t s = static_cast<hipStream_t>(stream);
     const uint32_t cus = default_workgroups() / 2u;
-    std::vector<std::pair<uint32_t, uint32_t>> cand;
-    for (uint32_t per_cu : {1u, 2u, 3u, 4u})
-        for (uint32_t eb : {8u, 16u, 24u, 32u, 40u}) cand.push_back({per_cu * cus, eb});
+    struct Shape {
+        uint32_t workgroups, event_batch, order;
+    };
+    std::vector<Shape> cand;
+    for (uint32_t order : {1u, 0u})
+        for (uint32_t per_cu : {1u, 2u, 3u, 4u})
+            for (uint32_t eb : {8u, 16u, 24u, 32u, 40u}) cand.push_back({per_cu * cus, eb, order});
     rounds = std::max<uint32_t>(1u, std::min<uint32_t>(rounds, 16u));
     std::vector<std::vector<float>> times(cand.size());
     hipEvent_t e0, e1;
@@ -1333,20 +1345,24 @@ int fpt_hip_autotune(FptHipPlan *plan, FptHipBatch *batch, float *device_accum, 
     HIP_TRY(hipEventCreate(&e1), "hipEventCreate");
     float4 *acc = reinterpret_cast<float4 *>(device_accum);
     if (plan && !plan->counter) HIP_TRY(hipMalloc(&plan->counter, sizeof(uint32_t)), "counter allocation");
-    auto launch = [&](uint32_t wg, uint32_t eb) -> hipError_t {
-        if (plan)
-            return fpt::launch_sdf_persistent(plan->dcfg, plan->host, acc, 0u, 1u, plan->counter, wg, int(eb), nullptr,
-                                              plan->proposer.enabled ? &plan->proposer.dev : nullptr, plan->tile_order,
-                                              s);
+    const uint32_t plan_proxy_spp = plan ? std::min<uint32_t>(clamp_samples(plan->host.samples), 4u) : 1u;
+    auto launch = [&](const Shape &sh) -> hipError_t {
+        if (plan)  // proxy: the frame's first samples (up to 4)
+            return fpt::launch_sdf_persistent(plan->dcfg, plan->host, acc, 0u, plan_proxy_spp, plan->counter,
+                                              sh.workgroups,
+                                              int(sh.event_batch), nullptr,
+                                              plan->proposer.enabled ? &plan->proposer.dev : nullptr,
+                                              sh.order ? plan->tile_order : nullptr, s);
         return fpt::launch_sdf_batch(batch->dcfgs, batch->dscenes, uint32_t(batch->host.size()), batch->total_slots, acc,
-                                     batch->counter, wg, int(eb), 1u, nullptr, batch->lowered, batch->tile_order, s);
+                                     batch->counter, sh.workgroups, int(sh.event_batch), 1u, nullptr, batch->lowered,
+                                     sh.order ? batch->tile_order : nullptr, s);
     };
-    HIP_TRY(launch(cand[0].first, cand[0].second), "autotune warm-up");
+    HIP_TRY(launch(cand[0]), "autotune warm-up");
     for (uint32_t r = 0; r < rounds; ++r) {
         for (size_t j = 0; j < cand.size(); ++j) {
             size_t c = (j + r * 7u) % cand.size();  // rotated probe order
             HIP_TRY(hipEventRecord(e0, s), "hipEventRecord");
-            HIP_TRY(launch(cand[c].first, cand[c].second), "autotune launch");
+            HIP_TRY(launch(cand[c]), "autotune launch");
             HIP_TRY(hipEventRecord(e1, s), "hipEventRecord");
             HIP_TRY(hipEventSynchronize(e1), "autotune");
             float ms = 0.f;
@@ -1362,7 +1378,7 @@ int fpt_hip_autotune(FptHipPlan *plan, FptHipBatch *batch, float *device_accum, 
         std::vector<float> t = times[c];
         std::sort(t.begin(), t.end());
         float med = t[t.size() / 2];
-        if (cand[c].first == 2u * cus && cand[c].second == 24u) default_ms = med;
+        if (cand[c].workgroups == 2u * cus && cand[c].event_batch == 24u && cand[c].order == 1u) default_ms = med;
         if (med < best_ms) {
             best_ms = med;
             best = c;
@@ -1370,15 +1386,19 @@ int fpt_hip_autotune(FptHipPlan *plan, FptHipBatch *batch, float *device_accum, 
     }
     if (plan) {
         plan->kernel = 1u;
-        plan->workgroups = cand[best].first;
-        plan->event_batch = cand[best].second;
+        plan->workgroups = cand[best].workgroups;
+        plan->event_batch = cand[best].event_batch;
+        plan->tile_order_mode = cand[best].order;
     } else {
-        batch->workgroups = cand[best].first;
-        batch->event_batch = cand[best].second;
+        batch->workgroups = cand[best].workgroups;
+        batch->event_batch = cand[best].event_batch;
+        batch->tile_order_mode = cand[best].order;
     }
     if (result) {
-        result->workgroups = cand[best].first;
-        result->event_batch = cand[best].second;
+        result->workgroups = cand[best].workgroups;
+        result->event_batch = cand[best].event_batch;
+        result->tile_order = cand[best].order;
+        result->_pad0 = 0u;
         result->candidates = uint32_t(cand.size());
         result->rounds = rounds;
         result->best_ms = best_ms;
@@ -1390,6 +1410,16 @@ int fpt_hip_autotune(FptHipPlan *plan, FptHipBatch *batch, float *device_accum, 
 }  // extern "C"
 
 extern "C" {
+
+int fpt_hip_set_tile_order(FptHipPlan *plan, FptHipBatch *batch, uint32_t mode, char *error, size_t error_len) {
+    if ((!plan) == (!batch) || mode > 1u) {
+        set_error(error, error_len, "tile order needs exactly one of plan/batch and mode 0 or 1");
+        return 1;
+    }
+    if (plan) plan->tile_order_mode = mode;
+    else batch->tile_order_mode = mode;
+    return 0;
+}
 
 int fpt_metal_device_name(char *name, size_t name_len) {
     if (!name || name_len == 0) return 1;

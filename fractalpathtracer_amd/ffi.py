@@ -249,7 +249,7 @@ class FptLaneStats(C.Structure):
 
 class FptTuneResult(C.Structure):
     _fields_ = [("workgroups", u32), ("event_batch", u32), ("candidates", u32), ("rounds", u32),
-                ("best_ms", f32), ("default_ms", f32)]
+                ("best_ms", f32), ("default_ms", f32), ("tile_order", u32), ("_pad0", u32)]
 
 
 class FptProposerSettings(C.Structure):
@@ -282,7 +282,7 @@ EXPECTED_SIZES = {
     FptIndexedPrimitive: 32, FptTypedSoAProgram: 5136, FptStitchPipelineStats: 72,
     FptRenderConfig: 30448, FptSdfProfileStats: 288, FptDiagnosticConfig: 24,
     FptMandelbulberFieldSample: 16, FptAsyncJitStats: 40, FptRayHit: 40, FptVoxelBuildInfo: 48,
-    FptTuneResult: 24, FptProposerSettings: 16, FptProposerStats: 32, FptSpecializationStats: 80,
+    FptTuneResult: 32, FptProposerSettings: 16, FptProposerStats: 32, FptSpecializationStats: 80,
     FptSceneOptions: 56, FptVoxelTraceStats: 32, FptLaneStats: 96,
 }
 
@@ -299,7 +299,7 @@ EXPORTED_SYMBOLS = (
     "fpt_hip_voxel_trace_cells", "fpt_hip_plan_tune", "fpt_hip_batch_create", "fpt_hip_batch_pixels",
     "fpt_hip_batch_tune", "fpt_hip_batch_render", "fpt_hip_batch_destroy", "fpt_hip_render_batch",
     "fpt_hip_render_batch_noalloc",
-    "fpt_hip_autotune", "fpt_hip_device_alloc", "fpt_hip_device_free", "fpt_hip_copy", "fpt_hip_fill_f32",
+    "fpt_hip_autotune", "fpt_hip_set_tile_order", "fpt_hip_device_alloc", "fpt_hip_device_free", "fpt_hip_copy", "fpt_hip_fill_f32",
     "fpt_hip_stream_create", "fpt_hip_stream_destroy", "fpt_hip_synchronize", "fpt_hip_time_renders",
     "fpt_hip_proposer_samples", "fpt_hip_proposer_eval", "fpt_hip_plan_set_proposer", "fpt_hip_plan_profile",
     "fpt_hip_plan_voxel_stats", "fpt_hip_plan_lane_stats",
@@ -387,6 +387,8 @@ def _declare(lib):
     lib.fpt_hip_render_batch_noalloc.argtypes = [P(FptRenderConfig), u32, C.c_void_p, C.c_size_t, C.c_void_p, cp,
                                                  C.c_size_t]
     lib.fpt_hip_render_batch_noalloc.restype = C.c_int
+    lib.fpt_hip_set_tile_order.argtypes = [C.c_void_p, C.c_void_p, u32, cp, C.c_size_t]
+    lib.fpt_hip_set_tile_order.restype = C.c_int
     lib.fpt_hip_autotune.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, u32, P(FptTuneResult), cp,
                                      C.c_size_t]
     lib.fpt_hip_autotune.restype = C.c_int

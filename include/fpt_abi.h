@@ -885,9 +885,9 @@ int fpt_hip_render_batch(const struct FptRenderConfig *configs,
                          size_t error_len);
 
 /*
- * Autotuner for exactly one of `plan` / `batch`: 20 launch shapes
- * (workgroups 1-4 per CU x event_batch 8..40) timed on a proxy of the
- * workload (the plan's first sample; the batch capped at one sample per
+ * Autotuner for exactly one of `plan` / `batch`: 40 launch shapes
+ * (workgroups 1-4 per CU x event_batch 8..40 x both tile orders) timed on a proxy of the
+ * workload (the plan's first samples, up to 4; the batch capped at one sample per
  * pixel) written into device_accum (contents are overwritten), `rounds`
  * rounds in a rotated probe order, median per shape; the fastest shape is
  * stored in the plan/batch and reported.  Synchronous.
@@ -898,9 +898,11 @@ struct FptTuneResult {
     uint32_t candidates;
     uint32_t rounds;
     float best_ms;
-    float default_ms;   /* median of the default shape (2 per CU, 24) */
+    float default_ms;   /* median of the default shape (2 per CU, 24, longest-first tiles) */
+    uint32_t tile_order; /* 1 longest-first, 0 row-major (fpt_hip_set_tile_order) */
+    uint32_t _pad0;
 };
-FPT_ABI_ASSERT(sizeof(struct FptTuneResult) == 24, "FptTuneResult");
+FPT_ABI_ASSERT(sizeof(struct FptTuneResult) == 32, "FptTuneResult");
 
 int fpt_hip_autotune(FptHipPlan *plan,
                      FptHipBatch *batch,
@@ -910,6 +912,16 @@ int fpt_hip_autotune(FptHipPlan *plan,
                      struct FptTuneResult *result,
                      char *error,
                      size_t error_len);
+
+/*
+ * Order in which a plan's or batch's persistent launch hands out its 8x8
+ * tiles (exactly one of plan / batch): 1 (default) longest-first, from a
+ * probe of four primary rays per tile at creation (march steps + 256 per
+ * surface hit), so cheap tiles form the frame's tail and lanes finish
+ * together; 0 row-major, scene after scene, which keeps neighbouring tiles
+ * together.  Images do not depend on it.  The autotuner tries both.
+ */
+int fpt_hip_set_tile_order(FptHipPlan *plan, FptHipBatch *batch, uint32_t mode, char *error, size_t error_len);
 
 /*
  * Device plumbing for callers that do not hold a HIP runtime of their own

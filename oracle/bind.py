@@ -61,6 +61,9 @@ def lib():
         L.orc_hash13.restype = C.c_float
         L.orc_last_counters.argtypes = [P(C.c_uint64)]
         L.orc_random_vector.argtypes = [C.c_int, dp, P(C.c_float), C.c_size_t, dp]
+        L.orc_voxel_render_tables.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, P(C.c_uint32),
+                                               C.c_size_t, C.c_void_p, C.c_size_t, P(C.c_uint16), P(C.c_uint8),
+                                               C.c_size_t, P(C.c_float), C.c_uint32, dp]
         L.orc_scatter.argtypes = [C.c_int, dp, dp]
         _voxel_decls(L)
         _lib = L
@@ -129,6 +132,27 @@ def voxel_render(cfg, threads: int = 0, rows=None) -> np.ndarray:
     out = np.zeros((cfg.height, cfg.width, 4), np.float64)
     if lib().orc_voxel_render(C.addressof(cfg), threads, y0, y1, _dp(out)) != 0:
         raise RuntimeError("orc_voxel_render failed")
+    return out
+
+
+def voxel_render_tables(cfg, tables: dict, threads: int = 0, rows=None) -> np.ndarray:
+    """renderVoxelPath over tables built elsewhere (a dict like
+    api.voxel_build's: page_table, records, distance, material, palette,
+    active_bricks)."""
+    y0, y1 = (0, cfg.height) if rows is None else rows
+    out = np.zeros((cfg.height, cfg.width, 4), np.float64)
+    pt = np.ascontiguousarray(tables["page_table"], np.uint32)
+    rec = np.ascontiguousarray(tables["records"])
+    dist = np.ascontiguousarray(tables["distance"], np.uint16)
+    mat = np.ascontiguousarray(tables["material"], np.uint8)
+    pal = np.ascontiguousarray(tables["palette"], np.float32).reshape(-1)
+    P = C.POINTER
+    rc = lib().orc_voxel_render_tables(C.addressof(cfg), threads, y0, y1, pt.ctypes.data_as(P(C.c_uint32)), len(pt),
+                                       rec.ctypes.data_as(C.c_void_p), len(rec), dist.ctypes.data_as(P(C.c_uint16)),
+                                       mat.ctypes.data_as(P(C.c_uint8)), len(mat), pal.ctypes.data_as(P(C.c_float)),
+                                       int(tables["active_bricks"]), _dp(out))
+    if rc != 0:
+        raise RuntimeError("orc_voxel_render_tables failed")
     return out
 
 

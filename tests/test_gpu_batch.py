@@ -63,7 +63,8 @@ def test_autotune_picks_a_valid_shape_and_keeps_the_image():
         st.synchronize()
         before = buf.numpy()
         r = b.autotune(buf.address, st.address, rounds=1)
-        assert r.candidates == 20 and r.best_ms > 0 and 8 <= r.event_batch <= 40 and r.workgroups > 0
+        assert r.candidates == 40 and r.best_ms > 0 and 8 <= r.event_batch <= 40 and r.workgroups > 0
+        assert r.tile_order in (0, 1)
         b.render_into(buf.address, st.address)
         st.synchronize()
         assert np.array_equal(buf.numpy(), before)
@@ -97,3 +98,28 @@ def test_batch_workload_at_its_configuration():
         rmse_twin = float(np.sqrt(np.mean((twin - ref) ** 2)))
         print(f"scene {i} ({c.width}x{c.height}, {c.samples} spp): rmse gpu {rmse:.3e} twin {rmse_twin:.3e}")
         assert rmse <= 1.25 * rmse_twin + 2e-3
+
+
+def test_tile_order_does_not_change_images():
+    """Longest-first and row-major hand-out give the same images, for a plan
+    and for a batch (every pixel is rendered by one lane alone)."""
+    cfg = scenes.mandelbulb_scene(96, 64, spp=3, bounces=2)
+    imgs = []
+    for mode in (1, 0):
+        with api.Plan(cfg) as plan:
+            plan.set_tile_order(mode)
+            buf = api.DeviceBuffer(cfg.width * cfg.height * 4)
+            plan.render_into(buf.address)
+            imgs.append(buf.numpy())
+    assert np.array_equal(imgs[0], imgs[1])
+    cfgs = _small_batch(n=5, seed=8)
+    st = api.Stream()
+    outs = []
+    with api.Batch(cfgs) as b:
+        for mode in (1, 0, 1):
+            b.set_tile_order(mode)
+            buf = api.DeviceBuffer(b.pixels * 4)
+            b.render_into(buf.address, st.address)
+            st.synchronize()
+            outs.append(buf.numpy())
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])

@@ -265,3 +265,15 @@ def test_direct_build_without_rejection_or_programs_equals_staging():
     mb.voxel_build_mode, mb.voxel_brick_rejection = 1, 1
     b = bind.voxel_build(mb)
     assert b["rejected_bricks"] == 0 and np.array_equal(a["page_table"], b["page_table"])
+
+
+@pytest.mark.parametrize("storage", [ffi.VOXEL_STORAGE_DENSE, ffi.VOXEL_STORAGE_SPARSE_BRICKS,
+                                     ffi.VOXEL_STORAGE_SPARSE_BRICKS8])
+def test_render_over_given_tables_equals_render_over_own_build(storage):
+    """orc_voxel_render_tables (the CPU baseline over tables read back from
+    the GPU) renders exactly what the oracle renders from its own build."""
+    cfg = _small(storage, res=40)
+    cfg.width, cfg.height, cfg.samples = 40, 24, 2
+    own = bind.voxel_render(cfg)
+    given = bind.voxel_render_tables(cfg, bind.voxel_build(cfg))
+    assert np.array_equal(own, given)
