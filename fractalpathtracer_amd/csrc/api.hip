@@ -440,7 +440,10 @@ struct FptHipPlan {
     // SDF launch shape (fpt_hip_plan_tune).
     uint32_t kernel = 1;         // 0 one path per lane, 1 persistent
     uint32_t workgroups = 0;     // persistent grid; 0 = 2 per compute unit
-    uint32_t event_batch = 24;   // waiting lanes before per-hit work runs
+    uint32_t event_batch = 0;    // waiting paths before per-hit work runs (0: the instance's default)
+    // Path-stash refill: non-marching lanes of a wave before they exchange
+    // their paths for parked marching ones (fpt_hip_plan_tune_stash).
+    uint32_t refill = 4;
     uint32_t *counter = nullptr; // persistent pixel counter
     // Hand-out order of the persistent kernel's 8x8 tiles: the most work
     // first (longest-processing-time order from a probe of primary rays), so
@@ -475,11 +478,12 @@ hipError_t launch_sdf(FptHipPlan *plan, float4 *acc, uint32_t start, uint32_t ch
         hipError_t e = fpt::launch_zero_counter(plan->counter, s);
         if (e != hipSuccess) return e;
         return fpt::launch_specialized(*plan->spec, plan->dcfg, acc, start, chunk, plan->counter,
-                                       fpt::persistent_slots(c.width, c.height), wg, int(plan->event_batch),
-                                       plan->tile_order_mode ? plan->tile_order : nullptr, s);
+                                       fpt::persistent_slots(c.width, c.height), wg,
+                                       plan->event_batch ? int(plan->event_batch) : fpt::default_event_batch(true),
+                                       int(plan->refill), plan->tile_order_mode ? plan->tile_order : nullptr, s);
     }
     return fpt::launch_sdf_persistent(plan->dcfg, c, acc, start, chunk, plan->counter, wg, int(plan->event_batch),
-                                      prof, plan->proposer.enabled ? &plan->proposer.dev : nullptr,
+                                      int(plan->refill), prof, plan->proposer.enabled ? &plan->proposer.dev : nullptr,
                                       plan->tile_order_mode ? plan->tile_order : nullptr, s);
 }
 
@@ -670,7 +674,20 @@ int fpt_hip_plan_tune(FptHipPlan *plan, uint32_t kernel, uint32_t workgroups, ui
     }
     plan->kernel = kernel;
     plan->workgroups = workgroups;
-    plan->event_batch = event_batch ? event_batch : 24u;
+    plan->event_batch = event_batch;
+    return 0;
+}
+
+int fpt_hip_plan_tune_stash(FptHipPlan *plan, uint32_t refill, char *error, size_t error_len) {
+    if (!plan) {
+        set_error(error, error_len, "missing plan");
+        return 1;
+    }
+    if (refill < 1u || refill > 64u) {
+        set_error(error, error_len, "invalid stash refill threshold %u (1..64 lanes)", refill);
+        return 1;
+    }
+    plan->refill = refill;
     return 0;
 }
 
@@ -1335,9 +1352,17 @@ int fpt_hip_autotune(FptHipPlan *plan, FptHipBatch *batch, float *device_accum, 
         uint32_t workgroups, event_batch, order;
     };
     std::vector<Shape> cand;
+    // With the path stash a workgroup holds 53-57 KB of LDS, so at most two
+    // are resident per compute unit, and the event batch counts the wave's
+    // up to 128 paths.
+    const bool stash = plan && fpt::persistent_uses_stash(plan->host, plan->proposer.enabled);
+    const std::vector<uint32_t> per_cus = stash ? std::vector<uint32_t>{1u, 2u} : std::vector<uint32_t>{1u, 2u, 3u, 4u};
+    const std::vector<uint32_t> ebs =
+        stash ? std::vector<uint32_t>{32u, 40u, 48u, 56u, 64u} : std::vector<uint32_t>{8u, 16u, 24u, 32u, 40u};
+    const uint32_t default_eb = uint32_t(fpt::default_event_batch(stash));
     for (uint32_t order : {1u, 0u})
-        for (uint32_t per_cu : {1u, 2u, 3u, 4u})
-            for (uint32_t eb : {8u, 16u, 24u, 32u, 40u}) cand.push_back({per_cu * cus, eb, order});
+        for (uint32_t per_cu : per_cus)
+            for (uint32_t eb : ebs) cand.push_back({per_cu * cus, eb, order});
     rounds = std::max<uint32_t>(1u, std::min<uint32_t>(rounds, 16u));
     std::vector<std::vector<float>> times(cand.size());
     hipEvent_t e0, e1;
@@ -1349,8 +1374,7 @@ int fpt_hip_autotune(FptHipPlan *plan, FptHipBatch *batch, float *device_accum, 
     auto launch = [&](const Shape &sh) -> hipError_t {
         if (plan)  // proxy: the frame's first samples (up to 4)
             return fpt::launch_sdf_persistent(plan->dcfg, plan->host, acc, 0u, plan_proxy_spp, plan->counter,
-                                              sh.workgroups,
-                                              int(sh.event_batch), nullptr,
+                                              sh.workgroups, int(sh.event_batch), int(plan->refill), nullptr,
                                               plan->proposer.enabled ? &plan->proposer.dev : nullptr,
                                               sh.order ? plan->tile_order : nullptr, s);
         return fpt::launch_sdf_batch(batch->dcfgs, batch->dscenes, uint32_t(batch->host.size()), batch->total_slots, acc,
@@ -1378,7 +1402,7 @@ int fpt_hip_autotune(FptHipPlan *plan, FptHipBatch *batch, float *device_accum, 
         std::vector<float> t = times[c];
         std::sort(t.begin(), t.end());
         float med = t[t.size() / 2];
-        if (cand[c].workgroups == 2u * cus && cand[c].event_batch == 24u && cand[c].order == 1u) default_ms = med;
+        if (cand[c].workgroups == 2u * cus && cand[c].event_batch == default_eb && cand[c].order == 1u) default_ms = med;
         if (med < best_ms) {
             best_ms = med;
             best = c;

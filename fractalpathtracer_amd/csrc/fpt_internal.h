@@ -53,6 +53,12 @@ uint32_t persistent_slots(uint32_t width, uint32_t height);
 // The scene's program is one Mandelbulb leaf: the persistent kernel's
 // interleaved-orbit variant applies.
 bool single_bulb_program(const FptRenderConfig &c);
+// The single-scene persistent instance that renders c (learned: with a
+// step proposer) keeps parked paths in LDS (the path stash).
+bool persistent_uses_stash(const FptRenderConfig &c, bool learned);
+// Event batch of a persistent launch given 0: 64 paths with the path stash,
+// 24 lanes without.
+int default_event_batch(bool stash);
 // The scene's program carries flat-union, canonical or typed-SoA tables: the
 // kernels' instances with the table evaluators apply.
 bool has_lowered_tables(const FptRenderConfig &c);
@@ -91,10 +97,11 @@ hipError_t launch_surface_sample(const FptRenderConfig *dcfg, const float4 *pts,
 hipError_t launch_typed_soa_validation(const FptRenderConfig *dcfg, uint32_t n, uint32_t *counts,
                                        hipStream_t stream);
 // tile_order (optional, device): the order in which the 8x8 tiles are handed
-// out (a permutation of the row-major tile indices).
+// out (a permutation of the row-major tile indices).  event_batch 0 takes
+// default_event_batch of the instance launched.
 hipError_t launch_sdf_persistent(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, float4 *accum,
                                  uint32_t start, uint32_t count, uint32_t *counter, uint32_t workgroups,
-                                 int event_batch, DeviceProfile *prof, const ProposerDevice *prop,
+                                 int event_batch, int refill, DeviceProfile *prof, const ProposerDevice *prop,
                                  const uint32_t *tile_order, hipStream_t stream);
 hipError_t launch_sdf_diagnostic(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg,
                                  const FptDiagnosticConfig &dg, uchar4 *out, hipStream_t stream);

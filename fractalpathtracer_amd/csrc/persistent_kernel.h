@@ -120,6 +120,57 @@ __host__ __device__ __forceinline__ uint32_t slot_count(uint32_t w, uint32_t h) 
     return ((w + 7u) / 8u) * ((h + 7u) / 8u) * 64u;
 }
 
+// Position of the r-th (0-based) set bit of m; requires r < popcount(m).
+__device__ __forceinline__ uint32_t select_bit(unsigned long long m, uint32_t r) {
+    uint32_t pos = 0u, word = uint32_t(m);
+    uint32_t c = uint32_t(__popc(word));
+    if (r >= c) {
+        r -= c;
+        word = uint32_t(m >> 32);
+        pos = 32u;
+    }
+#pragma unroll
+    for (uint32_t half = 16u; half >= 1u; half >>= 1) {
+        c = uint32_t(__popc(word & ((1u << half) - 1u)));
+        if (r >= c) {
+            r -= c;
+            word >>= half;
+            pos += half;
+        }
+    }
+    return pos;
+}
+
+__device__ __forceinline__ uint32_t as_word(float v) { return __float_as_uint(v); }
+__device__ __forceinline__ uint32_t as_word(int v) { return uint32_t(v); }
+__device__ __forceinline__ uint32_t as_word(uint32_t v) { return v; }
+__device__ __forceinline__ void from_word(float &v, uint32_t w) { v = __uint_as_float(w); }
+__device__ __forceinline__ void from_word(int &v, uint32_t w) { v = int(w); }
+__device__ __forceinline__ void from_word(uint32_t &v, uint32_t w) { v = w; }
+
+// Stores four 32-bit lane values into one 16-byte LDS word and, when `read`,
+// takes the word's previous contents in exchange.
+template <class A, class B, class C, class D>
+__device__ __forceinline__ void xchg4(uint4 *slot, bool read, A &a, B &b, C &c, D &d) {
+    uint4 t;
+    if (read) t = *slot;
+    *slot = make_uint4(as_word(a), as_word(b), as_word(c), as_word(d));
+    if (read) {
+        from_word(a, t.x);
+        from_word(b, t.y);
+        from_word(c, t.z);
+        from_word(d, t.w);
+    }
+}
+
+// Orders LDS accesses of different lanes of one wave: the compiler keeps
+// every access on its side of this point.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Minimum resident waves per SIMD the register allocation must allow (exact
 // and learned variants); the compiler caps VGPRs at 512 / waves.
 #ifndef FPT_PERSISTENT_WAVES
@@ -144,6 +195,15 @@ __host__ __device__ __forceinline__ uint32_t slot_count(uint32_t w, uint32_t h) 
 #ifndef FPT_MANDELBULBER_PERSISTENT_WAVES
 #define FPT_MANDELBULBER_PERSISTENT_WAVES 2
 #endif
+// Path stash of the single-scene exact instances (on/off).
+#ifndef FPT_PATH_STASH
+#define FPT_PATH_STASH 1
+#endif
+
+// Uses the LDS path stash: single-scene exact-march instances.
+__host__ __device__ constexpr bool uses_path_stash(bool multi, bool learned, bool mb) {
+    return FPT_PATH_STASH != 0 && !multi && !learned && !mb;
+}
 
 // MB = true (single scene): an imported Mandelbulber scene, with the
 // Mandelbulber camera, march (threshold, clamped step, FP32 fixed-point and
@@ -159,9 +219,24 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
                                                              const BatchScene *__restrict__ scenes, uint32_t nscenes,
                                                              float4 *__restrict__ accum, uint32_t start,
                                                              uint32_t count, uint32_t *__restrict__ counter,
-                                                             uint32_t total_slots, int event_batch,
+                                                             uint32_t total_slots, int event_batch, int refill,
                                                              DeviceProfile *prof, ProposerDevice prop,
                                                              const uint32_t *__restrict__ tile_order) {
+    // Path stash: each wave keeps up to 64 further paths parked in LDS, one
+    // slot per lane index (slot j's phase lives in lane j's `slot_phase`).
+    // Lanes whose path waits for the per-hit pass (or is idle) exchange it
+    // for a parked marching path once `refill` lanes are not marching, and
+    // the per-hit pass first pulls parked waiting paths into the lanes that
+    // are not waiting; it runs once `event_batch` of the wave's up to 128
+    // paths wait, or fewer than `event_batch` march.  A path's whole state
+    // moves with it, so every path executes the same operations as without
+    // the stash and images are bit-identical.
+    constexpr bool STASH = uses_path_stash(MULTI, LEARNED, MB);
+    constexpr int SG = BULB ? 14 : 13;  // 16-byte words per parked path
+    __shared__ uint4 stash_lds[STASH ? 4 * SG * 64 : 1];
+    uint4 *const st = stash_lds + (STASH ? (threadIdx.x >> 6) * SG * 64 : 0);
+    int slot_phase = PH_IDLE;
+
     const FptRenderConfig *cp = cfgs;  // the lane's scene (fixed when !MULTI)
     float4 *out = accum;               // the lane's scene accumulation buffer
 
@@ -338,8 +413,65 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
         start_path();
     };
 
+    // Moves the lane's path into stash slot j; with `read`, the path parked
+    // there comes back in exchange (32-bit fields in 16-byte words, slot-
+    // major so the 64 lanes of a wave touch 64 consecutive words).
+    auto move_path = [&](uint32_t j, bool read) __attribute__((always_inline)) {
+        uint4 *q = st + j;
+        uint32_t flags = uint32_t(shadow) | (uint32_t(borb) << 1);
+        xchg4(q + 0 * 64, read, phase, pix, s, bounce);
+        xchg4(q + 1 * 64, read, uv.x, uv.y, local_ni, it);
+        xchg4(q + 2 * 64, read, acc.x, acc.y, acc.z, side);
+        xchg4(q + 3 * 64, read, light.x, light.y, light.z, sky_mask);
+        xchg4(q + 4 * 64, read, color.x, color.y, color.z, max_iter);
+        xchg4(q + 5 * 64, read, gcol.x, gcol.y, gcol.z, flags);
+        xchg4(q + 6 * 64, read, dr.x, dr.y, dr.z, mat.roughness);
+        xchg4(q + 7 * 64, read, hit_p.x, hit_p.y, hit_p.z, mat.specular);
+        xchg4(q + 8 * 64, read, hit_n.x, hit_n.y, hit_n.z, mat.translucency);
+        xchg4(q + 9 * 64, read, mat.rgb.x, mat.rgb.y, mat.rgb.z, mat.ior);
+        xchg4(q + 10 * 64, read, p.x, p.y, p.z, mat.emission);
+        if (BULB) {
+            xchg4(q + 11 * 64, read, mdir.x, mdir.y, mdir.z, bdr);
+            xchg4(q + 12 * 64, read, mstart.x, mstart.y, mstart.z, boi);
+            xchg4(q + 13 * 64, read, bz.x, bz.y, bz.z, br);
+        } else {
+            uint32_t pad0 = 0u, pad1 = 0u;
+            xchg4(q + 11 * 64, read, mdir.x, mdir.y, mdir.z, pad0);
+            xchg4(q + 12 * 64, read, mstart.x, mstart.y, mstart.z, pad1);
+        }
+        shadow = (flags & 1u) != 0u;
+        borb = (flags & 2u) != 0u;
+    };
+    // The r-th lane (in lane order) with `want` exchanges its path with the
+    // path in the r-th slot of `offer`, for as many pairs as both sides have.
+    // Slots are counted from `rot`, one past the last slot taken, so parked
+    // paths are taken round-robin and none waits behind lower-numbered slots.
+    // Wave-uniform call.
+    uint32_t rot = 0u;
+    auto exchange_in = [&](bool want, unsigned long long offer) __attribute__((always_inline)) {
+        const unsigned long long wm = __ballot(want);
+        const uint32_t nw = uint32_t(__popcll(wm)), no = uint32_t(__popcll(offer));
+        const uint32_t n = nw < no ? nw : no;
+        if (n == 0u) return;
+        const uint32_t r = uint32_t(__popcll(wm & ((1ull << lane_id()) - 1ull)));
+        const unsigned long long off = rot ? (offer >> rot) | (offer << (64u - rot)) : offer;
+        wave_lds_sync();
+        if (want && r < n) move_path((select_bit(off, r) + rot) & 63u, true);
+        wave_lds_sync();
+        slot_phase = int(st[lane_id()].x);
+        rot = (rot + select_bit(off, n - 1u) + 1u) & 63u;
+    };
+
     take_slot(true);
     while (__any(retry)) take_slot(retry);
+    if (STASH) {
+        // Park the first paths and take a second pixel per lane.
+        move_path(lane_id(), false);
+        slot_phase = phase;
+        phase = PH_IDLE;
+        take_slot(true);
+        while (__any(retry)) take_slot(retry);
+    }
 
     while (true) {
         if (PROFILE) {
@@ -488,8 +620,31 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
         }
         const unsigned long long ev = __ballot(phase == PH_EVENT);
         const unsigned long long mar = __ballot(phase == PH_MARCH);
-        if (!ev && !mar) break;
-        if (!(__popcll(ev) >= event_batch || (ev && !mar))) continue;
+        if (STASH) {
+            const unsigned long long sev = __ballot(slot_phase == PH_EVENT);
+            const unsigned long long smar = __ballot(slot_phase == PH_MARCH);
+            if (!ev && !mar && !sev && !smar) break;
+            // The per-hit pass runs once `event_batch` paths wait, or once at
+            // least as many paths wait as march: when the wave's pixels run
+            // out and fewer paths are left, a threshold on waiting paths
+            // alone would hold every waiting path until the slowest march
+            // ends.
+            const int nev = __popcll(ev) + __popcll(sev);
+            const int nmar = __popcll(mar) + __popcll(smar);
+            if (nev >= event_batch || (nev > 0 && nev >= nmar)) {
+                // Per-hit pass: waiting paths from the stash into the lanes
+                // that are not waiting.
+                if (sev && ~ev) exchange_in(phase != PH_EVENT, sev);
+            } else {
+                // Keep marching: parked marching paths into the lanes that
+                // wait or idle, once enough of them do.
+                if (smar && 64 - __popcll(mar) >= refill) exchange_in(phase != PH_MARCH, smar);
+                continue;
+            }
+        } else {
+            if (!ev && !mar) break;
+            if (!(__popcll(ev) >= event_batch || (ev && !mar))) continue;
+        }
 
         bool need_slot = false;
         if (PROFILE) lane_stat(ls, LS_EVENT_PASSES, LS_EVENT_LANES, phase == PH_EVENT);
@@ -652,6 +807,13 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
         // Wave-uniform hand-out for lanes whose pixel is finished.
         take_slot(need_slot);
         while (__any(retry)) take_slot(retry);
+        if (STASH) {
+            // Lanes left idle (no pixel to take) take parked marching paths:
+            // once the wave's pixels run out, most trips end here, and the
+            // refill above would not run.
+            const unsigned long long smar = __ballot(slot_phase == PH_MARCH);
+            if (smar && 64 - __popcll(__ballot(phase == PH_MARCH)) >= refill) exchange_in(phase != PH_MARCH, smar);
+        }
     }
 
     if (PROFILE) {

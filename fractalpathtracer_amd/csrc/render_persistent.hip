@@ -20,6 +20,12 @@ bool single_bulb_program(const FptRenderConfig &c) {
            c.sdf_program[0].opcode == FPT_SDF_OP_MANDELBULB && !has_lowered_tables(c);
 }
 
+bool persistent_uses_stash(const FptRenderConfig &c, bool learned) {
+    return uses_path_stash(false, learned, c.sdf_id == FPT_SDF_MANDELBULBER);
+}
+
+int default_event_batch(bool stash) { return stash ? 64 : 24; }
+
 // The dispatch condition of SM:3576-3584, computed exactly as the device's
 // has_lowered(): each typed count capped at the table size before the sum,
 // so a huge count cannot wrap the sum to zero on one side only.
@@ -36,42 +42,43 @@ namespace {
 template <bool PROFILE, bool MULTI, bool LEARNED, bool BULB, bool MB, bool LOWERED>
 void launch_instance(dim3 grid, hipStream_t stream, const FptRenderConfig *cfgs, const BatchScene *scenes,
                      uint32_t nscenes, float4 *accum, uint32_t start, uint32_t count, uint32_t *counter,
-                     uint32_t slots, int event_batch, DeviceProfile *prof, const ProposerDevice &prop,
+                     uint32_t slots, int event_batch, int refill, DeviceProfile *prof, const ProposerDevice &prop,
                      const uint32_t *order) {
     hipLaunchKernelGGL((sdf_persistent_kernel<PROFILE, MULTI, LEARNED, BULB, MB, LOWERED>), grid, dim3(256), 0,
-                       stream, cfgs, scenes, nscenes, accum, start, count, counter, slots, event_batch,
+                       stream, cfgs, scenes, nscenes, accum, start, count, counter, slots, event_batch, refill,
                        PROFILE ? prof : nullptr, prop, order);
 }
 
 template <bool MULTI, bool LEARNED, bool BULB, bool MB, bool LOWERED>
 void launch_profiled(bool profiled, dim3 grid, hipStream_t stream, const FptRenderConfig *cfgs,
                      const BatchScene *scenes, uint32_t nscenes, float4 *accum, uint32_t start, uint32_t count,
-                     uint32_t *counter, uint32_t slots, int event_batch, DeviceProfile *prof,
+                     uint32_t *counter, uint32_t slots, int event_batch, int refill, DeviceProfile *prof,
                      const ProposerDevice &prop, const uint32_t *order) {
     if (profiled)
         launch_instance<true, MULTI, LEARNED, BULB, MB, LOWERED>(grid, stream, cfgs, scenes, nscenes, accum, start,
-                                                                  count, counter, slots, event_batch, prof, prop,
-                                                                  order);
+                                                                  count, counter, slots, event_batch, refill, prof,
+                                                                  prop, order);
     else
         launch_instance<false, MULTI, LEARNED, BULB, MB, LOWERED>(grid, stream, cfgs, scenes, nscenes, accum, start,
-                                                                   count, counter, slots, event_batch, prof, prop,
-                                                                   order);
+                                                                   count, counter, slots, event_batch, refill, prof,
+                                                                   prop, order);
 }
 
 }  // namespace
 
 hipError_t launch_sdf_persistent(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, float4 *accum,
                                  uint32_t start, uint32_t count, uint32_t *counter, uint32_t workgroups,
-                                 int event_batch, DeviceProfile *prof, const ProposerDevice *prop,
+                                 int event_batch, int refill, DeviceProfile *prof, const ProposerDevice *prop,
                                  const uint32_t *tile_order, hipStream_t stream) {
     const uint32_t slots = slot_count(hcfg.width, hcfg.height);
+    if (event_batch <= 0) event_batch = default_event_batch(persistent_uses_stash(hcfg, prop != nullptr));
     hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, counter);
     const ProposerDevice none{};
     const dim3 grid(workgroups);
     const bool p = prof != nullptr, lowered = has_lowered_tables(hcfg);
 #define FPT_LAUNCH(LEARNED, BULB, MB, LOWERED, PROP)                                                              \
     launch_profiled<false, LEARNED, BULB, MB, LOWERED>(p, grid, stream, dcfg, nullptr, 1u, accum, start, count,  \
-                                                       counter, slots, event_batch, prof, PROP, tile_order)
+                                                       counter, slots, event_batch, refill, prof, PROP, tile_order)
     if (prop && lowered) FPT_LAUNCH(true, false, false, true, *prop);
     else if (prop) FPT_LAUNCH(true, false, false, false, *prop);
     else if (hcfg.sdf_id == FPT_SDF_MANDELBULBER) FPT_LAUNCH(false, false, true, false, none);
@@ -93,11 +100,11 @@ hipError_t launch_sdf_batch(const FptRenderConfig *dcfgs, const BatchScene *dsce
     const bool p = prof != nullptr;
     if (lowered)
         launch_profiled<true, false, false, false, true>(p, dim3(workgroups), stream, dcfgs, dscenes, nscenes, accum,
-                                                         0u, spp_cap, counter, total_slots, event_batch, prof, none,
+                                                         0u, spp_cap, counter, total_slots, event_batch, 64, prof, none,
                                                          tile_order);
     else
         launch_profiled<true, false, false, false, false>(p, dim3(workgroups), stream, dcfgs, dscenes, nscenes, accum,
-                                                          0u, spp_cap, counter, total_slots, event_batch, prof, none,
+                                                          0u, spp_cap, counter, total_slots, event_batch, 64, prof, none,
                                                           tile_order);
     return hipGetLastError();
 }

@@ -296,7 +296,7 @@ EXPORTED_SYMBOLS = (
     "fpt_test_async_stitch_context", "fpt_test_typed_soa", "fpt_metal_preview",
     "fpt_hip_trace_primary", "fpt_hip_plan_create", "fpt_hip_plan_render",
     "fpt_hip_plan_destroy", "fpt_hip_last_error", "fpt_hip_program_surface", "fpt_hip_voxel_build",
-    "fpt_hip_voxel_trace_cells", "fpt_hip_plan_tune", "fpt_hip_batch_create", "fpt_hip_batch_pixels",
+    "fpt_hip_voxel_trace_cells", "fpt_hip_plan_tune", "fpt_hip_plan_tune_stash", "fpt_hip_batch_create", "fpt_hip_batch_pixels",
     "fpt_hip_batch_tune", "fpt_hip_batch_render", "fpt_hip_batch_destroy", "fpt_hip_render_batch",
     "fpt_hip_render_batch_noalloc",
     "fpt_hip_autotune", "fpt_hip_set_tile_order", "fpt_hip_device_alloc", "fpt_hip_device_free", "fpt_hip_copy", "fpt_hip_fill_f32",
@@ -371,6 +371,8 @@ def _declare(lib):
     lib.fpt_hip_program_surface.restype = C.c_int
     lib.fpt_hip_plan_tune.argtypes = [C.c_void_p, u32, u32, u32, cp, C.c_size_t]
     lib.fpt_hip_plan_tune.restype = C.c_int
+    lib.fpt_hip_plan_tune_stash.argtypes = [C.c_void_p, u32, cp, C.c_size_t]
+    lib.fpt_hip_plan_tune_stash.restype = C.c_int
     lib.fpt_hip_batch_create.argtypes = [P(FptRenderConfig), u32, P(C.c_void_p), cp, C.c_size_t]
     lib.fpt_hip_batch_create.restype = C.c_int
     lib.fpt_hip_batch_pixels.argtypes = [C.c_void_p]

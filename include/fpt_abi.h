@@ -772,9 +772,13 @@ void fpt_hip_plan_destroy(FptHipPlan *plan);
  *                1 = persistent, path-regenerating kernel (default);
  *   workgroups   persistent grid size in 256-thread workgroups, 0 = two per
  *                compute unit;
- *   event_batch  how many lanes of a 64-lane wave must have finished their
+ *   event_batch  how many paths of a 64-lane wave must have finished their
  *                march before the per-hit work (material, analytic normal,
- *                sun and scatter) runs for all of them, 1..64 (0 = 24).
+ *                sun and scatter) runs for all of them, 1..64.  The
+ *                single-scene exact-march instances count the wave's lanes
+ *                and its paths parked in LDS (see below), and also run the
+ *                per-hit work once as many paths wait as march.  0 = the
+ *                instance's default: 64 with parked paths, 24 without.
  */
 int fpt_hip_plan_tune(FptHipPlan *plan,
                       uint32_t kernel,
@@ -782,6 +786,16 @@ int fpt_hip_plan_tune(FptHipPlan *plan,
                       uint32_t event_batch,
                       char *error,
                       size_t error_len);
+
+/*
+ * Path stash of the persistent kernel's single-scene exact-march instances
+ * (no reference counterpart: a launch-shape knob of this library).  Each
+ * wave parks up to 64 further paths in LDS; lanes whose path waits for the
+ * per-hit work, or that are idle, exchange it for a parked marching path
+ * once `refill` lanes of the wave are not marching (1..64, default 4).
+ * Images do not depend on it.
+ */
+int fpt_hip_plan_tune_stash(FptHipPlan *plan, uint32_t refill, char *error, size_t error_len);
 
 /*
  * Voxel tables of `config` (renderer_backend ignored; voxel_* fields used),
