@@ -242,6 +242,9 @@ def main():
         plan.set_proposer(params, box_half=meta.get("box_half", proposer.BOX_HALF),
                           switch_distance=proposer.SWITCH_DISTANCE, resume_distance=proposer.RESUME_DISTANCE,
                           step_scale=proposer.STEP_SCALE)
+        # The exact march's event batch counts lanes and parked paths; the
+        # learned instance keeps no parked paths and takes its own default.
+        plan.tune(1, tune.workgroups, 0)
         ms, img, prof, learned = run_method(api, plan, cfg, stream, accum, args.steps, args.warmup)
         plan.set_proposer(None)
         methods["learned"] = {"ms_per_step": ms, "value": samples_per_step / (ms / 1e3), "img": img,

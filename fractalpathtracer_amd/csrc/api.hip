@@ -443,7 +443,7 @@ struct FptHipPlan {
     uint32_t event_batch = 0;    // waiting paths before per-hit work runs (0: the instance's default)
     // Path-stash refill: non-marching lanes of a wave before they exchange
     // their paths for parked marching ones (fpt_hip_plan_tune_stash).
-    uint32_t refill = 4;
+    uint32_t refill = fpt::kDefaultRefill;
     uint32_t *counter = nullptr; // persistent pixel counter
     // Hand-out order of the persistent kernel's 8x8 tiles: the most work
     // first (longest-processing-time order from a probe of primary rays), so
@@ -989,7 +989,7 @@ struct FptHipBatch {
     size_t pixels = 0;
     bool lowered = false;  // some scene carries lowered program tables
     uint32_t workgroups = 0;
-    uint32_t event_batch = 24;
+    uint32_t event_batch = 0;  // 0: the instance's default
     uint32_t use_graph = 1;
     // Captured launch sequence (counter reset + persistent kernel) for one
     // output buffer.
@@ -1118,7 +1118,7 @@ int fpt_hip_batch_tune(FptHipBatch *batch, uint32_t workgroups, uint32_t event_b
         return 1;
     }
     batch->workgroups = workgroups;
-    batch->event_batch = event_batch ? event_batch : 24u;
+    batch->event_batch = event_batch;
     batch->use_graph = use_graph;
     return 0;
 }
@@ -1324,7 +1324,7 @@ int fpt_hip_render_batch_noalloc(const FptRenderConfig *configs, uint32_t count,
             "scene table upload");
     if (need_focus) HIP_TRY(fpt::launch_focus_prepass(dcfgs, count, s), "focus-distance prepass");
     HIP_TRY(fpt::launch_sdf_batch(dcfgs, dscenes, count, uint32_t(slot), reinterpret_cast<float4 *>(device_accum),
-                                  counter, default_workgroups(), 24, 0u, nullptr, lowered, nullptr, s),
+                                  counter, default_workgroups(), 0, 0u, nullptr, lowered, nullptr, s),
             "batch kernel launch");
     return 0;
 }
@@ -1355,7 +1355,7 @@ int fpt_hip_autotune(FptHipPlan *plan, FptHipBatch *batch, float *device_accum, 
     // With the path stash a workgroup holds 53-57 KB of LDS, so at most two
     // are resident per compute unit, and the event batch counts the wave's
     // up to 128 paths.
-    const bool stash = plan && fpt::persistent_uses_stash(plan->host, plan->proposer.enabled);
+    const bool stash = plan ? fpt::persistent_uses_stash(plan->host, plan->proposer.enabled) : fpt::batch_uses_stash();
     const std::vector<uint32_t> per_cus = stash ? std::vector<uint32_t>{1u, 2u} : std::vector<uint32_t>{1u, 2u, 3u, 4u};
     const std::vector<uint32_t> ebs =
         stash ? std::vector<uint32_t>{32u, 40u, 48u, 56u, 64u} : std::vector<uint32_t>{8u, 16u, 24u, 32u, 40u};

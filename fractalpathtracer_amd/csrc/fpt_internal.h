@@ -56,14 +56,19 @@ bool single_bulb_program(const FptRenderConfig &c);
 // The single-scene persistent instance that renders c (learned: with a
 // step proposer) keeps parked paths in LDS (the path stash).
 bool persistent_uses_stash(const FptRenderConfig &c, bool learned);
+// The batched (multi-scene) persistent instance keeps parked paths in LDS.
+bool batch_uses_stash();
 // Event batch of a persistent launch given 0: 64 paths with the path stash,
 // 24 lanes without.
 int default_event_batch(bool stash);
+// Non-marching lanes of a wave before they take parked marching paths.
+constexpr int kDefaultRefill = 4;
 // The scene's program carries flat-union, canonical or typed-SoA tables: the
 // kernels' instances with the table evaluators apply.
 bool has_lowered_tables(const FptRenderConfig &c);
 // tile_order (optional, device): hand-out order of the batch's global 8x8
-// tiles (scene after scene, row-major within a scene).
+// tiles (scene after scene, row-major within a scene).  event_batch 0 takes
+// the instance's default.
 hipError_t launch_sdf_batch(const FptRenderConfig *dcfgs, const BatchScene *dscenes, uint32_t nscenes,
                             uint32_t total_slots, float4 *accum, uint32_t *counter, uint32_t workgroups,
                             int event_batch, uint32_t spp_cap, DeviceProfile *prof, bool lowered,

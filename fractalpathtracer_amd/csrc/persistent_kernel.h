@@ -200,9 +200,9 @@ __device__ __forceinline__ void wave_lds_sync() {
 #define FPT_PATH_STASH 1
 #endif
 
-// Uses the LDS path stash: single-scene exact-march instances.
+// Uses the LDS path stash: exact-march instances (single scene or batch).
 __host__ __device__ constexpr bool uses_path_stash(bool multi, bool learned, bool mb) {
-    return FPT_PATH_STASH != 0 && !multi && !learned && !mb;
+    return FPT_PATH_STASH != 0 && !learned && !mb;
 }
 
 // MB = true (single scene): an imported Mandelbulber scene, with the
@@ -248,6 +248,7 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
 
     // Per-scene constants, refreshed when a lane takes a pixel.
     uint32_t W = 0, H = 0, s_end = 0, s_begin = 0;
+    uint32_t scene = 0;  // MULTI: index of the lane's scene
     f3 cam = mk(0.f, 0.f, 0.f);
     float focal = 1.f, aa = 0.f, maxd = 0.f, maxd2 = 0.f, far2 = 0.f, min_dist = 0.f, adapt = 0.f;
     bool check_t = false, sun_on = false;
@@ -388,6 +389,7 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
                 else hi = mid - 1u;
             }
             const BatchScene sc = scenes[lo];
+            scene = lo;
             cp = cfgs + lo;
             out = accum + sc.accum_offset;
             W = sc.width;
@@ -435,8 +437,8 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
             xchg4(q + 12 * 64, read, mstart.x, mstart.y, mstart.z, boi);
             xchg4(q + 13 * 64, read, bz.x, bz.y, bz.z, br);
         } else {
-            uint32_t pad0 = 0u, pad1 = 0u;
-            xchg4(q + 11 * 64, read, mdir.x, mdir.y, mdir.z, pad0);
+            uint32_t pad1 = 0u;
+            xchg4(q + 11 * 64, read, mdir.x, mdir.y, mdir.z, scene);
             xchg4(q + 12 * 64, read, mstart.x, mstart.y, mstart.z, pad1);
         }
         shadow = (flags & 1u) != 0u;
@@ -456,7 +458,17 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
         const uint32_t r = uint32_t(__popcll(wm & ((1ull << lane_id()) - 1ull)));
         const unsigned long long off = rot ? (offer >> rot) | (offer << (64u - rot)) : offer;
         wave_lds_sync();
-        if (want && r < n) move_path((select_bit(off, r) + rot) & 63u, true);
+        if (want && r < n) {
+            move_path((select_bit(off, r) + rot) & 63u, true);
+            if (MULTI) {  // the path's scene: its config, image and constants
+                const BatchScene sc = scenes[scene];
+                cp = cfgs + scene;
+                out = accum + sc.accum_offset;
+                W = sc.width;
+                H = sc.height;
+                load_scene();
+            }
+        }
         wave_lds_sync();
         slot_phase = int(st[lane_id()].x);
         rot = (rot + select_bit(off, n - 1u) + 1u) & 63u;

@@ -26,6 +26,8 @@ bool persistent_uses_stash(const FptRenderConfig &c, bool learned) {
 
 int default_event_batch(bool stash) { return stash ? 64 : 24; }
 
+bool batch_uses_stash() { return uses_path_stash(true, false, false); }
+
 // The dispatch condition of SM:3576-3584, computed exactly as the device's
 // has_lowered(): each typed count capped at the table size before the sum,
 // so a huge count cannot wrap the sum to zero on one side only.
@@ -98,13 +100,14 @@ hipError_t launch_sdf_batch(const FptRenderConfig *dcfgs, const BatchScene *dsce
     hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, counter);
     const ProposerDevice none{};
     const bool p = prof != nullptr;
+    if (event_batch <= 0) event_batch = default_event_batch(uses_path_stash(true, false, false));
     if (lowered)
         launch_profiled<true, false, false, false, true>(p, dim3(workgroups), stream, dcfgs, dscenes, nscenes, accum,
-                                                         0u, spp_cap, counter, total_slots, event_batch, 64, prof, none,
+                                                         0u, spp_cap, counter, total_slots, event_batch, kDefaultRefill, prof, none,
                                                          tile_order);
     else
         launch_profiled<true, false, false, false, false>(p, dim3(workgroups), stream, dcfgs, dscenes, nscenes, accum,
-                                                          0u, spp_cap, counter, total_slots, event_batch, 64, prof, none,
+                                                          0u, spp_cap, counter, total_slots, event_batch, kDefaultRefill, prof, none,
                                                           tile_order);
     return hipGetLastError();
 }

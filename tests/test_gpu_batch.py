@@ -63,7 +63,9 @@ def test_autotune_picks_a_valid_shape_and_keeps_the_image():
         st.synchronize()
         before = buf.numpy()
         r = b.autotune(buf.address, st.address, rounds=1)
-        assert r.candidates == 40 and r.best_ms > 0 and 8 <= r.event_batch <= 40 and r.workgroups > 0
+        # the batch instance parks paths in LDS: at most two workgroups per
+        # compute unit, event batches of 32-64 paths, two tile orders
+        assert r.candidates == 20 and r.best_ms > 0 and 32 <= r.event_batch <= 64 and r.workgroups > 0
         assert r.tile_order in (0, 1)
         b.render_into(buf.address, st.address)
         st.synchronize()
