@@ -87,9 +87,9 @@ TYPED_FLOPS_PER_DISTANCE_EVAL = 36
 # brick entered) + 1 B per sparse-bricks8 material id read.
 VOXEL_PAGE_ENTRY_BYTES, VOXEL_CELL_BYTES = 4, 1
 # HBM traffic of one voxel frame launch (1920x1080, 4 spp), FETCH_SIZE and
-# WRITE_SIZE KiB per dispatch (profiles/voxel_accumulate_table_mandelbox_1024_1080p_4spp.md),
+# WRITE_SIZE KiB per dispatch (profiles/voxel_persistent_mandelbox_1024_1080p_4spp.md),
 # scaled like the default line's.
-VOXEL_TRAFFIC_FETCH_KIB, VOXEL_TRAFFIC_WRITE_KIB = 580290.8125, 671285.78125
+VOXEL_TRAFFIC_FETCH_KIB, VOXEL_TRAFFIC_WRITE_KIB = 95512.375, 76903.0
 VOXEL_TRAFFIC_WORKLOAD = (1920, 1080, 4, 1024)
 # Batched workload: arithmetic FLOPs per orbit iteration per leaf family.
 # Mandelbulb of a general power in the triplex form (as above, 22 + 7
@@ -533,7 +533,7 @@ def run_workload(args, api, scenes):
                               VOXEL_TRAFFIC_WRITE_KIB * TRAFFIC_WRITE_SCALE)
         line["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                            "kernel": "voxel_accumulate_table_kernel",
+                            "kernel": "voxel_persistent_kernel",
                             "launches_per_step": launches, "bytes_per_step": nbytes}
         line["config"]["counts"] = st
         if not args.no_cpu_baseline:

@@ -650,7 +650,8 @@ int fpt_hip_plan_render(FptHipPlan *plan, float *device_accum, void *stream, cha
     float4 *acc = reinterpret_cast<float4 *>(device_accum);
     for (uint32_t start = 0; start < spp; start += chunk) {
         hipError_t e = c.renderer_backend == FPT_RENDERER_VOXEL
-                           ? fpt::launch_voxel_accumulate(plan->dcfg, c, plan->voxels, acc, start, chunk, s, nullptr)
+                           ? fpt::launch_voxel_accumulate(plan->dcfg, c, plan->voxels, acc, start, chunk, s, nullptr,
+                                                          plan->kernel == 1u, plan->workgroups, int(plan->event_batch))
                            : launch_sdf(plan, acc, start, chunk, nullptr, s);
         HIP_TRY(e, "render kernel launch");
     }
@@ -751,7 +752,8 @@ int fpt_hip_plan_voxel_stats(FptHipPlan *plan, float *device_accum, FptVoxelTrac
     float4 *acc = reinterpret_cast<float4 *>(device_accum);
     for (uint32_t s0 = 0; s0 < spp; s0 += chunk)
         HIP_TRY(fpt::launch_voxel_accumulate(plan->dcfg, c, plan->voxels, acc, s0, chunk, st.s,
-                                             counts.as<unsigned long long>()),
+                                             counts.as<unsigned long long>(), plan->kernel == 1u, plan->workgroups,
+                                             int(plan->event_batch)),
                 "voxel counting launch");
     unsigned long long h[3] = {0, 0, 0};
     HIP_TRY(hipMemcpyAsync(h, counts.p, sizeof(h), hipMemcpyDeviceToHost, st.s), "counter readback");

@@ -851,8 +851,9 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
     }
 }
 
-// Counter reset as a kernel (not a memset node), so a captured graph replays it.
-__global__ void zero_counter_kernel(uint32_t *c) {
+// Counter reset as a kernel (not a memset node), so a captured graph replays
+// it.  Internal linkage: every translation unit that launches it has its own.
+static __global__ void zero_counter_kernel(uint32_t *c) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0u;
 }
 

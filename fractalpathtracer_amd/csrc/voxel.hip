@@ -40,6 +40,7 @@
 #include "post.h"
 #include "bounds.h"
 #include "voxel_api.h"
+#include "persistent_kernel.h"
 
 namespace fpt {
 
@@ -948,6 +949,421 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void v
     voxel_accumulate<false>(cfgp, v, focus_p, accum, start, count, dda);
 }
 
+// ---------------------------------------------------------------------------
+// Persistent voxel path tracer for stored materials and face normals.
+//
+// The kernel above runs one pixel per lane through all its samples, bounces
+// and sun rays, so a wave waits at every traversal for its longest ray (ray
+// lengths through the grid range from a few cells to thousands).  Here every
+// lane is a state machine over the same path (SM:6526-6621):
+//   TRACE  one cell of the exact DDA per loop trip (SM:4560-4611), for
+//          camera, bounce and sun rays;
+//   EVENT  the traversal has ended; once `event_batch` lanes of the wave
+//          wait (or no lane traverses) they shade together: material, sun
+//          ray set-up, scatter, path end;
+//   IDLE   no pixel left.
+// A finished path starts the pixel's next sample in the same lane, a
+// finished pixel takes the next 8x8-tile slot from a global counter, so the
+// running mean and every RNG key are those of the one-path kernel and the
+// image is bit-identical.  BRICKS8: the grid is stored as 8^3 bricks of u8
+// material ids (the lookup is specialised); otherwise any layout.
+// ---------------------------------------------------------------------------
+enum : int { VX_IDLE = 0, VX_TRACE = 1, VX_EVENT = 2 };
+
+// Face normal as a code: 0 = zero vector, 1 + 2*axis + (component > 0).
+__device__ __forceinline__ uint32_t normal_code(f3 n) {
+    if (n.x != 0.0f) return 1u + (n.x > 0.0f ? 1u : 0u);
+    if (n.y != 0.0f) return 3u + (n.y > 0.0f ? 1u : 0u);
+    if (n.z != 0.0f) return 5u + (n.z > 0.0f ? 1u : 0u);
+    return 0u;
+}
+__device__ __forceinline__ f3 normal_of(uint32_t c) {
+    const float s = (c & 1u) ? -1.0f : 1.0f;  // codes 1, 3, 5 are negative
+    return c == 0u ? mk(0.f, 0.f, 0.f) : (c <= 2u ? mk(s, 0.f, 0.f) : (c <= 4u ? mk(0.f, s, 0.f) : mk(0.f, 0.f, s)));
+}
+
+// advance_dda with the entry face normal kept as its code (same float
+// operations, same cell sequence).
+__device__ __forceinline__ bool advance_dda_code(i3 &cell, i3 step, f3 &next_t, f3 delta_t, float &entry_t,
+                                                 uint32_t &enc) {  // SM:4169-4190
+    float b = fminf(next_t.x, fminf(next_t.y, next_t.z));
+    if (!isfinite(b)) return false;
+    bool cx = next_t.x == b, cy = next_t.y == b, cz = next_t.z == b;
+    float ne = b;
+    if (!(ne > entry_t)) {
+        ne = next_positive_float(entry_t);
+        if (!isfinite(ne) || !(ne > entry_t)) return false;
+    }
+    entry_t = ne;
+    const int axis = cx ? 0 : (cy ? 1 : 2);
+    enc = 1u + 2u * uint32_t(axis) + (icomp(step, axis) > 0 ? 0u : 1u);  // normal -1 (code odd) against the step
+    if (cx) { cell.x += step.x; next_t.x = next_t.x + delta_t.x; }
+    if (cy) { cell.y += step.y; next_t.y = next_t.y + delta_t.y; }
+    if (cz) { cell.z += step.z; next_t.z = next_t.z + delta_t.z; }
+    return true;
+}
+
+#ifndef FPT_VOXEL_PERSISTENT_WAVES
+#define FPT_VOXEL_PERSISTENT_WAVES 4
+#endif
+// DDA cells a traversing lane visits per loop trip.
+#ifndef FPT_VOXEL_STEPS_PER_TRIP
+#define FPT_VOXEL_STEPS_PER_TRIP 4
+#endif
+
+template <bool PROFILE, bool BRICKS8>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FPT_VOXEL_PERSISTENT_WAVES), flatten))
+void voxel_persistent_kernel(const FptRenderConfig *__restrict__ cfgp, VoxelView v, const float *__restrict__ focus_p,
+                             float4 *__restrict__ accum, uint32_t start, uint32_t count,
+                             uint32_t *__restrict__ counter, uint32_t total_slots, int event_batch,
+                             unsigned long long *dda) {
+    const FptRenderConfig &cfg = *cfgp;
+    const uint32_t W = cfg.width, H = cfg.height;
+    const uint32_t total = cfg.samples < 1u ? 1u : (cfg.samples > 512u ? 512u : cfg.samples);
+    const uint32_t s_end = start + count < total ? start + count : total;
+    const float focus = *focus_p;
+    const f3 cam = ld3(cfg.camera_position);
+    const float focal = 1.0f / tanf(cfg.camera_fov / 2.0f * kPi / 180.0f);
+    const float aa = 0.3f / fmaxf(float(W), float(H));
+    int bounces = int(cfg.render[0]) < 8 ? int(cfg.render[0]) : 8;
+    if (cfg.sdf_bounce_cap > 0u && int(cfg.sdf_bounce_cap) < bounces) bounces = int(cfg.sdf_bounce_cap);
+    const int res = int(cfg.voxel_resolution > 1u ? cfg.voxel_resolution : 1u);
+    const f3 bmin = ld3(cfg.voxel_bounds_min), bmax = ld3(cfg.voxel_bounds_max);
+    const f3 cs = cell_size_of(cfg);
+    const float cmin = fminf(cs.x, fminf(cs.y, cs.z));
+    const uint32_t max_steps = uint32_t(res) * 3u + 3u;
+    const bool sun_on = cfg.sun[0] == 1.0f;
+    const f3 sun_dir0 = rotate_camera(mk(0.f, 0.f, 1.f), cfg.sun[1] * kPi / 180.0f, cfg.sun[2] * kPi / 180.0f);
+
+    DdaWork dw = {0u, 0u, 0u};
+    // Pixel and path state.
+    int phase = VX_IDLE;
+    uint32_t pix = 0, s = 0;
+    UV uv = {0.f, 0.f};
+    f3 acc = mk(0.f, 0.f, 0.f), pos = mk(0.f, 0.f, 0.f), dir = mk(0.f, 0.f, 1.f);
+    f3 light = mk(0.f, 0.f, 0.f), color = mk(1.f, 1.f, 1.f), gcol = mk(0.f, 0.f, 0.f);
+    float sky_mask = 0.0f;
+    int b = 0;
+    bool shadow = false;
+    // The surface hit a sun ray starts from (kept across the sun ray).
+    f3 hpos = mk(0.f, 0.f, 0.f);
+    uint32_t hpay = 0u, hidx = 0u, hnc = 0u;
+    // Traversal state of the ray in flight (trace_voxel's locals).
+    f3 ro = mk(0.f, 0.f, 0.f), rd = mk(0.f, 0.f, 1.f);
+    i3 cell = {0, 0, 0}, step = {0, 0, 0};
+    f3 next_t = mk(kInf, kInf, kInf), delta_t = mk(kInf, kInf, kInf);
+    float entry_t = 0.f, far_t = 0.f;
+    uint32_t enc = 0u, it = 0u, cur_brick = 0xffffffffu, page = 0u;
+    // Result of the traversal that ended: hit, its entry distance, payload,
+    // record index and face normal.
+    bool rhit = false;
+    float rt = kInf;
+    uint32_t rpay = 0u, ridx = 0u, rnc = 0u;
+
+    // traceVoxel's set-up (SM:4492-4559): slab test, entry cell, steps.
+    auto start_trace = [&](f3 o, f3 d) __attribute__((always_inline)) {
+        ro = o;
+        rd = d;
+        rhit = false;
+        it = 0u;
+        cur_brick = 0xffffffffu;
+        page = 0u;
+        phase = VX_EVENT;  // a ray that misses the grid ends here
+        float near_t;
+        f3 en;
+        if (!ray_aabb(o, d, bmin, bmax, near_t, far_t, en)) return;
+        entry_t = fmaxf(near_t, 0.0f);
+        f3 st = o + d * entry_t;
+        st = mk(clampf(st.x, bmin.x, bmax.x), clampf(st.y, bmin.y, bmax.y), clampf(st.z, bmin.z, bmax.z));
+        if (!cell_coordinate(st, d, bmin, bmax, cs, res, cell)) return;
+        for (int a = 0; a < 3; ++a) {
+            float da = comp(d, a);
+            int sg = 0;
+            float nt = kInf, dt = kInf;
+            if (da > 0.0f) {
+                sg = 1;
+                float boundary = comp(bmin, a) + float(icomp(cell, a) + 1) * comp(cs, a);
+                nt = __fdiv_rn(boundary - comp(o, a), da);
+                dt = __fdiv_rn(comp(cs, a), da);
+            } else if (da < 0.0f) {
+                sg = -1;
+                float boundary = comp(bmin, a) + float(icomp(cell, a)) * comp(cs, a);
+                nt = __fdiv_rn(boundary - comp(o, a), da);
+                dt = __fdiv_rn(-comp(cs, a), da);
+            }
+            if (!isfinite(nt) || !isfinite(dt)) {
+                nt = kInf;
+                dt = kInf;
+            }
+            if (a == 0) { step.x = sg; next_t.x = nt; delta_t.x = dt; }
+            else if (a == 1) { step.y = sg; next_t.y = nt; delta_t.y = dt; }
+            else { step.z = sg; next_t.z = nt; delta_t.z = dt; }
+        }
+        enc = normal_code(en);
+        phase = VX_TRACE;
+    };
+
+    // Camera ray of sample s of the current pixel (render_voxel_path's start).
+    auto start_sample = [&]() __attribute__((always_inline)) {
+        const float frame = float(s);
+        float jx, jy;
+        random_point(aa, uv, frame, jx, jy);
+        f3 d = rotate_camera_roll(normalize(mk(uv.x + jx, uv.y + jy, focal)), cfg.camera_yaw_pitch[0],
+                                  cfg.camera_yaw_pitch[1], cfg.camera_roll);
+        const f3 fp = cam + d * focus;
+        float lx, ly;
+        random_point(cfg.camera_dof, uv, frame, lx, ly);
+        pos = cam + rotate_camera_roll(mk(lx, ly, 0.f), cfg.camera_yaw_pitch[0], cfg.camera_yaw_pitch[1],
+                                       cfg.camera_roll);
+        dir = normalize(fp - pos);
+        light = mk(0.f, 0.f, 0.f);
+        color = mk(1.f, 1.f, 1.f);
+        sky_mask = 0.0f;
+        const float gt = clampf((dir.y + 1.0f) / 2.0f, 0.f, 1.f);
+        gcol = vmix(ld3(&cfg.background_gradient[3]), ld3(&cfg.background_gradient[0]), gt);
+        b = 0;
+        shadow = false;
+        start_trace(pos, dir);
+    };
+
+    bool retry = false;
+    auto take_slot = [&](bool want) __attribute__((always_inline)) {
+        const uint32_t k = fetch_slot(want, counter);
+        retry = false;
+        if (!want) return;
+        if (k >= total_slots) {
+            phase = VX_IDLE;
+            return;
+        }
+        uint32_t x, y;
+        pixel_of(k, W, nullptr, x, y);
+        if (x >= W || y >= H) {
+            retry = true;
+            return;
+        }
+        pix = y * W + x;
+        uv = screen_uv(x, y, W, H);
+        s = start;
+        if (start != 0u) {
+            const float4 prev = accum[pix];
+            acc = mk(prev.x, prev.y, prev.z);
+        } else {
+            acc = mk(0.f, 0.f, 0.f);
+        }
+        start_sample();
+    };
+
+    take_slot(true);
+    while (__any(retry)) take_slot(retry);
+
+    while (true) {
+#pragma unroll 1
+        for (int k = 0; k < FPT_VOXEL_STEPS_PER_TRIP && phase == VX_TRACE; ++k) {
+            // One iteration of traceVoxel's cell loop (SM:4560-4611).
+            bool end = it >= max_steps || !(entry_t <= far_t) || cell.x < 0 || cell.y < 0 || cell.z < 0 ||
+                       cell.x >= res || cell.y >= res || cell.z >= res;
+            if (!end) {
+                dw.visited++;
+                uint32_t payload = 0u, index = 0u;
+                bool occupied;
+                if (BRICKS8) {
+                    const uint32_t bx = uint32_t(cell.x) >> 3, by = uint32_t(cell.y) >> 3, bz = uint32_t(cell.z) >> 3;
+                    const uint32_t brick = bx + by * v.brick_grid + bz * v.brick_grid * v.brick_grid;
+                    if (brick != cur_brick) {
+                        cur_brick = brick;
+                        page = v.page_table[brick];
+                        dw.pages++;
+                    }
+                    occupied = false;
+                    if (page != 0u) {
+                        dw.cells++;
+                        const uint32_t local = (uint32_t(cell.x) & 7u) + ((uint32_t(cell.y) & 7u) << 3) +
+                                               ((uint32_t(cell.z) & 7u) << 6);
+                        index = ((page - 1u) << 9) + local;
+                        payload = v.material[index];
+                        occupied = payload != 0u;
+                    }
+                } else if (v.storage == FPT_VOXEL_STORAGE_DENSE) {
+                    index = uint32_t(cell.x) + uint32_t(cell.y) * uint32_t(res) +
+                            uint32_t(cell.z) * uint32_t(res) * uint32_t(res);
+                    payload = v.records[index].packed_color;
+                    dw.cells++;
+                    occupied = (payload & 0x80000000u) != 0u;
+                } else {
+                    const uint32_t sh = v.brick_shift, mask = (1u << sh) - 1u;
+                    uint32_t bx = uint32_t(cell.x) >> sh, by = uint32_t(cell.y) >> sh, bz = uint32_t(cell.z) >> sh;
+                    uint32_t brick = bx + by * v.brick_grid + bz * v.brick_grid * v.brick_grid;
+                    if (brick != cur_brick) {
+                        cur_brick = brick;
+                        page = v.page_table[brick];
+                        dw.pages++;
+                    }
+                    occupied = false;
+                    if (page != 0u) {
+                        dw.cells++;
+                        uint32_t local = (uint32_t(cell.x) & mask) + ((uint32_t(cell.y) & mask) << sh) +
+                                         ((uint32_t(cell.z) & mask) << (2u * sh));
+                        index = ((page - 1u) << (3u * sh)) + local;
+                        if (v.storage == FPT_VOXEL_STORAGE_SPARSE_BRICKS8) {
+                            payload = v.material[index];
+                            occupied = payload != 0u;
+                        } else {
+                            payload = v.records[index].packed_color;
+                            occupied = (payload & 0x80000000u) != 0u;
+                        }
+                    }
+                }
+                if (occupied) {
+                    rhit = true;
+                    rt = entry_t;
+                    rpay = payload;
+                    ridx = index;
+                    rnc = enc;
+                    end = true;
+                } else {
+                    if (!advance_dda_code(cell, step, next_t, delta_t, entry_t, enc)) end = true;
+                    ++it;
+                }
+            }
+            if (end) phase = VX_EVENT;
+        }
+        const unsigned long long ev = __ballot(phase == VX_EVENT);
+        const unsigned long long tr = __ballot(phase == VX_TRACE);
+        if (!ev && !tr) break;
+        if (!(__popcll(ev) >= event_batch || (ev && !tr))) continue;
+
+        bool need_slot = false;
+        if (phase == VX_EVENT) {
+            const float frame = float(s);
+            const float fb = float(b);
+            bool path_done = false;
+            bool scatter = false;
+            f3 n = mk(0.f, 0.f, 0.f);
+            Material mat;
+            if (!shadow) {
+                // End of a camera / bounce traversal (SM:6532-6560).
+                if (!rhit) {
+                    if (b == 0) sky_mask = 1.0f;
+                    light = light + environment(dir, cfg);
+                    path_done = true;
+                } else {
+                    hpos = ro + rd * rt;
+                    hpay = rpay;
+                    hidx = ridx;
+                    hnc = rnc;
+                    VHit h;
+                    h.payload = hpay;
+                    h.index = hidx;
+                    mat = table_material(h, v);
+                    if (mat.emission > 0.001f) light = light + mat.rgb * mat.emission;
+                    if (sun_on) {
+                        // Sun ray (SM:6490-6515): its traversal runs as a
+                        // TRACE phase; shading resumes at its end.
+                        float h1 = hash13(uv.x, uv.y, madd_rn(frame, 5.0f, 1.0f));
+                        float h2 = hash13(uv.x, uv.y, madd_rn(frame, 3.0f, 5.0f));
+                        float sn, cn;
+                        sincosf(h1 * 2.0f * kPi, &sn, &cn);
+                        float m = sqrtf(h2) * cfg.sun[4];
+                        const f3 ld = rotate_camera(sun_dir0, cn * m, sn * m);
+                        n = normal_of(hnc);
+                        if (dot(n, dir) > 0.0f) n = -n;
+                        shadow = true;
+                        start_trace(offset_origin(hpos, n, ld, cfg), ld);
+                    } else {
+                        scatter = true;
+                    }
+                }
+            } else {
+                // End of the sun ray: light it carries (SM:6510-6514).
+                shadow = false;
+                VHit h;
+                h.payload = hpay;
+                h.index = hidx;
+                mat = table_material(h, v);
+                if (!rhit) {
+                    n = normal_of(hnc);
+                    if (dot(n, dir) > 0.0f) n = -n;
+                    float diff = fmaxf(dot(n, rd), 0.0f);
+                    light = light + ld3(cfg.sun_color) * (cfg.sun[3] * diff * mat.roughness * (1.0f - mat.translucency));
+                }
+                scatter = true;
+            }
+            if (scatter) {
+                // Scatter (SM:6561-6613).
+                n = normal_of(hnc);
+                if (dot(n, dir) > 0.0f) n = -n;
+                const f3 on = n;
+                float rs = hash13(uv.x, uv.y, madd_rn(frame, 1.37f, fb));
+                float rf = hash13(uv.x, uv.y, madd_rn(frame, 7.91f, fb));
+                bool transmitted = false;
+                if (rs > mat.translucency) {
+                    f3 refl = reflect(dir, n);
+                    f3 diff = random_vector(n, uv, madd_rn(frame, 13.37f, fb));
+                    float q = (mat.ior - 1.0f) / (mat.ior + 1.0f);
+                    float f0 = q * q;
+                    float om = 1.0f - clampf(dot(n, -dir), 0.f, 1.f);
+                    float om2 = om * om;
+                    float fres = f0 + (1.0f - f0) * (om2 * om2 * om);
+                    dir = normalize(vmix(refl, diff, mat.roughness));
+                    if (rf < fres * mat.specular) dir = refl;
+                    else color = color * mat.rgb;
+                } else {
+                    float q = (mat.ior - 1.0f) / (mat.ior + 1.0f);
+                    float f0 = q * q;
+                    float om = 1.0f - clampf(dot(n, -dir), 0.f, 1.f);
+                    float om2 = om * om;
+                    float fres = f0 + (1.0f - f0) * (om2 * om2 * om);
+                    f3 refr = refract(dir, n, 1.0f / mat.ior);
+                    f3 refl = reflect(dir, n);
+                    if (dot(refr, refr) < 0.000001f || !isfinite(refr.x) || rf < fres) {
+                        dir = normalize(refl);
+                    } else {
+                        dir = normalize(refr);
+                        color = color * (vmix(mk(1.f, 1.f, 1.f), mat.rgb, 0.35f) * (1.0f - fres * 0.5f));
+                        transmitted = true;
+                    }
+                }
+                pos = transmitted ? hpos + dir * (cmin * 1.05f) : offset_origin(hpos, on, dir, cfg);
+                bool stop = false;
+                if (cfg.sdf_russian_roulette != 0u && b + 1 < bounces && float(b + 1) >= cfg.sdf_rr_start) {
+                    float surv = clampf(fmaxf(color.x, fmaxf(color.y, color.z)), clampf(cfg.sdf_rr_min_prob, 0.01f, 1.0f),
+                                        1.0f);
+                    float rr = hash13(uv.x, uv.y, madd_rn(frame, 19.19f, __fmul_rn(fb, 3.17f)));
+                    if (rr > surv) stop = true;
+                    else color = color / surv;
+                }
+                ++b;
+                if (stop || b >= bounces) path_done = true;
+                else start_trace(pos, dir);
+            }
+            if (path_done) {
+                f3 col = light * color;
+                if (cfg.world[6] == 1.0f) col = vmix(col, gcol, sky_mask);
+                col = vmin(col, 8.0f);
+                acc = vmix(acc, col, 1.0f / float(s + 1u));
+                ++s;
+                if (s < s_end) {
+                    start_sample();
+                } else {
+                    accum[pix] = make_float4(acc.x, acc.y, acc.z, 1.0f);
+                    need_slot = true;
+                }
+            }
+        }
+        take_slot(need_slot);
+        while (__any(retry)) take_slot(retry);
+    }
+
+    if (PROFILE && dda) {  // counting pass: {cells visited, page-table reads, payload reads}
+        const uint32_t vals[3] = {dw.visited, dw.pages, dw.cells};
+        for (int k = 0; k < 3; ++k) {
+            unsigned long long sum = (unsigned long long)vals[k];
+            for (int off = 32; off > 0; off >>= 1) sum += __shfl_down(sum, off, 64);
+            if ((threadIdx.x & 63u) == 0u && sum) atomicAdd(&dda[k], sum);
+        }
+    }
+}
+
 // Test hook: per-ray visit lists and hits for given rays.
 struct VisitRecorder {
     int *out;
@@ -1318,14 +1734,14 @@ int voxel_build(const FptRenderConfig &cfg, const FptRenderConfig *dcfg, VoxelSt
 }
 
 void voxel_free(VoxelStore &s) {
-    for (void *p : {s.page_table, s.pool, s.material, s.distance, s.palette, s.focus})
+    for (void *p : {s.page_table, s.pool, s.material, s.distance, s.palette, s.focus, s.counter})
         if (p) (void)hipFree(p);
     s = VoxelStore();
 }
 
 hipError_t launch_voxel_accumulate(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, VoxelStore &store,
                                    float4 *accum, uint32_t start, uint32_t count, hipStream_t stream,
-                                   unsigned long long *dda) {
+                                   unsigned long long *dda, bool persistent, uint32_t workgroups, int event_batch) {
     VoxelView v = view_of(store);
     if (!store.focus) {
         hipError_t e = hipMalloc(&store.focus, sizeof(float));
@@ -1333,6 +1749,36 @@ hipError_t launch_voxel_accumulate(const FptRenderConfig *dcfg, const FptRenderC
         hipLaunchKernelGGL(voxel_focus_kernel, dim3(1), dim3(64), 0, stream, dcfg, v, static_cast<float *>(store.focus));
         e = hipGetLastError();
         if (e != hipSuccess) return e;
+    }
+    if (persistent && !voxel_needs_field(hcfg)) {
+        if (!store.counter) {
+            hipError_t e = hipMalloc(&store.counter, sizeof(uint32_t));
+            if (e != hipSuccess) return e;
+        }
+        uint32_t *counter = static_cast<uint32_t *>(store.counter);
+        hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, counter);
+        if (!workgroups) {
+            int dev = 0, cus = 256;
+            if (hipGetDevice(&dev) == hipSuccess)
+                (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+            workgroups = uint32_t(cus > 0 ? cus : 256) * uint32_t(FPT_VOXEL_PERSISTENT_WAVES);
+        }
+        if (event_batch <= 0) event_batch = kVoxelEventBatch;
+        const uint32_t slots = slot_count(hcfg.width, hcfg.height);
+        const float *focus = static_cast<const float *>(store.focus);
+        const bool b8 = store.storage == FPT_VOXEL_STORAGE_SPARSE_BRICKS8;
+#define FPT_VX_LAUNCH(P, B8)                                                                                       \
+    hipLaunchKernelGGL((voxel_persistent_kernel<P, B8>), dim3(workgroups), dim3(256), 0, stream, dcfg, v, focus,   \
+                       accum, start, count, counter, slots, event_batch, dda)
+        if (dda) {
+            if (b8) FPT_VX_LAUNCH(true, true);
+            else FPT_VX_LAUNCH(true, false);
+        } else {
+            if (b8) FPT_VX_LAUNCH(false, true);
+            else FPT_VX_LAUNCH(false, false);
+        }
+#undef FPT_VX_LAUNCH
+        return hipGetLastError();
     }
     dim3 grid((hcfg.width + 15u) / 16u, (hcfg.height + 15u) / 16u, 1u);
     if (voxel_needs_field(hcfg))
