@@ -487,7 +487,7 @@ hipError_t launch_sdf(FptHipPlan *plan, float4 *acc, uint32_t start, uint32_t ch
                                       plan->tile_order_mode ? plan->tile_order : nullptr, s);
 }
 
-// Hand-out order of tiles from their probe estimates (tile_cost_kernel):
+// Hand-out order of tiles from their work estimates (scene_tile_costs):
 // most work first, ties in row-major order.  Cheap tiles then form the
 // frame's tail, so lanes run out of work together.  Any order gives the same
 // image, since every pixel is rendered by one lane alone.
@@ -499,14 +499,28 @@ std::vector<uint32_t> tile_order_from_costs(const std::vector<float> &cost) {
 }
 
 // The persistent kernel's tile order (tile_order_from_costs).
+// Work estimate of each 8x8 tile of one scene (tiles row-major) into
+// cost[tile]: one whole path sample per pixel (launch_pixel_cost), summed
+// over the tile's pixels.
+hipError_t scene_tile_costs(const FptRenderConfig *dcfg, const FptRenderConfig &c, float *cost) {
+    const uint32_t tiles_x = (c.width + 7u) / 8u;
+    const size_t npix = size_t(c.width) * c.height;
+    DevBuf px_dev;
+    hipError_t e = px_dev.alloc(npix * sizeof(float));
+    if (e == hipSuccess) e = fpt::launch_pixel_cost(dcfg, c, px_dev.as<float>(), nullptr);
+    std::vector<float> px(npix);
+    if (e == hipSuccess) e = hipMemcpy(px.data(), px_dev.p, npix * sizeof(float), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return e;
+    for (uint32_t y = 0; y < c.height; ++y)
+        for (uint32_t x = 0; x < c.width; ++x) cost[(y / 8u) * tiles_x + x / 8u] += px[size_t(y) * c.width + x];
+    return hipSuccess;
+}
+
 int plan_tile_order(FptHipPlan *plan, char *error, size_t error_len) {
     const FptRenderConfig &c = plan->host;
     const uint32_t tiles_x = (c.width + 7u) / 8u, ntiles = tiles_x * ((c.height + 7u) / 8u);
-    DevBuf cost;
-    HIP_TRY(cost.alloc(size_t(ntiles) * sizeof(float)), "tile cost buffer");
-    HIP_TRY(fpt::launch_tile_cost(plan->dcfg, c, cost.as<float>(), nullptr), "tile cost kernel");
-    std::vector<float> h(ntiles);
-    HIP_TRY(hipMemcpy(h.data(), cost.p, size_t(ntiles) * sizeof(float), hipMemcpyDeviceToHost), "tile cost readback");
+    std::vector<float> h(ntiles, 0.0f);
+    HIP_TRY(scene_tile_costs(plan->dcfg, c, h.data()), "tile cost probe");
     const std::vector<uint32_t> order = tile_order_from_costs(h);
     HIP_TRY(hipMalloc(&plan->tile_order, size_t(ntiles) * sizeof(uint32_t)), "tile order allocation");
     HIP_TRY(hipMemcpy(plan->tile_order, order.data(), size_t(ntiles) * sizeof(uint32_t), hipMemcpyHostToDevice),
@@ -1085,14 +1099,9 @@ int fpt_hip_batch_create(const FptRenderConfig *configs, uint32_t count, FptHipB
     // Longest-first hand-out of the batch's tiles, over all scenes.
     {
         const uint32_t ntiles = b->total_slots / 64u;
-        DevBuf cost;
-        e = cost.alloc(size_t(ntiles) * sizeof(float));
+        std::vector<float> h(ntiles, 0.0f);
         for (uint32_t i = 0; i < count && e == hipSuccess; ++i)
-            e = fpt::launch_tile_cost(b->dcfgs + i, b->host[i], cost.as<float>() + b->scenes[i].slot_begin / 64u,
-                                      nullptr);
-        std::vector<float> h(ntiles);
-        if (e == hipSuccess)
-            e = hipMemcpy(h.data(), cost.p, size_t(ntiles) * sizeof(float), hipMemcpyDeviceToHost);
+            e = scene_tile_costs(b->dcfgs + i, b->host[i], h.data() + b->scenes[i].slot_begin / 64u);
         const std::vector<uint32_t> order = tile_order_from_costs(h);
         if (e == hipSuccess) e = hipMalloc(&b->tile_order, size_t(ntiles) * sizeof(uint32_t));
         if (e == hipSuccess)

@@ -90,9 +90,9 @@ uint16_t half_bits_round_up(float v);
 hipError_t launch_focus_prepass(FptRenderConfig *dcfgs, uint32_t n, hipStream_t stream);
 hipError_t launch_trace_primary(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, FptRayHit *hits,
                                 hipStream_t stream);
-// Work estimate per 8x8 tile (tiles row-major) into cost[ceil(W/8)*ceil(H/8)].
-hipError_t launch_tile_cost(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, float *cost,
-                            hipStream_t stream);
+// Work of one whole path sample per pixel into cost[W*H] (row-major).
+hipError_t launch_pixel_cost(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, float *cost,
+                             hipStream_t stream);
 hipError_t launch_field_sample(const FptRenderConfig *dcfg, const float4 *pts, size_t n,
                                FptMandelbulberFieldSample *out, hipStream_t stream);
 hipError_t launch_surface_sample(const FptRenderConfig *dcfg, const float4 *pts, size_t n, float4 *out,

@@ -194,6 +194,23 @@ void sdf_accumulate_kernel(const FptRenderConfig *__restrict__ cfgp,
     }
 }
 
+// Work of one path sample per pixel (sample 0 of the frame's sequence, the
+// whole path: camera, bounce and sun marches and the per-hit work), as
+// distance evaluations + orbit iterations, into cost[y * width + x].  The
+// persistent kernel hands out tiles in the order of these costs.
+template <int K>
+__global__ __launch_bounds__(256) void pixel_cost_kernel(const FptRenderConfig *__restrict__ cfgp,
+                                                         float *__restrict__ cost) {
+    const FptRenderConfig &cfg = *cfgp;
+    uint32_t x, y;
+    tile_pixel(x, y);
+    if (x >= cfg.width || y >= cfg.height) return;
+    Work w = {0, 0};
+    PathCounts pc = {0, 0, 0, 0, 0};
+    (void)render_path<K>(screen_uv_any<K>(x, y, cfg), 0u, cfg, w, pc);
+    cost[size_t(y) * cfg.width + x] = float(w.distance_evals) + float(w.orbit_iterations);
+}
+
 // Un-jittered primary ray + normal per pixel (per-ray parity harness).
 __global__ __launch_bounds__(256) void trace_primary_kernel(const FptRenderConfig *__restrict__ cfgp,
                                                             FptRayHit *__restrict__ hits) {
@@ -221,34 +238,6 @@ __global__ __launch_bounds__(256) void trace_primary_kernel(const FptRenderConfi
     h.steps = uint32_t(steps);
     h._pad0 = 0u;
     hits[size_t(y) * cfg.width + x] = h;
-}
-
-// Work estimate of each 8x8 tile for the persistent kernel's hand-out order:
-// the un-jittered primary rays of four pixels of the tile, each charged its
-// march steps plus 256 when it reaches the surface (a surface hit starts the
-// bounce and shadow marches that dominate a path's cost).
-__global__ __launch_bounds__(256) void tile_cost_kernel(const FptRenderConfig *__restrict__ cfgp, uint32_t tiles_x,
-                                                        uint32_t ntiles, float *__restrict__ cost) {
-    const FptRenderConfig &cfg = *cfgp;
-    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    if (t >= ntiles) return;
-    const uint32_t x0 = (t % tiles_x) * 8u, y0 = (t / tiles_x) * 8u;
-    const f3 cam = ld3(cfg.camera_position);
-    const float far = cfg.render[4] * 0.99f;
-    float c = 0.0f;
-    for (uint32_t k = 0; k < 4u; ++k) {
-        const uint32_t x = min(x0 + ((k & 1u) ? 6u : 1u), cfg.width - 1u);
-        const uint32_t y = min(y0 + ((k & 2u) ? 6u : 1u), cfg.height - 1u);
-        const UV uv = screen_uv_any(x, y, cfg);
-        f3 dr = mk(0.f, 0.f, 0.f);
-        if (!camera_ray_any(uv.x, uv.y, cfg, dr)) continue;
-        Work w = {0, 0};
-        int steps = 0;
-        const f3 rp = march_any(dr, cam, int(cfg.render[1]), cfg.render[3], 0.0002f, cfg, w, steps);
-        const f3 tv = rp - cam;
-        c += float(steps) + (dot(tv, tv) <= far * far ? 256.0f : 0.0f);
-    }
-    cost[t] = c;
 }
 
 // Distance field samples {distance, radius, derivative, iterations}: the
@@ -670,11 +659,15 @@ hipError_t launch_trace_primary(const FptRenderConfig *dcfg, const FptRenderConf
     return hipGetLastError();
 }
 
-hipError_t launch_tile_cost(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, float *cost,
-                            hipStream_t stream) {
-    const uint32_t tiles_x = (hcfg.width + 7u) / 8u, ntiles = tiles_x * ((hcfg.height + 7u) / 8u);
-    hipLaunchKernelGGL(tile_cost_kernel, dim3((ntiles + 255u) / 256u), dim3(256), 0, stream, dcfg, tiles_x, ntiles,
-                       cost);
+hipError_t launch_pixel_cost(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, float *cost,
+                             hipStream_t stream) {
+    const dim3 grid = tile_grid(hcfg.width, hcfg.height), block(256);
+    if (hcfg.sdf_id == FPT_SDF_MANDELBULBER)
+        hipLaunchKernelGGL(pixel_cost_kernel<SCENE_MB>, grid, block, 0, stream, dcfg, cost);
+    else if (has_lowered_tables(hcfg))
+        hipLaunchKernelGGL(pixel_cost_kernel<SCENE_LOWERED>, grid, block, 0, stream, dcfg, cost);
+    else
+        hipLaunchKernelGGL(pixel_cost_kernel<SCENE_SDF>, grid, block, 0, stream, dcfg, cost);
     return hipGetLastError();
 }
 
