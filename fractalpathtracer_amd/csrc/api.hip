@@ -479,7 +479,7 @@ hipError_t launch_sdf(FptHipPlan *plan, float4 *acc, uint32_t start, uint32_t ch
         if (e != hipSuccess) return e;
         return fpt::launch_specialized(*plan->spec, plan->dcfg, acc, start, chunk, plan->counter,
                                        fpt::persistent_slots(c.width, c.height), wg,
-                                       plan->event_batch ? int(plan->event_batch) : fpt::default_event_batch(true),
+                                       plan->event_batch ? int(plan->event_batch) : fpt::default_event_batch(true, false),
                                        int(plan->refill), plan->tile_order_mode ? plan->tile_order : nullptr, s);
     }
     return fpt::launch_sdf_persistent(plan->dcfg, c, acc, start, chunk, plan->counter, wg, int(plan->event_batch),
@@ -1365,12 +1365,15 @@ int fpt_hip_autotune(FptHipPlan *plan, FptHipBatch *batch, float *device_accum, 
     std::vector<Shape> cand;
     // With the path stash a workgroup holds 53-57 KB of LDS, so at most two
     // are resident per compute unit, and the event batch counts the wave's
-    // up to 128 paths.
+    // up to 128 paths.  Single-scene frames run best at 32-48 paths (larger
+    // batches look as good on the few-sample proxy but lengthen the frame's
+    // tail); batches of scenes at 48-64.
     const bool stash = plan ? fpt::persistent_uses_stash(plan->host, plan->proposer.enabled) : fpt::batch_uses_stash();
     const std::vector<uint32_t> per_cus = stash ? std::vector<uint32_t>{1u, 2u} : std::vector<uint32_t>{1u, 2u, 3u, 4u};
-    const std::vector<uint32_t> ebs =
-        stash ? std::vector<uint32_t>{32u, 40u, 48u, 56u, 64u} : std::vector<uint32_t>{8u, 16u, 24u, 32u, 40u};
-    const uint32_t default_eb = uint32_t(fpt::default_event_batch(stash));
+    const std::vector<uint32_t> ebs = !stash ? std::vector<uint32_t>{8u, 16u, 24u, 32u, 40u}
+                                      : plan ? std::vector<uint32_t>{32u, 40u, 48u}
+                                             : std::vector<uint32_t>{32u, 40u, 48u, 56u, 64u};
+    const uint32_t default_eb = uint32_t(fpt::default_event_batch(stash, !plan));
     for (uint32_t order : {1u, 0u})
         for (uint32_t per_cu : per_cus)
             for (uint32_t eb : ebs) cand.push_back({per_cu * cus, eb, order});

@@ -58,9 +58,9 @@ bool single_bulb_program(const FptRenderConfig &c);
 bool persistent_uses_stash(const FptRenderConfig &c, bool learned);
 // The batched (multi-scene) persistent instance keeps parked paths in LDS.
 bool batch_uses_stash();
-// Event batch of a persistent launch given 0: 64 paths with the path stash,
-// 24 lanes without.
-int default_event_batch(bool stash);
+// Event batch of a persistent launch given 0: with the path stash 40 paths
+// for one scene, 64 for a batch of scenes; 24 lanes without.
+int default_event_batch(bool stash, bool multi);
 // Non-marching lanes of a wave before they take parked marching paths.
 constexpr int kDefaultRefill = 4;
 // The scene's program carries flat-union, canonical or typed-SoA tables: the

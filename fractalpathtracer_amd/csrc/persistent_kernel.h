@@ -189,7 +189,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 #define FPT_INTERLEAVED_BULB 1
 #endif
 #ifndef FPT_ORBIT_BATCH
-#define FPT_ORBIT_BATCH 32
+#define FPT_ORBIT_BATCH 40
 #endif
 // Minimum resident waves per SIMD for imported Mandelbulber scenes (MB).
 #ifndef FPT_MANDELBULBER_PERSISTENT_WAVES

@@ -24,7 +24,7 @@ bool persistent_uses_stash(const FptRenderConfig &c, bool learned) {
     return uses_path_stash(false, learned, c.sdf_id == FPT_SDF_MANDELBULBER);
 }
 
-int default_event_batch(bool stash) { return stash ? 64 : 24; }
+int default_event_batch(bool stash, bool multi) { return stash ? (multi ? 64 : 40) : 24; }
 
 bool batch_uses_stash() { return uses_path_stash(true, false, false); }
 
@@ -73,7 +73,7 @@ hipError_t launch_sdf_persistent(const FptRenderConfig *dcfg, const FptRenderCon
                                  int event_batch, int refill, DeviceProfile *prof, const ProposerDevice *prop,
                                  const uint32_t *tile_order, hipStream_t stream) {
     const uint32_t slots = slot_count(hcfg.width, hcfg.height);
-    if (event_batch <= 0) event_batch = default_event_batch(persistent_uses_stash(hcfg, prop != nullptr));
+    if (event_batch <= 0) event_batch = default_event_batch(persistent_uses_stash(hcfg, prop != nullptr), false);
     hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, counter);
     const ProposerDevice none{};
     const dim3 grid(workgroups);
@@ -100,7 +100,7 @@ hipError_t launch_sdf_batch(const FptRenderConfig *dcfgs, const BatchScene *dsce
     hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, counter);
     const ProposerDevice none{};
     const bool p = prof != nullptr;
-    if (event_batch <= 0) event_batch = default_event_batch(uses_path_stash(true, false, false));
+    if (event_batch <= 0) event_batch = default_event_batch(uses_path_stash(true, false, false), true);
     if (lowered)
         launch_profiled<true, false, false, false, true>(p, dim3(workgroups), stream, dcfgs, dscenes, nscenes, accum,
                                                          0u, spp_cap, counter, total_slots, event_batch, kDefaultRefill, prof, none,

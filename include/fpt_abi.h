@@ -778,7 +778,8 @@ void fpt_hip_plan_destroy(FptHipPlan *plan);
  *                single-scene exact-march instances count the wave's lanes
  *                and its paths parked in LDS (see below), and also run the
  *                per-hit work once as many paths wait as march.  0 = the
- *                instance's default: 64 with parked paths, 24 without.
+ *                instance's default: 40 with parked paths, 24 without
+ *                (batches of scenes, fpt_hip_batch_tune: 64 and 24).
  */
 int fpt_hip_plan_tune(FptHipPlan *plan,
                       uint32_t kernel,

@@ -214,3 +214,28 @@ def test_single_mandelbulb_leaf_matches_the_generic_program_march(power, iterati
     assert np.isfinite(a.linear).all() and a.linear[..., :3].max() > 0
     for key in ("distance_evals", "march_orbit_iterations", "primary_steps", "shadow_steps", "normal_evals"):
         assert a.profile[key] == b.profile[key], key
+
+
+@pytest.mark.parametrize("scene", ["bulb", "typed"])
+def test_path_stash_settings_do_not_change_images(scene):
+    """The path stash's knobs (refill threshold, event batch counted over a
+    wave's lanes and parked paths) change only which lane runs which path:
+    every setting renders the same frame as the one-path kernel, and refill
+    thresholds outside 1..64 are refused."""
+    if scene == "bulb":
+        cfg = scenes.mandelbulb_scene(72, 56, 3, bounces=2)
+    else:
+        cfg = scenes.typed_program_scene(64, 48, 2)
+    buf = api.DeviceBuffer(cfg.width * cfg.height * 4)
+    with api.Plan(cfg) as plan:
+        plan.tune(kernel=0)
+        plan.render_into(buf.address)
+        ref = buf.numpy().copy()
+        for refill, eb, wg in ((1, 1, 0), (64, 64, 0), (4, 0, 3), (17, 33, 1)):
+            plan.tune(kernel=1, workgroups=wg, event_batch=eb)
+            plan.tune_stash(refill)
+            plan.render_into(buf.address)
+            assert np.array_equal(buf.numpy(), ref), (refill, eb, wg)
+        for bad in (0, 65):
+            with pytest.raises(ffi.FptError, match="refill"):
+                plan.tune_stash(bad)
