@@ -397,10 +397,11 @@ class Plan:
         err = ffi.error_buffer()
         ffi.check(self.lib.fpt_hip_plan_tune(self.handle, kernel, workgroups, event_batch, err, len(err)), err)
 
-    def tune_stash(self, refill: int) -> None:
-        """Non-marching lanes before a wave refills them from its LDS path stash."""
+    def tune_stash(self, refill: int = 4, sample_groups: int = 4) -> None:
+        """Non-marching lanes before a wave refills them from its LDS path
+        stash, and sample runs per pixel (fpt_hip_plan_tune_stash)."""
         err = ffi.error_buffer()
-        ffi.check(self.lib.fpt_hip_plan_tune_stash(self.handle, refill, err, len(err)), err)
+        ffi.check(self.lib.fpt_hip_plan_tune_stash(self.handle, refill, sample_groups, err, len(err)), err)
 
     def set_tile_order(self, mode: int) -> None:
         """1 longest-first (default), 0 row-major (fpt_hip_set_tile_order)."""

@@ -444,6 +444,12 @@ struct FptHipPlan {
     // Path-stash refill: non-marching lanes of a wave before they exchange
     // their paths for parked marching ones (fpt_hip_plan_tune_stash).
     uint32_t refill = fpt::kDefaultRefill;
+    // Sample runs per pixel of the single-scene persistent kernels
+    // (fpt_hip_plan_tune_stash); their per-sample colours and run counters.
+    uint32_t sample_groups = 4;
+    float4 *split_colors = nullptr;
+    uint32_t *split_done = nullptr;
+    size_t split_colors_len = 0;
     uint32_t *counter = nullptr; // persistent pixel counter
     // Hand-out order of the persistent kernel's 8x8 tiles: the most work
     // first (longest-processing-time order from a probe of primary rays), so
@@ -463,6 +469,50 @@ uint32_t default_workgroups() {
     return uint32_t(cus > 0 ? cus : 256) * 2u;
 }
 
+// Shortest sample run: every finished run publishes its colours with a
+// device-scope release (an L2 write-back on gfx950) and the last one
+// acquires them (an L2 invalidate), so runs must be long enough for that to
+// be rare (a 28-primitive union at 16 spp split into runs of 4 ran 17%
+// slower; the 64-spp Mandelbulb in runs of 16 ran 4.5% faster).
+constexpr uint32_t kMinSampleRun = 16u;
+
+// Sample runs of a launch over samples [start, start + chunk) of the frame:
+// runs of ceil(n / groups) samples, every run non-empty and at least
+// kMinSampleRun long.  Single-scene exact-march plans only (no proposer, no
+// imported Mandelbulber scene), and only while the per-sample colours fit in
+// 8 GB; otherwise one slot per pixel.  Allocates the colour and counter
+// buffers on first use.
+hipError_t plan_split(FptHipPlan *plan, uint32_t start, uint32_t chunk, fpt::SampleSplit &split) {
+    split = fpt::SampleSplit{nullptr, nullptr, 1u, 0u, 0u};
+    const FptRenderConfig &c = plan->host;
+    const uint32_t spp = clamp_samples(c.samples);
+    const uint32_t n = std::min(start + chunk, spp) > start ? std::min(start + chunk, spp) - start : 0u;
+    if (plan->sample_groups <= 1u || n <= 1u || plan->proposer.enabled || c.sdf_id == FPT_SDF_MANDELBULBER)
+        return hipSuccess;
+    const size_t npix = size_t(c.width) * c.height, len = npix * spp;
+    if (len * sizeof(float4) > (size_t(8) << 30)) return hipSuccess;
+    const uint32_t groups = std::min(plan->sample_groups, n / kMinSampleRun);
+    if (groups <= 1u) return hipSuccess;
+    const uint32_t run = (n + groups - 1u) / groups;
+    if (plan->split_colors_len < len) {
+        if (plan->split_colors) (void)hipFree(plan->split_colors);
+        if (plan->split_done) (void)hipFree(plan->split_done);
+        plan->split_colors = nullptr;
+        plan->split_done = nullptr;
+        plan->split_colors_len = 0;
+        hipError_t e = hipMalloc(&plan->split_colors, len * sizeof(float4));
+        if (e == hipSuccess) e = hipMalloc(&plan->split_done, npix * sizeof(uint32_t));
+        if (e != hipSuccess) return e;
+        plan->split_colors_len = len;
+    }
+    split.colors = plan->split_colors;
+    split.done = plan->split_done;
+    split.groups = (n + run - 1u) / run;
+    split.chunk = run;
+    split.stride = spp;
+    return hipSuccess;
+}
+
 hipError_t launch_sdf(FptHipPlan *plan, float4 *acc, uint32_t start, uint32_t chunk, fpt::DeviceProfile *prof,
                       hipStream_t s) {
     const FptRenderConfig &c = plan->host;
@@ -471,20 +521,24 @@ hipError_t launch_sdf(FptHipPlan *plan, float4 *acc, uint32_t start, uint32_t ch
         hipError_t e = hipMalloc(&plan->counter, sizeof(uint32_t));
         if (e != hipSuccess) return e;
     }
+    fpt::SampleSplit split;
+    hipError_t e = plan_split(plan, start, chunk, split);
+    if (e != hipSuccess) return e;
     uint32_t wg = plan->workgroups ? plan->workgroups : default_workgroups();
     if (plan->spec && !plan->proposer.enabled && !prof && !fpt::single_bulb_program(c)) {
         // Same kernel, built for the program's topology (counting passes use
         // the interpreter: the work counts are the same).
-        hipError_t e = fpt::launch_zero_counter(plan->counter, s);
+        e = fpt::launch_zero_counter(plan->counter, s);
+        if (e == hipSuccess && split.groups > 1u) e = fpt::launch_zero_words(split.done, c.width * c.height, s);
         if (e != hipSuccess) return e;
         return fpt::launch_specialized(*plan->spec, plan->dcfg, acc, start, chunk, plan->counter,
-                                       fpt::persistent_slots(c.width, c.height), wg,
+                                       fpt::persistent_slots(c.width, c.height) * split.groups, wg,
                                        plan->event_batch ? int(plan->event_batch) : fpt::default_event_batch(true, false),
-                                       int(plan->refill), plan->tile_order_mode ? plan->tile_order : nullptr, s);
+                                       int(plan->refill), plan->tile_order_mode ? plan->tile_order : nullptr, split, s);
     }
     return fpt::launch_sdf_persistent(plan->dcfg, c, acc, start, chunk, plan->counter, wg, int(plan->event_batch),
                                       int(plan->refill), prof, plan->proposer.enabled ? &plan->proposer.dev : nullptr,
-                                      plan->tile_order_mode ? plan->tile_order : nullptr, s);
+                                      plan->tile_order_mode ? plan->tile_order : nullptr, split, s);
 }
 
 // Hand-out order of tiles from their work estimates (scene_tile_costs):
@@ -693,7 +747,8 @@ int fpt_hip_plan_tune(FptHipPlan *plan, uint32_t kernel, uint32_t workgroups, ui
     return 0;
 }
 
-int fpt_hip_plan_tune_stash(FptHipPlan *plan, uint32_t refill, char *error, size_t error_len) {
+int fpt_hip_plan_tune_stash(FptHipPlan *plan, uint32_t refill, uint32_t sample_groups, char *error,
+                            size_t error_len) {
     if (!plan) {
         set_error(error, error_len, "missing plan");
         return 1;
@@ -702,7 +757,12 @@ int fpt_hip_plan_tune_stash(FptHipPlan *plan, uint32_t refill, char *error, size
         set_error(error, error_len, "invalid stash refill threshold %u (1..64 lanes)", refill);
         return 1;
     }
+    if (sample_groups < 1u || sample_groups > 64u) {
+        set_error(error, error_len, "invalid sample run count %u (1..64 per pixel)", sample_groups);
+        return 1;
+    }
     plan->refill = refill;
+    plan->sample_groups = sample_groups;
     return 0;
 }
 
@@ -710,6 +770,8 @@ void fpt_hip_plan_destroy(FptHipPlan *plan) {
     if (!plan) return;
     if (plan->dcfg) (void)hipFree(plan->dcfg);
     if (plan->counter) (void)hipFree(plan->counter);
+    if (plan->split_colors) (void)hipFree(plan->split_colors);
+    if (plan->split_done) (void)hipFree(plan->split_done);
     if (plan->tile_order) (void)hipFree(plan->tile_order);
     fpt::voxel_free(plan->voxels);
     fpt::proposer_release(plan->proposer);
@@ -1386,11 +1448,15 @@ int fpt_hip_autotune(FptHipPlan *plan, FptHipBatch *batch, float *device_accum, 
     if (plan && !plan->counter) HIP_TRY(hipMalloc(&plan->counter, sizeof(uint32_t)), "counter allocation");
     const uint32_t plan_proxy_spp = plan ? std::min<uint32_t>(clamp_samples(plan->host.samples), 4u) : 1u;
     auto launch = [&](const Shape &sh) -> hipError_t {
-        if (plan)  // proxy: the frame's first samples (up to 4)
+        if (plan) {  // proxy: the frame's first samples (up to 4)
+            fpt::SampleSplit split;
+            const hipError_t e = plan_split(plan, 0u, plan_proxy_spp, split);
+            if (e != hipSuccess) return e;
             return fpt::launch_sdf_persistent(plan->dcfg, plan->host, acc, 0u, plan_proxy_spp, plan->counter,
                                               sh.workgroups, int(sh.event_batch), int(plan->refill), nullptr,
                                               plan->proposer.enabled ? &plan->proposer.dev : nullptr,
-                                              sh.order ? plan->tile_order : nullptr, s);
+                                              sh.order ? plan->tile_order : nullptr, split, s);
+        }
         return fpt::launch_sdf_batch(batch->dcfgs, batch->dscenes, uint32_t(batch->host.size()), batch->total_slots, acc,
                                      batch->counter, sh.workgroups, int(sh.event_batch), 1u, nullptr, batch->lowered,
                                      sh.order ? batch->tile_order : nullptr, s);

@@ -36,6 +36,19 @@ struct DeviceProfile {
 
 struct ProposerDevice;
 
+// Sample groups of the single-scene persistent kernels: each pixel's samples
+// [start, start + count) are split into `groups` consecutive runs of `chunk`
+// samples, each a work slot of its own, so different lanes render them at
+// once.  Every sample's clamped colour goes to colors[pix * stride + s]; the
+// run that finishes last (done[pix] counts finished runs) folds the colours
+// in sample order into the running mean, exactly as one lane would have.
+// groups <= 1: one slot per pixel, the running mean in a register.
+struct SampleSplit {
+    float4 *colors;
+    uint32_t *done;
+    uint32_t groups, chunk, stride;
+};
+
 // One scene of a batched launch: its work slots are
 // [slot_begin, slot_begin + slots) of the global slot range (8x8 tiles of its
 // pixels), its image is accum[accum_offset .. accum_offset + width*height).
@@ -50,6 +63,8 @@ struct BatchScene {
 #ifndef __HIPCC_RTC__  // host launchers
 hipError_t launch_zero_counter(uint32_t *counter, hipStream_t stream);
 uint32_t persistent_slots(uint32_t width, uint32_t height);
+// Clears n words on the stream with a kernel (capturable).
+hipError_t launch_zero_words(uint32_t *p, uint32_t n, hipStream_t stream);
 // The scene's program is one Mandelbulb leaf: the persistent kernel's
 // interleaved-orbit variant applies.
 bool single_bulb_program(const FptRenderConfig &c);
@@ -104,10 +119,12 @@ hipError_t launch_typed_soa_validation(const FptRenderConfig *dcfg, uint32_t n, 
 // tile_order (optional, device): the order in which the 8x8 tiles are handed
 // out (a permutation of the row-major tile indices).  event_batch 0 takes
 // default_event_batch of the instance launched.
+// split: sample runs of the single-scene instances (SampleSplit; groups <= 1
+// renders one slot per pixel); the launcher clears split.done.
 hipError_t launch_sdf_persistent(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, float4 *accum,
                                  uint32_t start, uint32_t count, uint32_t *counter, uint32_t workgroups,
                                  int event_batch, int refill, DeviceProfile *prof, const ProposerDevice *prop,
-                                 const uint32_t *tile_order, hipStream_t stream);
+                                 const uint32_t *tile_order, const SampleSplit &split, hipStream_t stream);
 hipError_t launch_sdf_diagnostic(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg,
                                  const FptDiagnosticConfig &dg, uchar4 *out, hipStream_t stream);
 hipError_t launch_present(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, const float4 *accum,

@@ -221,7 +221,8 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
                                                              uint32_t count, uint32_t *__restrict__ counter,
                                                              uint32_t total_slots, int event_batch, int refill,
                                                              DeviceProfile *prof, ProposerDevice prop,
-                                                             const uint32_t *__restrict__ tile_order) {
+                                                             const uint32_t *__restrict__ tile_order,
+                                                             SampleSplit split) {
     // Path stash: each wave keeps up to 64 further paths parked in LDS, one
     // slot per lane index (slot j's phase lives in lane j's `slot_phase`).
     // Lanes whose path waits for the per-hit pass (or is idle) exchange it
@@ -283,9 +284,12 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
         load_scene();
     }
 
+    // Sample groups (single scene only): a slot is a run of a pixel's samples.
+    const bool SPLIT = !MULTI && !LEARNED && !MB && split.groups > 1u;
+
     // Lane state.
     int phase = PH_IDLE;
-    uint32_t pix = 0, s = 0;
+    uint32_t pix = 0, s = 0, s_stop = 0;  // s_stop: end of the slot's samples
     UV uv = {0.f, 0.f};
     f3 acc = mk(0.f, 0.f, 0.f);
     int bounce = 0, local_ni = 0;
@@ -376,7 +380,14 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
             phase = PH_IDLE;
             return;
         }
-        uint32_t local = k;
+        uint32_t local = k, run = 0u;
+        if (SPLIT) {
+            // Slots run tile by tile, each tile's sample runs one after the
+            // other: k = ((tile rank * groups + run) * 64 + pixel in tile).
+            const uint32_t kg = k >> 6, rank = kg / split.groups;
+            run = kg - rank * split.groups;
+            local = (rank << 6) + (k & 63u);
+        }
         if (MULTI) {
             // Slot in the hand-out order: tile_order permutes the batch's global
             // 8x8 tiles (scene after scene, row-major within a scene).
@@ -405,8 +416,9 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
         }
         pix = y * W + x;
         uv = MB ? screen_uv_any<SCENE_MB>(x, y, *cp) : screen_uv(x, y, W, H);
-        s = s_begin;
-        if (s_begin != 0u) {
+        s = SPLIT ? s_begin + run * split.chunk : s_begin;
+        s_stop = SPLIT && s + split.chunk < s_end ? s + split.chunk : s_end;
+        if (!SPLIT && s_begin != 0u) {
             float4 prev = out[pix];
             acc = mk(prev.x, prev.y, prev.z);
         } else {
@@ -420,7 +432,7 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
     // major so the 64 lanes of a wave touch 64 consecutive words).
     auto move_path = [&](uint32_t j, bool read) __attribute__((always_inline)) {
         uint4 *q = st + j;
-        uint32_t flags = uint32_t(shadow) | (uint32_t(borb) << 1);
+        uint32_t flags = uint32_t(shadow) | (uint32_t(borb) << 1) | (s_stop << 16);
         xchg4(q + 0 * 64, read, phase, pix, s, bounce);
         xchg4(q + 1 * 64, read, uv.x, uv.y, local_ni, it);
         xchg4(q + 2 * 64, read, acc.x, acc.y, acc.z, side);
@@ -443,6 +455,7 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
         }
         shadow = (flags & 1u) != 0u;
         borb = (flags & 2u) != 0u;
+        s_stop = flags >> 16;
     };
     // The r-th lane (in lane order) with `want` exchanges its path with the
     // path in the r-th slot of `offer`, for as many pairs as both sides have.
@@ -805,13 +818,36 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
                 f3 col = light * color;
                 if (cfg.world[6] == 1.0f) col = vmix(col, gcol, sky_mask);
                 col = vmin(col, 8.0f);
-                acc = vmix(acc, col, 1.0f / float(s + 1u));
+                if (SPLIT) split.colors[size_t(pix) * split.stride + s] = make_float4(col.x, col.y, col.z, 0.0f);
+                else acc = vmix(acc, col, 1.0f / float(s + 1u));
                 ++s;
-                if (s < s_end) {
+                if (s < s_stop) {
                     start_path();
                 } else {
-                    out[pix] = make_float4(acc.x, acc.y, acc.z, 1.0f);
-                    ++pixels_done;
+                    if (SPLIT) {
+                        // The last run of the pixel to finish folds every
+                        // sample's colour in order (release the colours
+                        // before the count, acquire them after it).
+                        __threadfence();
+                        if (atomicAdd(&split.done[pix], 1u) + 1u == split.groups) {
+                            __threadfence();
+                            f3 a = mk(0.f, 0.f, 0.f);
+                            if (s_begin != 0u) {
+                                const float4 prev = out[pix];
+                                a = mk(prev.x, prev.y, prev.z);
+                            }
+                            const float4 *cs = split.colors + size_t(pix) * split.stride;
+                            for (uint32_t i = s_begin; i < s_end; ++i) {
+                                const float4 c = cs[i];
+                                a = vmix(a, mk(c.x, c.y, c.z), 1.0f / float(i + 1u));
+                            }
+                            out[pix] = make_float4(a.x, a.y, a.z, 1.0f);
+                            ++pixels_done;
+                        }
+                    } else {
+                        out[pix] = make_float4(acc.x, acc.y, acc.z, 1.0f);
+                        ++pixels_done;
+                    }
                     need_slot = true;
                 }
             }
@@ -855,6 +891,12 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
 // it.  Internal linkage: every translation unit that launches it has its own.
 static __global__ void zero_counter_kernel(uint32_t *c) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0u;
+}
+
+// Clears n words (the sample-run counters of a frame) with a kernel.
+static __global__ void zero_words_kernel(uint32_t *p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
 }
 
 }  // namespace fpt

@@ -15,6 +15,11 @@ hipError_t launch_zero_counter(uint32_t *counter, hipStream_t stream) {
 
 uint32_t persistent_slots(uint32_t width, uint32_t height) { return slot_count(width, height); }
 
+hipError_t launch_zero_words(uint32_t *p, uint32_t n, hipStream_t stream) {
+    if (n) hipLaunchKernelGGL(zero_words_kernel, dim3((n + 255u) / 256u), dim3(256), 0, stream, p, n);
+    return hipGetLastError();
+}
+
 bool single_bulb_program(const FptRenderConfig &c) {
     return FPT_INTERLEAVED_BULB && c.sdf_id == FPT_SDF_PROGRAM && c.sdf_program_count == 1u &&
            c.sdf_program[0].opcode == FPT_SDF_OP_MANDELBULB && !has_lowered_tables(c);
@@ -45,25 +50,25 @@ template <bool PROFILE, bool MULTI, bool LEARNED, bool BULB, bool MB, bool LOWER
 void launch_instance(dim3 grid, hipStream_t stream, const FptRenderConfig *cfgs, const BatchScene *scenes,
                      uint32_t nscenes, float4 *accum, uint32_t start, uint32_t count, uint32_t *counter,
                      uint32_t slots, int event_batch, int refill, DeviceProfile *prof, const ProposerDevice &prop,
-                     const uint32_t *order) {
+                     const uint32_t *order, const SampleSplit &split) {
     hipLaunchKernelGGL((sdf_persistent_kernel<PROFILE, MULTI, LEARNED, BULB, MB, LOWERED>), grid, dim3(256), 0,
                        stream, cfgs, scenes, nscenes, accum, start, count, counter, slots, event_batch, refill,
-                       PROFILE ? prof : nullptr, prop, order);
+                       PROFILE ? prof : nullptr, prop, order, split);
 }
 
 template <bool MULTI, bool LEARNED, bool BULB, bool MB, bool LOWERED>
 void launch_profiled(bool profiled, dim3 grid, hipStream_t stream, const FptRenderConfig *cfgs,
                      const BatchScene *scenes, uint32_t nscenes, float4 *accum, uint32_t start, uint32_t count,
                      uint32_t *counter, uint32_t slots, int event_batch, int refill, DeviceProfile *prof,
-                     const ProposerDevice &prop, const uint32_t *order) {
+                     const ProposerDevice &prop, const uint32_t *order, const SampleSplit &split) {
     if (profiled)
         launch_instance<true, MULTI, LEARNED, BULB, MB, LOWERED>(grid, stream, cfgs, scenes, nscenes, accum, start,
                                                                   count, counter, slots, event_batch, refill, prof,
-                                                                  prop, order);
+                                                                  prop, order, split);
     else
         launch_instance<false, MULTI, LEARNED, BULB, MB, LOWERED>(grid, stream, cfgs, scenes, nscenes, accum, start,
                                                                    count, counter, slots, event_batch, refill, prof,
-                                                                   prop, order);
+                                                                   prop, order, split);
 }
 
 }  // namespace
@@ -71,8 +76,13 @@ void launch_profiled(bool profiled, dim3 grid, hipStream_t stream, const FptRend
 hipError_t launch_sdf_persistent(const FptRenderConfig *dcfg, const FptRenderConfig &hcfg, float4 *accum,
                                  uint32_t start, uint32_t count, uint32_t *counter, uint32_t workgroups,
                                  int event_batch, int refill, DeviceProfile *prof, const ProposerDevice *prop,
-                                 const uint32_t *tile_order, hipStream_t stream) {
-    const uint32_t slots = slot_count(hcfg.width, hcfg.height);
+                                 const uint32_t *tile_order, const SampleSplit &split, hipStream_t stream) {
+    uint32_t slots = slot_count(hcfg.width, hcfg.height);
+    if (split.groups > 1u) {
+        slots *= split.groups;
+        const uint32_t npix = hcfg.width * hcfg.height;
+        hipLaunchKernelGGL(zero_words_kernel, dim3((npix + 255u) / 256u), dim3(256), 0, stream, split.done, npix);
+    }
     if (event_batch <= 0) event_batch = default_event_batch(persistent_uses_stash(hcfg, prop != nullptr), false);
     hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, counter);
     const ProposerDevice none{};
@@ -80,7 +90,8 @@ hipError_t launch_sdf_persistent(const FptRenderConfig *dcfg, const FptRenderCon
     const bool p = prof != nullptr, lowered = has_lowered_tables(hcfg);
 #define FPT_LAUNCH(LEARNED, BULB, MB, LOWERED, PROP)                                                              \
     launch_profiled<false, LEARNED, BULB, MB, LOWERED>(p, grid, stream, dcfg, nullptr, 1u, accum, start, count,  \
-                                                       counter, slots, event_batch, refill, prof, PROP, tile_order)
+                                                       counter, slots, event_batch, refill, prof, PROP, tile_order,  \
+                                                       split)
     if (prop && lowered) FPT_LAUNCH(true, false, false, true, *prop);
     else if (prop) FPT_LAUNCH(true, false, false, false, *prop);
     else if (hcfg.sdf_id == FPT_SDF_MANDELBULBER) FPT_LAUNCH(false, false, true, false, none);
@@ -104,11 +115,11 @@ hipError_t launch_sdf_batch(const FptRenderConfig *dcfgs, const BatchScene *dsce
     if (lowered)
         launch_profiled<true, false, false, false, true>(p, dim3(workgroups), stream, dcfgs, dscenes, nscenes, accum,
                                                          0u, spp_cap, counter, total_slots, event_batch, kDefaultRefill, prof, none,
-                                                         tile_order);
+                                                         tile_order, SampleSplit{});
     else
         launch_profiled<true, false, false, false, false>(p, dim3(workgroups), stream, dcfgs, dscenes, nscenes, accum,
                                                           0u, spp_cap, counter, total_slots, event_batch, kDefaultRefill, prof, none,
-                                                          tile_order);
+                                                          tile_order, SampleSplit{});
     return hipGetLastError();
 }
 

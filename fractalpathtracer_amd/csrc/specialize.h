@@ -10,6 +10,8 @@
 
 namespace fpt {
 
+struct SampleSplit;
+
 struct SpecializedModule {
     hipModule_t module = nullptr;
     hipFunction_t kernel = nullptr;    // sdf_persistent_kernel<false, false, false>
@@ -39,6 +41,6 @@ hipError_t sample_specialized(const SpecializedModule &m, const FptRenderConfig 
                               float4 *out, hipStream_t stream);
 hipError_t launch_specialized(const SpecializedModule &m, const FptRenderConfig *dcfg, float4 *accum, uint32_t start,
                               uint32_t count, uint32_t *counter, uint32_t slots, uint32_t workgroups, int event_batch,
-                              int refill, const uint32_t *tile_order, hipStream_t stream);
+                              int refill, const uint32_t *tile_order, const SampleSplit &split, hipStream_t stream);
 
 }  // namespace fpt

@@ -286,7 +286,7 @@ int specialize(const FptRenderConfig &cfg, std::shared_ptr<SpecializedModule> &o
         }
         if (hipModuleGetFunction(&m.kernel, m.module,
                                  "_ZN3fpt21sdf_persistent_kernelILb0ELb0ELb0ELb0ELb0ELb0EEEvPK15FptRenderConfigPKNS_"
-                                 "10BatchSceneEjP15HIP_vector_typeIfLj4EEjjPjjiiPNS_13DeviceProfileENS_14ProposerDeviceEPKj") !=
+                                 "10BatchSceneEjP15HIP_vector_typeIfLj4EEjjPjjiiPNS_13DeviceProfileENS_14ProposerDeviceEPKjNS_11SampleSplitE") !=
                 hipSuccess ||
             hipModuleGetFunction(&m.validate, m.module, "fpt_topology_validate") != hipSuccess ||
             hipModuleGetFunction(&m.sample, m.module, "fpt_topology_sample") != hipSuccess) {
@@ -369,13 +369,14 @@ hipError_t sample_specialized(const SpecializedModule &m, const FptRenderConfig 
 
 hipError_t launch_specialized(const SpecializedModule &m, const FptRenderConfig *dcfg, float4 *accum, uint32_t start,
                               uint32_t count, uint32_t *counter, uint32_t slots, uint32_t workgroups, int event_batch,
-                              int refill, const uint32_t *tile_order, hipStream_t stream) {
+                              int refill, const uint32_t *tile_order, const SampleSplit &split, hipStream_t stream) {
     const BatchScene *scenes = nullptr;
     uint32_t nscenes = 1u;
     DeviceProfile *prof = nullptr;
     ProposerDevice none{};
     void *args[] = {const_cast<FptRenderConfig **>(&dcfg), &scenes, &nscenes, &accum, &start, &count, &counter,
-                    &slots, &event_batch, &refill, &prof, &none, const_cast<uint32_t **>(&tile_order)};
+                    &slots, &event_batch, &refill, &prof, &none, const_cast<uint32_t **>(&tile_order),
+                    const_cast<SampleSplit *>(&split)};
     return hipModuleLaunchKernel(m.kernel, workgroups, 1, 1, 256, 1, 1, 0, stream, args, nullptr);
 }
 

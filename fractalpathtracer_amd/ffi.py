@@ -371,7 +371,7 @@ def _declare(lib):
     lib.fpt_hip_program_surface.restype = C.c_int
     lib.fpt_hip_plan_tune.argtypes = [C.c_void_p, u32, u32, u32, cp, C.c_size_t]
     lib.fpt_hip_plan_tune.restype = C.c_int
-    lib.fpt_hip_plan_tune_stash.argtypes = [C.c_void_p, u32, cp, C.c_size_t]
+    lib.fpt_hip_plan_tune_stash.argtypes = [C.c_void_p, u32, u32, cp, C.c_size_t]
     lib.fpt_hip_plan_tune_stash.restype = C.c_int
     lib.fpt_hip_batch_create.argtypes = [P(FptRenderConfig), u32, P(C.c_void_p), cp, C.c_size_t]
     lib.fpt_hip_batch_create.restype = C.c_int

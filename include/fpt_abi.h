@@ -789,14 +789,23 @@ int fpt_hip_plan_tune(FptHipPlan *plan,
                       size_t error_len);
 
 /*
- * Path stash of the persistent kernel's single-scene exact-march instances
- * (no reference counterpart: a launch-shape knob of this library).  Each
- * wave parks up to 64 further paths in LDS; lanes whose path waits for the
- * per-hit work, or that are idle, exchange it for a parked marching path
- * once `refill` lanes of the wave are not marching (1..64, default 4).
- * Images do not depend on it.
+ * Path stash and sample runs of the persistent kernel's single-scene
+ * exact-march instances (no reference counterpart: launch-shape knobs of
+ * this library; images do not depend on them).
+ *   refill         each wave parks up to 64 further paths in LDS; lanes whose
+ *                  path waits for the per-hit work, or that are idle, exchange
+ *                  it for a parked marching path once `refill` lanes of the
+ *                  wave are not marching (1..64, default 4);
+ *   sample_groups  each pixel's samples of a launch are split into at most
+ *                  this many runs of at least 16 samples (1..64, default 4),
+ *                  separate work slots that different lanes render at once;
+ *                  the run that finishes last folds the per-sample colours
+ *                  in sample order.  1 = one slot per pixel.  The plan keeps
+ *                  the per-sample colours on the device (16 B per pixel and
+ *                  sample; not used beyond 8 GB).
  */
-int fpt_hip_plan_tune_stash(FptHipPlan *plan, uint32_t refill, char *error, size_t error_len);
+int fpt_hip_plan_tune_stash(FptHipPlan *plan, uint32_t refill, uint32_t sample_groups, char *error,
+                            size_t error_len);
 
 /*
  * Voxel tables of `config` (renderer_backend ignored; voxel_* fields used),

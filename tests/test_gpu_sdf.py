@@ -216,26 +216,35 @@ def test_single_mandelbulb_leaf_matches_the_generic_program_march(power, iterati
         assert a.profile[key] == b.profile[key], key
 
 
-@pytest.mark.parametrize("scene", ["bulb", "typed"])
-def test_path_stash_settings_do_not_change_images(scene):
+@pytest.mark.parametrize("scene,spp,accumulation", [("bulb", 35, ffi.SDF_ACCUMULATION_BATCH),
+                                                    ("typed", 33, ffi.SDF_ACCUMULATION_BATCH),
+                                                    ("bulb", 37, ffi.SDF_ACCUMULATION_CHUNKED)])
+def test_path_stash_and_sample_runs_do_not_change_images(scene, spp, accumulation):
     """The path stash's knobs (refill threshold, event batch counted over a
-    wave's lanes and parked paths) change only which lane runs which path:
-    every setting renders the same frame as the one-path kernel, and refill
-    thresholds outside 1..64 are refused."""
+    wave's lanes and parked paths) and the sample runs (a pixel's samples
+    split into runs rendered by different lanes, the last run folding the
+    per-sample colours in order) change only which lane runs which path:
+    every setting renders the same frame as the one-path kernel, with
+    chunked launches that resume the running mean from the buffer too, and
+    out-of-range settings are refused."""
     if scene == "bulb":
-        cfg = scenes.mandelbulb_scene(72, 56, 3, bounces=2)
+        cfg = scenes.mandelbulb_scene(40, 32, spp, bounces=2)
     else:
-        cfg = scenes.typed_program_scene(64, 48, 2)
+        cfg = scenes.typed_program_scene(40, 24, spp)
+    cfg.sdf_accumulation_mode = accumulation
+    cfg.sdf_chunk_samples = 33  # chunked: a launch of 33 samples (two runs), then one of 4 (no split)
     buf = api.DeviceBuffer(cfg.width * cfg.height * 4)
     with api.Plan(cfg) as plan:
         plan.tune(kernel=0)
         plan.render_into(buf.address)
         ref = buf.numpy().copy()
-        for refill, eb, wg in ((1, 1, 0), (64, 64, 0), (4, 0, 3), (17, 33, 1)):
+        for refill, runs, eb, wg in ((1, 1, 1, 0), (64, 64, 64, 0), (4, 4, 0, 3), (17, 3, 33, 1), (4, 2, 0, 0)):
             plan.tune(kernel=1, workgroups=wg, event_batch=eb)
-            plan.tune_stash(refill)
+            plan.tune_stash(refill, runs)
             plan.render_into(buf.address)
-            assert np.array_equal(buf.numpy(), ref), (refill, eb, wg)
+            assert np.array_equal(buf.numpy(), ref), (refill, runs, eb, wg)
         for bad in (0, 65):
             with pytest.raises(ffi.FptError, match="refill"):
-                plan.tune_stash(bad)
+                plan.tune_stash(bad, 4)
+            with pytest.raises(ffi.FptError, match="sample run"):
+                plan.tune_stash(4, bad)
