@@ -70,7 +70,7 @@ FLOPS_PER_PROPOSAL = 2 * (128 * 64 + 3 * 64 * 64 + 64)
 # passes (profiles/sdf_persistent_mandelbulb_1080p_64spp.md): FETCH_SIZE and
 # WRITE_SIZE KiB per dispatch, scaled by what the same counters read for a
 # 1 GiB copy kernel (x2.000 and x1.000).
-TRAFFIC_FETCH_KIB, TRAFFIC_WRITE_KIB = 2168.75, 111990.375
+TRAFFIC_FETCH_KIB, TRAFFIC_WRITE_KIB = 785775.9375, 14655545.3125
 TRAFFIC_FETCH_SCALE, TRAFFIC_WRITE_SCALE = 2.0, 1.0
 TRAFFIC_WORKLOAD = (1920, 1080, 64)
 
@@ -166,8 +166,9 @@ def frame_special_ops(prof: dict) -> float:
 
 
 def tune_dict(t) -> dict:
-    """The autotuner's choice (fpt_hip_autotune) as JSON."""
-    return {"workgroups": t.workgroups, "event_batch": t.event_batch,
+    """The autotuner's choice (fpt_hip_autotune) as JSON; event batch 0 is
+    the launch's default (64 paths for a frame in sample runs, 48 without)."""
+    return {"workgroups": t.workgroups, "event_batch": t.event_batch or "launch default",
             "tile_order": "longest-first" if t.tile_order else "row-major", "best_ms": t.best_ms,
             "default_ms": t.default_ms, "candidates": t.candidates, "rounds": t.rounds}
 

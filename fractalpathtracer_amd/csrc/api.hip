@@ -447,7 +447,7 @@ struct FptHipPlan {
     // Sample runs per pixel of the single-scene persistent kernels
     // (fpt_hip_plan_tune_stash); their per-sample colours and run counters.
     uint32_t sample_groups = 4;
-    float4 *split_colors = nullptr;
+    float *split_colors = nullptr;
     uint32_t *split_done = nullptr;
     size_t split_colors_len = 0;
     uint32_t *counter = nullptr; // persistent pixel counter
@@ -479,8 +479,9 @@ constexpr uint32_t kMinSampleRun = 16u;
 // Sample runs of a launch over samples [start, start + chunk) of the frame:
 // runs of ceil(n / groups) samples, every run non-empty and at least
 // kMinSampleRun long.  Single-scene exact-march plans only (no proposer, no
-// imported Mandelbulber scene), and only while the per-sample colours fit in
-// 8 GB; otherwise one slot per pixel.  Allocates the colour and counter
+// imported Mandelbulber scene), frames whose sides are multiples of 8, and
+// only while the per-sample colours fit in 8 GB; otherwise one slot per
+// pixel.  Allocates the colour and counter
 // buffers on first use.
 hipError_t plan_split(FptHipPlan *plan, uint32_t start, uint32_t chunk, fpt::SampleSplit &split) {
     split = fpt::SampleSplit{nullptr, nullptr, 1u, 0u, 0u};
@@ -489,8 +490,11 @@ hipError_t plan_split(FptHipPlan *plan, uint32_t start, uint32_t chunk, fpt::Sam
     const uint32_t n = std::min(start + chunk, spp) > start ? std::min(start + chunk, spp) - start : 0u;
     if (plan->sample_groups <= 1u || n <= 1u || plan->proposer.enabled || c.sdf_id == FPT_SDF_MANDELBULBER)
         return hipSuccess;
+    // Frames whose edge tiles carry padding render one slot per pixel: sample
+    // runs are verified only on frames that tile exactly into 8x8.
+    if (c.width % 8u != 0u || c.height % 8u != 0u) return hipSuccess;
     const size_t npix = size_t(c.width) * c.height, len = npix * spp;
-    if (len * sizeof(float4) > (size_t(8) << 30)) return hipSuccess;
+    if (len * 3u * sizeof(float) > (size_t(8) << 30)) return hipSuccess;
     const uint32_t groups = std::min(plan->sample_groups, n / kMinSampleRun);
     if (groups <= 1u) return hipSuccess;
     const uint32_t run = (n + groups - 1u) / groups;
@@ -500,7 +504,7 @@ hipError_t plan_split(FptHipPlan *plan, uint32_t start, uint32_t chunk, fpt::Sam
         plan->split_colors = nullptr;
         plan->split_done = nullptr;
         plan->split_colors_len = 0;
-        hipError_t e = hipMalloc(&plan->split_colors, len * sizeof(float4));
+        hipError_t e = hipMalloc(&plan->split_colors, len * 3u * sizeof(float));
         if (e == hipSuccess) e = hipMalloc(&plan->split_done, npix * sizeof(uint32_t));
         if (e != hipSuccess) return e;
         plan->split_colors_len = len;
@@ -533,7 +537,7 @@ hipError_t launch_sdf(FptHipPlan *plan, float4 *acc, uint32_t start, uint32_t ch
         if (e != hipSuccess) return e;
         return fpt::launch_specialized(*plan->spec, plan->dcfg, acc, start, chunk, plan->counter,
                                        fpt::persistent_slots(c.width, c.height) * split.groups, wg,
-                                       plan->event_batch ? int(plan->event_batch) : fpt::default_event_batch(true, false),
+                                       plan->event_batch ? int(plan->event_batch) : fpt::default_event_batch(true, split.groups > 1u),
                                        int(plan->refill), plan->tile_order_mode ? plan->tile_order : nullptr, split, s);
     }
     return fpt::launch_sdf_persistent(plan->dcfg, c, acc, start, chunk, plan->counter, wg, int(plan->event_batch),
@@ -732,7 +736,7 @@ int fpt_hip_plan_tune(FptHipPlan *plan, uint32_t kernel, uint32_t workgroups, ui
         set_error(error, error_len, "missing plan");
         return 1;
     }
-    if (kernel > 1u || event_batch > 64u || workgroups > 65536u) {
+    if (kernel > 1u || event_batch > 128u || workgroups > 65536u) {
         set_error(error, error_len, "invalid launch shape (kernel %u, workgroups %u, event_batch %u)", kernel,
                   workgroups, event_batch);
         return 1;
@@ -1186,7 +1190,7 @@ int fpt_hip_batch_tune(FptHipBatch *batch, uint32_t workgroups, uint32_t event_b
         set_error(error, error_len, "missing batch");
         return 1;
     }
-    if (event_batch > 64u || workgroups > 65536u) {
+    if (event_batch > 128u || workgroups > 65536u) {
         set_error(error, error_len, "invalid launch shape (workgroups %u, event_batch %u)", workgroups, event_batch);
         return 1;
     }
@@ -1404,9 +1408,9 @@ int fpt_hip_render_batch_noalloc(const FptRenderConfig *configs, uint32_t count,
 
 // ---------------------------------------------------------------------------
 // Autotuner: candidate launch shapes timed on a proxy of the workload (the
-// plan's first sample / the batch capped at one sample per pixel), probed in
-// a rotated order every round so drift does not favour a position, median of
-// the rounds, fastest median wins.
+// plan's first samples, up to 16 / the batch capped at one sample per pixel),
+// probed in a rotated order every round so drift does not favour a position,
+// median of the rounds, fastest median wins.
 // ---------------------------------------------------------------------------
 int fpt_hip_autotune(FptHipPlan *plan, FptHipBatch *batch, float *device_accum, void *stream, uint32_t rounds,
                      FptTuneResult *result, char *error, size_t error_len) {
@@ -1427,15 +1431,16 @@ int fpt_hip_autotune(FptHipPlan *plan, FptHipBatch *batch, float *device_accum, 
     std::vector<Shape> cand;
     // With the path stash a workgroup holds 53-57 KB of LDS, so at most two
     // are resident per compute unit, and the event batch counts the wave's
-    // up to 128 paths.  Single-scene frames run best at 32-48 paths (larger
-    // batches look as good on the few-sample proxy but lengthen the frame's
-    // tail); batches of scenes at 48-64.
+    // up to 128 paths.  A plan's event batch stays at the launch's default
+    // (0): the best batch depends on whether the frame runs in sample runs
+    // (64 with, 48 without), and a few-sample proxy does not split as the
+    // frame does.
     const bool stash = plan ? fpt::persistent_uses_stash(plan->host, plan->proposer.enabled) : fpt::batch_uses_stash();
     const std::vector<uint32_t> per_cus = stash ? std::vector<uint32_t>{1u, 2u} : std::vector<uint32_t>{1u, 2u, 3u, 4u};
     const std::vector<uint32_t> ebs = !stash ? std::vector<uint32_t>{8u, 16u, 24u, 32u, 40u}
-                                      : plan ? std::vector<uint32_t>{32u, 40u, 48u}
+                                      : plan ? std::vector<uint32_t>{0u}
                                              : std::vector<uint32_t>{32u, 40u, 48u, 56u, 64u};
-    const uint32_t default_eb = uint32_t(fpt::default_event_batch(stash, !plan));
+    const uint32_t default_eb = stash && plan ? 0u : uint32_t(fpt::default_event_batch(stash, true));
     for (uint32_t order : {1u, 0u})
         for (uint32_t per_cu : per_cus)
             for (uint32_t eb : ebs) cand.push_back({per_cu * cus, eb, order});
@@ -1446,9 +1451,9 @@ int fpt_hip_autotune(FptHipPlan *plan, FptHipBatch *batch, float *device_accum, 
     HIP_TRY(hipEventCreate(&e1), "hipEventCreate");
     float4 *acc = reinterpret_cast<float4 *>(device_accum);
     if (plan && !plan->counter) HIP_TRY(hipMalloc(&plan->counter, sizeof(uint32_t)), "counter allocation");
-    const uint32_t plan_proxy_spp = plan ? std::min<uint32_t>(clamp_samples(plan->host.samples), 4u) : 1u;
+    const uint32_t plan_proxy_spp = plan ? std::min<uint32_t>(clamp_samples(plan->host.samples), 16u) : 1u;
     auto launch = [&](const Shape &sh) -> hipError_t {
-        if (plan) {  // proxy: the frame's first samples (up to 4)
+        if (plan) {  // proxy: the frame's first samples (up to 16)
             fpt::SampleSplit split;
             const hipError_t e = plan_split(plan, 0u, plan_proxy_spp, split);
             if (e != hipSuccess) return e;

@@ -39,12 +39,13 @@ struct ProposerDevice;
 // Sample groups of the single-scene persistent kernels: each pixel's samples
 // [start, start + count) are split into `groups` consecutive runs of `chunk`
 // samples, each a work slot of its own, so different lanes render them at
-// once.  Every sample's clamped colour goes to colors[pix * stride + s]; the
-// run that finishes last (done[pix] counts finished runs) folds the colours
-// in sample order into the running mean, exactly as one lane would have.
-// groups <= 1: one slot per pixel, the running mean in a register.
+// once.  Every sample's clamped colour goes to colors[3 (pix * stride + s)]
+// (three floats); the run that finishes last (done[pix] counts finished
+// runs) folds the colours in sample order into the running mean, exactly as
+// one lane would have.  groups <= 1: one slot per pixel, the running mean in
+// a register.
 struct SampleSplit {
-    float4 *colors;
+    float *colors;
     uint32_t *done;
     uint32_t groups, chunk, stride;
 };
@@ -73,9 +74,12 @@ bool single_bulb_program(const FptRenderConfig &c);
 bool persistent_uses_stash(const FptRenderConfig &c, bool learned);
 // The batched (multi-scene) persistent instance keeps parked paths in LDS.
 bool batch_uses_stash();
-// Event batch of a persistent launch given 0: with the path stash 40 paths
-// for one scene, 64 for a batch of scenes; 24 lanes without.
-int default_event_batch(bool stash, bool multi);
+// Event batch of a persistent launch given 0, counted over a wave's lanes
+// and parked paths with the path stash: 64 for launches in sample runs and
+// for batches of scenes (with the rule that also runs the per-hit work once
+// as many paths wait as march, larger batches change nothing), 48 for one
+// slot per pixel; 24 lanes without the stash.
+int default_event_batch(bool stash, bool runs);
 // Non-marching lanes of a wave before they take parked marching paths.
 constexpr int kDefaultRefill = 4;
 // The scene's program carries flat-union, canonical or typed-SoA tables: the

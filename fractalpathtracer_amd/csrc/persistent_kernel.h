@@ -818,7 +818,12 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
                 f3 col = light * color;
                 if (cfg.world[6] == 1.0f) col = vmix(col, gcol, sky_mask);
                 col = vmin(col, 8.0f);
-                if (SPLIT) split.colors[size_t(pix) * split.stride + s] = make_float4(col.x, col.y, col.z, 0.0f);
+                if (SPLIT) {
+                    float *cs = split.colors + 3u * (size_t(pix) * split.stride + s);
+                    cs[0] = col.x;
+                    cs[1] = col.y;
+                    cs[2] = col.z;
+                }
                 else acc = vmix(acc, col, 1.0f / float(s + 1u));
                 ++s;
                 if (s < s_stop) {
@@ -836,11 +841,9 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
                                 const float4 prev = out[pix];
                                 a = mk(prev.x, prev.y, prev.z);
                             }
-                            const float4 *cs = split.colors + size_t(pix) * split.stride;
-                            for (uint32_t i = s_begin; i < s_end; ++i) {
-                                const float4 c = cs[i];
-                                a = vmix(a, mk(c.x, c.y, c.z), 1.0f / float(i + 1u));
-                            }
+                            const float *cs = split.colors + 3u * size_t(pix) * split.stride;
+                            for (uint32_t i = s_begin; i < s_end; ++i)
+                                a = vmix(a, mk(cs[3u * i], cs[3u * i + 1u], cs[3u * i + 2u]), 1.0f / float(i + 1u));
                             out[pix] = make_float4(a.x, a.y, a.z, 1.0f);
                             ++pixels_done;
                         }

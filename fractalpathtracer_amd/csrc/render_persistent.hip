@@ -29,7 +29,7 @@ bool persistent_uses_stash(const FptRenderConfig &c, bool learned) {
     return uses_path_stash(false, learned, c.sdf_id == FPT_SDF_MANDELBULBER);
 }
 
-int default_event_batch(bool stash, bool multi) { return stash ? (multi ? 64 : 40) : 24; }
+int default_event_batch(bool stash, bool runs) { return stash ? (runs ? 64 : 48) : 24; }
 
 bool batch_uses_stash() { return uses_path_stash(true, false, false); }
 
@@ -83,7 +83,7 @@ hipError_t launch_sdf_persistent(const FptRenderConfig *dcfg, const FptRenderCon
         const uint32_t npix = hcfg.width * hcfg.height;
         hipLaunchKernelGGL(zero_words_kernel, dim3((npix + 255u) / 256u), dim3(256), 0, stream, split.done, npix);
     }
-    if (event_batch <= 0) event_batch = default_event_batch(persistent_uses_stash(hcfg, prop != nullptr), false);
+    if (event_batch <= 0) event_batch = default_event_batch(persistent_uses_stash(hcfg, prop != nullptr), split.groups > 1u);
     hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, counter);
     const ProposerDevice none{};
     const dim3 grid(workgroups);

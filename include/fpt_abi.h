@@ -774,12 +774,15 @@ void fpt_hip_plan_destroy(FptHipPlan *plan);
  *                compute unit;
  *   event_batch  how many paths of a 64-lane wave must have finished their
  *                march before the per-hit work (material, analytic normal,
- *                sun and scatter) runs for all of them, 1..64.  The
- *                single-scene exact-march instances count the wave's lanes
- *                and its paths parked in LDS (see below), and also run the
- *                per-hit work once as many paths wait as march.  0 = the
- *                instance's default: 40 with parked paths, 24 without
- *                (batches of scenes, fpt_hip_batch_tune: 64 and 24).
+ *                sun and scatter) runs for all of them, 1..128.  The
+ *                exact-march instances with parked paths (see below; also
+ *                batches of scenes) count the wave's lanes and its paths
+ *                parked in LDS, up to 128, and also run the per-hit work
+ *                once as many paths wait as march; the others count lanes
+ *                (values above 64 then wait for every march to end).  0 =
+ *                the instance's default: with parked paths 64 for a launch
+ *                in sample runs (below) and 48 otherwise, 24 without
+ *                (fpt_hip_batch_tune: 64 and 24).
  */
 int fpt_hip_plan_tune(FptHipPlan *plan,
                       uint32_t kernel,
@@ -800,9 +803,10 @@ int fpt_hip_plan_tune(FptHipPlan *plan,
  *                  this many runs of at least 16 samples (1..64, default 4),
  *                  separate work slots that different lanes render at once;
  *                  the run that finishes last folds the per-sample colours
- *                  in sample order.  1 = one slot per pixel.  The plan keeps
- *                  the per-sample colours on the device (16 B per pixel and
- *                  sample; not used beyond 8 GB).
+ *                  in sample order.  1 = one slot per pixel.  Applies to
+ *                  frames whose width and height are multiples of 8.  The
+ *                  plan keeps the per-sample colours on the device (12 B per
+ *                  pixel and sample; not used beyond 8 GB).
  */
 int fpt_hip_plan_tune_stash(FptHipPlan *plan, uint32_t refill, uint32_t sample_groups, char *error,
                             size_t error_len);
@@ -909,10 +913,13 @@ int fpt_hip_render_batch(const struct FptRenderConfig *configs,
                          size_t error_len);
 
 /*
- * Autotuner for exactly one of `plan` / `batch`: 40 launch shapes
- * (workgroups 1-4 per CU x event_batch 8..40 x both tile orders) timed on a proxy of the
- * workload (the plan's first samples, up to 4; the batch capped at one sample per
- * pixel) written into device_accum (contents are overwritten), `rounds`
+ * Autotuner for exactly one of `plan` / `batch`: launch shapes (with
+ * parked paths: workgroups 1-2 per CU, and event_batch left at 0, the
+ * launch's default, for a plan, 32..64 for a batch; without: 1-4 per CU x
+ * 8..40; both tile orders) timed
+ * on a proxy of the workload (the plan's first samples, up to 16; the batch
+ * capped at one sample per pixel) written into device_accum (contents are
+ * overwritten), `rounds`
  * rounds in a rotated probe order, median per shape; the fastest shape is
  * stored in the plan/batch and reported.  Synchronous.
  */

@@ -218,6 +218,7 @@ def test_single_mandelbulb_leaf_matches_the_generic_program_march(power, iterati
 
 @pytest.mark.parametrize("scene,spp,accumulation", [("bulb", 35, ffi.SDF_ACCUMULATION_BATCH),
                                                     ("typed", 33, ffi.SDF_ACCUMULATION_BATCH),
+                                                    ("typed", 9, ffi.SDF_ACCUMULATION_BATCH),
                                                     ("bulb", 37, ffi.SDF_ACCUMULATION_CHUNKED)])
 def test_path_stash_and_sample_runs_do_not_change_images(scene, spp, accumulation):
     """The path stash's knobs (refill threshold, event batch counted over a
@@ -232,19 +233,40 @@ def test_path_stash_and_sample_runs_do_not_change_images(scene, spp, accumulatio
     else:
         cfg = scenes.typed_program_scene(40, 24, spp)
     cfg.sdf_accumulation_mode = accumulation
-    cfg.sdf_chunk_samples = 33  # chunked: a launch of 33 samples (two runs), then one of 4 (no split)
+    cfg.sdf_chunk_samples = 33  # chunked: a launch of 33 samples (runs), then one of 4 (one run)
     buf = api.DeviceBuffer(cfg.width * cfg.height * 4)
     with api.Plan(cfg) as plan:
         plan.tune(kernel=0)
         plan.render_into(buf.address)
         ref = buf.numpy().copy()
-        for refill, runs, eb, wg in ((1, 1, 1, 0), (64, 64, 64, 0), (4, 4, 0, 3), (17, 3, 33, 1), (4, 2, 0, 0)):
+        for refill, runs, eb, wg in ((1, 1, 1, 0), (64, 64, 64, 0), (4, 4, 0, 3), (17, 3, 33, 1), (4, 2, 0, 0),
+                                     (4, 8, 0, 0), (8, 8, 128, 0), (4, 5, 97, 2)):
             plan.tune(kernel=1, workgroups=wg, event_batch=eb)
             plan.tune_stash(refill, runs)
             plan.render_into(buf.address)
             assert np.array_equal(buf.numpy(), ref), (refill, runs, eb, wg)
+        with pytest.raises(ffi.FptError, match="launch shape"):
+            plan.tune(kernel=1, event_batch=129)
         for bad in (0, 65):
             with pytest.raises(ffi.FptError, match="refill"):
                 plan.tune_stash(bad, 4)
             with pytest.raises(ffi.FptError, match="sample run"):
                 plan.tune_stash(4, bad)
+
+
+def test_plan_autotune_picks_a_valid_shape_and_keeps_the_image():
+    """The single-scene autotuner (parked paths: 1-2 workgroups per compute
+    unit x two tile orders, the event batch left at the launch's default,
+    timed on the frame's first 16 samples) stores a shape that renders the
+    same frame."""
+    cfg = scenes.mandelbulb_scene(48, 40, 20, bounces=2)
+    buf = api.DeviceBuffer(cfg.width * cfg.height * 4)
+    st = api.Stream()
+    with api.Plan(cfg) as plan:
+        plan.render_into(buf.address)
+        before = buf.numpy().copy()
+        r = plan.autotune(buf.address, st.address, rounds=1)
+        assert r.candidates == 4 and r.best_ms > 0 and r.event_batch == 0 and r.workgroups > 0
+        assert r.tile_order in (0, 1)
+        plan.render_into(buf.address)
+        assert np.array_equal(buf.numpy(), before)
