@@ -819,10 +819,19 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
                 if (cfg.world[6] == 1.0f) col = vmix(col, gcol, sky_mask);
                 col = vmin(col, 8.0f);
                 if (SPLIT) {
-                    float *cs = split.colors + 3u * (size_t(pix) * split.stride + s);
-                    cs[0] = col.x;
-                    cs[1] = col.y;
-                    cs[2] = col.z;
+                    // Agent-scope stores write through the L2, so the fence at
+                    // a run's end finds no dirty colour lines to write back:
+                    // one 8-byte store for the aligned pair, one 4-byte store.
+                    const size_t e = 3u * (size_t(pix) * split.stride + s);
+                    float *cs = split.colors + e;
+                    const bool even = (e & 1u) == 0u;
+                    const float lone = even ? col.z : col.x;
+                    const unsigned long long pair =
+                        even ? (uint64_t(__float_as_uint(col.y)) << 32) | __float_as_uint(col.x)
+                             : (uint64_t(__float_as_uint(col.z)) << 32) | __float_as_uint(col.y);
+                    __hip_atomic_store(reinterpret_cast<unsigned long long *>(even ? cs : cs + 1), pair,
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(even ? cs + 2 : cs, lone, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
                 else acc = vmix(acc, col, 1.0f / float(s + 1u));
                 ++s;
