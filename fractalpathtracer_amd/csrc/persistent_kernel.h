@@ -840,11 +840,13 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
                 } else {
                     if (SPLIT) {
                         // The last run of the pixel to finish folds every
-                        // sample's colour in order (release the colours
-                        // before the count, acquire them after it).
-                        __threadfence();
+                        // sample's colour in order: a release fence before
+                        // the (relaxed, device-scope) count publishes this
+                        // run's colours, an acquire fence after it, in the
+                        // last run only, makes the others' visible.
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
                         if (atomicAdd(&split.done[pix], 1u) + 1u == split.groups) {
-                            __threadfence();
+                            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                             f3 a = mk(0.f, 0.f, 0.f);
                             if (s_begin != 0u) {
                                 const float4 prev = out[pix];
