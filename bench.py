@@ -170,7 +170,8 @@ def tune_dict(t) -> dict:
     the launch's default (64 paths for a frame in sample runs, 48 without)."""
     return {"workgroups": t.workgroups, "event_batch": t.event_batch or "launch default",
             "tile_order": "longest-first" if t.tile_order else "row-major", "best_ms": t.best_ms,
-            "default_ms": t.default_ms, "candidates": t.candidates, "rounds": t.rounds}
+            "default_ms": t.default_ms, "candidates": t.candidates, "rounds": t.rounds,
+            **({"sample_runs": t.sample_groups} if t.sample_groups else {})}
 
 
 def cpu_model() -> str:

@@ -471,10 +471,11 @@ uint32_t default_workgroups() {
 
 // Shortest sample run: every finished run publishes its colours with a
 // device-scope release (an L2 write-back on gfx950) and the last one
-// acquires them (an L2 invalidate), so runs must be long enough for that to
-// be rare (a 28-primitive union at 16 spp split into runs of 4 ran 17%
-// slower; the 64-spp Mandelbulb in runs of 16 ran 4.5% faster).
-constexpr uint32_t kMinSampleRun = 16u;
+// acquires them (an L1/L2 invalidate), so runs must be long enough for that
+// to be rare.  With write-through colour stores, runs of 8 sped the 16-spp
+// typed program 8% and slowed the 28-primitive union 4% (the autotuner
+// chooses for such frames); runs of 4 were slower still.
+constexpr uint32_t kMinSampleRun = 8u;
 
 // Sample runs of a launch over samples [start, start + chunk) of the frame:
 // runs of ceil(n / groups) samples, every run non-empty and at least
@@ -1426,7 +1427,7 @@ int fpt_hip_autotune(FptHipPlan *plan, FptHipBatch *batch, float *device_accum, 
     hipStream_t s = static_cast<hipStream_t>(stream);
     const uint32_t cus = default_workgroups() / 2u;
     struct Shape {
-        uint32_t workgroups, event_batch, order;
+        uint32_t workgroups, event_batch, order, groups;
     };
     std::vector<Shape> cand;
     // With the path stash a workgroup holds 53-57 KB of LDS, so at most two
@@ -1441,9 +1442,16 @@ int fpt_hip_autotune(FptHipPlan *plan, FptHipBatch *batch, float *device_accum, 
                                       : plan ? std::vector<uint32_t>{0u}
                                              : std::vector<uint32_t>{32u, 40u, 48u, 56u, 64u};
     const uint32_t default_eb = stash && plan ? 0u : uint32_t(fpt::default_event_batch(stash, true));
-    for (uint32_t order : {1u, 0u})
-        for (uint32_t per_cu : per_cus)
-            for (uint32_t eb : ebs) cand.push_back({per_cu * cus, eb, order});
+    // Sample runs against one slot per pixel, where the proxy is the whole
+    // frame (at most 16 samples per pixel) and so measures the choice.
+    const uint32_t spp = plan ? clamp_samples(plan->host.samples) : 0u;
+    const uint32_t plan_groups = plan ? plan->sample_groups : 0u;
+    std::vector<uint32_t> groups{plan_groups};
+    if (plan && stash && spp <= 16u && plan_groups > 1u) groups.push_back(1u);
+    for (uint32_t g : groups)
+        for (uint32_t order : {1u, 0u})
+            for (uint32_t per_cu : per_cus)
+                for (uint32_t eb : ebs) cand.push_back({per_cu * cus, eb, order, g});
     rounds = std::max<uint32_t>(1u, std::min<uint32_t>(rounds, 16u));
     std::vector<std::vector<float>> times(cand.size());
     hipEvent_t e0, e1;
@@ -1455,6 +1463,7 @@ int fpt_hip_autotune(FptHipPlan *plan, FptHipBatch *batch, float *device_accum, 
     auto launch = [&](const Shape &sh) -> hipError_t {
         if (plan) {  // proxy: the frame's first samples (up to 16)
             fpt::SampleSplit split;
+            plan->sample_groups = sh.groups;
             const hipError_t e = plan_split(plan, 0u, plan_proxy_spp, split);
             if (e != hipSuccess) return e;
             return fpt::launch_sdf_persistent(plan->dcfg, plan->host, acc, 0u, plan_proxy_spp, plan->counter,
@@ -1487,7 +1496,9 @@ int fpt_hip_autotune(FptHipPlan *plan, FptHipBatch *batch, float *device_accum, 
         std::vector<float> t = times[c];
         std::sort(t.begin(), t.end());
         float med = t[t.size() / 2];
-        if (cand[c].workgroups == 2u * cus && cand[c].event_batch == default_eb && cand[c].order == 1u) default_ms = med;
+        if (cand[c].workgroups == 2u * cus && cand[c].event_batch == default_eb && cand[c].order == 1u &&
+            cand[c].groups == plan_groups)
+            default_ms = med;
         if (med < best_ms) {
             best_ms = med;
             best = c;
@@ -1498,6 +1509,7 @@ int fpt_hip_autotune(FptHipPlan *plan, FptHipBatch *batch, float *device_accum, 
         plan->workgroups = cand[best].workgroups;
         plan->event_batch = cand[best].event_batch;
         plan->tile_order_mode = cand[best].order;
+        plan->sample_groups = cand[best].groups;
     } else {
         batch->workgroups = cand[best].workgroups;
         batch->event_batch = cand[best].event_batch;
@@ -1507,7 +1519,7 @@ int fpt_hip_autotune(FptHipPlan *plan, FptHipBatch *batch, float *device_accum, 
         result->workgroups = cand[best].workgroups;
         result->event_batch = cand[best].event_batch;
         result->tile_order = cand[best].order;
-        result->_pad0 = 0u;
+        result->sample_groups = cand[best].groups;
         result->candidates = uint32_t(cand.size());
         result->rounds = rounds;
         result->best_ms = best_ms;

@@ -249,7 +249,7 @@ class FptLaneStats(C.Structure):
 
 class FptTuneResult(C.Structure):
     _fields_ = [("workgroups", u32), ("event_batch", u32), ("candidates", u32), ("rounds", u32),
-                ("best_ms", f32), ("default_ms", f32), ("tile_order", u32), ("_pad0", u32)]
+                ("best_ms", f32), ("default_ms", f32), ("tile_order", u32), ("sample_groups", u32)]
 
 
 class FptProposerSettings(C.Structure):

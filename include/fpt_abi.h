@@ -800,7 +800,7 @@ int fpt_hip_plan_tune(FptHipPlan *plan,
  *                  it for a parked marching path once `refill` lanes of the
  *                  wave are not marching (1..64, default 4);
  *   sample_groups  each pixel's samples of a launch are split into at most
- *                  this many runs of at least 16 samples (1..64, default 4),
+ *                  this many runs of at least 8 samples (1..64, default 4),
  *                  separate work slots that different lanes render at once;
  *                  the run that finishes last folds the per-sample colours
  *                  in sample order.  1 = one slot per pixel.  Applies to
@@ -916,7 +916,9 @@ int fpt_hip_render_batch(const struct FptRenderConfig *configs,
  * Autotuner for exactly one of `plan` / `batch`: launch shapes (with
  * parked paths: workgroups 1-2 per CU, and event_batch left at 0, the
  * launch's default, for a plan, 32..64 for a batch; without: 1-4 per CU x
- * 8..40; both tile orders) timed
+ * 8..40; both tile orders; for a plan whose proxy is the whole frame, at
+ * most 16 samples per pixel, also one slot per pixel against the plan's
+ * sample runs) timed
  * on a proxy of the workload (the plan's first samples, up to 16; the batch
  * capped at one sample per pixel) written into device_accum (contents are
  * overwritten), `rounds`
@@ -929,9 +931,9 @@ struct FptTuneResult {
     uint32_t candidates;
     uint32_t rounds;
     float best_ms;
-    float default_ms;   /* median of the default shape (2 per CU, 24, longest-first tiles) */
+    float default_ms;   /* median of the default shape (2 per CU, default event batch, longest-first tiles) */
     uint32_t tile_order; /* 1 longest-first, 0 row-major (fpt_hip_set_tile_order) */
-    uint32_t _pad0;
+    uint32_t sample_groups; /* plan: sample runs per pixel chosen (fpt_hip_plan_tune_stash); batch: 0 */
 };
 FPT_ABI_ASSERT(sizeof(struct FptTuneResult) == 32, "FptTuneResult");
 
@@ -946,11 +948,12 @@ int fpt_hip_autotune(FptHipPlan *plan,
 
 /*
  * Order in which a plan's or batch's persistent launch hands out its 8x8
- * tiles (exactly one of plan / batch): 1 (default) longest-first, from a
- * probe of four primary rays per tile at creation (march steps + 256 per
- * surface hit), so cheap tiles form the frame's tail and lanes finish
- * together; 0 row-major, scene after scene, which keeps neighbouring tiles
- * together.  Images do not depend on it.  The autotuner tries both.
+ * tiles (exactly one of plan / batch): 1 (default) longest-first, from one
+ * whole path sample per pixel rendered at creation (distance evaluations +
+ * orbit iterations, summed per tile), so cheap tiles form the frame's tail
+ * and lanes finish together; 0 row-major, scene after scene, which keeps
+ * neighbouring tiles together.  Images do not depend on it.  The autotuner
+ * tries both.
  */
 int fpt_hip_set_tile_order(FptHipPlan *plan, FptHipBatch *batch, uint32_t mode, char *error, size_t error_len);
 

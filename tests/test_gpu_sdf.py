@@ -254,19 +254,21 @@ def test_path_stash_and_sample_runs_do_not_change_images(scene, spp, accumulatio
                 plan.tune_stash(4, bad)
 
 
-def test_plan_autotune_picks_a_valid_shape_and_keeps_the_image():
+@pytest.mark.parametrize("spp,candidates", [(20, 4), (16, 8)])
+def test_plan_autotune_picks_a_valid_shape_and_keeps_the_image(spp, candidates):
     """The single-scene autotuner (parked paths: 1-2 workgroups per compute
     unit x two tile orders, the event batch left at the launch's default,
-    timed on the frame's first 16 samples) stores a shape that renders the
-    same frame."""
-    cfg = scenes.mandelbulb_scene(48, 40, 20, bounces=2)
+    timed on the frame's first 16 samples; where those are the whole frame,
+    also one slot per pixel against the sample runs) stores a shape that
+    renders the same frame."""
+    cfg = scenes.mandelbulb_scene(48, 40, spp, bounces=2)
     buf = api.DeviceBuffer(cfg.width * cfg.height * 4)
     st = api.Stream()
     with api.Plan(cfg) as plan:
         plan.render_into(buf.address)
         before = buf.numpy().copy()
         r = plan.autotune(buf.address, st.address, rounds=1)
-        assert r.candidates == 4 and r.best_ms > 0 and r.event_batch == 0 and r.workgroups > 0
-        assert r.tile_order in (0, 1)
+        assert r.candidates == candidates and r.best_ms > 0 and r.event_batch == 0 and r.workgroups > 0
+        assert r.tile_order in (0, 1) and r.sample_groups in ((4,) if candidates == 4 else (1, 4))
         plan.render_into(buf.address)
         assert np.array_equal(buf.numpy(), before)
