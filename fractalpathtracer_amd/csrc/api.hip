@@ -1446,6 +1446,15 @@ int fpt_hip_autotune(FptHipPlan *plan, FptHipBatch *batch, float *device_accum, 
     // frame (at most 16 samples per pixel) and so measures the choice.
     const uint32_t spp = plan ? clamp_samples(plan->host.samples) : 0u;
     const uint32_t plan_groups = plan ? plan->sample_groups : 0u;
+    // Candidates set the plan's run count while they are timed; an early
+    // return (a failed launch) puts the caller's value back.
+    struct GroupsGuard {
+        FptHipPlan *p;
+        uint32_t g;
+        ~GroupsGuard() {
+            if (p) p->sample_groups = g;
+        }
+    } groups_guard{plan, plan_groups};
     std::vector<uint32_t> groups{plan_groups};
     if (plan && stash && spp <= 16u && plan_groups > 1u) groups.push_back(1u);
     for (uint32_t g : groups)
@@ -1509,7 +1518,7 @@ int fpt_hip_autotune(FptHipPlan *plan, FptHipBatch *batch, float *device_accum, 
         plan->workgroups = cand[best].workgroups;
         plan->event_batch = cand[best].event_batch;
         plan->tile_order_mode = cand[best].order;
-        plan->sample_groups = cand[best].groups;
+        groups_guard.g = cand[best].groups;
     } else {
         batch->workgroups = cand[best].workgroups;
         batch->event_batch = cand[best].event_batch;
