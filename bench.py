@@ -70,7 +70,7 @@ FLOPS_PER_PROPOSAL = 2 * (128 * 64 + 3 * 64 * 64 + 64)
 # passes (profiles/sdf_persistent_mandelbulb_1080p_64spp.md): FETCH_SIZE and
 # WRITE_SIZE KiB per dispatch, scaled by what the same counters read for a
 # 1 GiB copy kernel (x2.000 and x1.000).
-TRAFFIC_FETCH_KIB, TRAFFIC_WRITE_KIB = 785578.4375, 17771826.4375
+TRAFFIC_FETCH_KIB, TRAFFIC_WRITE_KIB = 784982.625, 18028032.65625
 TRAFFIC_FETCH_SCALE, TRAFFIC_WRITE_SCALE = 2.0, 1.0
 TRAFFIC_WORKLOAD = (1920, 1080, 64)
 

@@ -397,7 +397,7 @@ class Plan:
         err = ffi.error_buffer()
         ffi.check(self.lib.fpt_hip_plan_tune(self.handle, kernel, workgroups, event_batch, err, len(err)), err)
 
-    def tune_stash(self, refill: int = 4, sample_groups: int = 4) -> None:
+    def tune_stash(self, refill: int = 6, sample_groups: int = 8) -> None:
         """Non-marching lanes before a wave refills them from its LDS path
         stash, and sample runs per pixel (fpt_hip_plan_tune_stash)."""
         err = ffi.error_buffer()

@@ -446,7 +446,7 @@ struct FptHipPlan {
     uint32_t refill = fpt::kDefaultRefill;
     // Sample runs per pixel of the single-scene persistent kernels
     // (fpt_hip_plan_tune_stash); their per-sample colours and run counters.
-    uint32_t sample_groups = 4;
+    uint32_t sample_groups = 8;
     float *split_colors = nullptr;
     uint32_t *split_done = nullptr;
     size_t split_colors_len = 0;

@@ -80,8 +80,12 @@ bool batch_uses_stash();
 // as many paths wait as march, larger batches change nothing), 48 for one
 // slot per pixel; 24 lanes without the stash.
 int default_event_batch(bool stash, bool runs);
-// Non-marching lanes of a wave before they take parked marching paths.
-constexpr int kDefaultRefill = 4;
+// Non-marching lanes of a wave before they take parked marching paths: 6
+// for single-scene plans (the 1080p Mandelbulb in 8 sample runs: 551 ms at
+// 4 runs and refill 4, 549 at 8 runs, 546 at 8 runs and refill 6), 4 for
+// batches of scenes.
+constexpr int kDefaultRefill = 6;
+constexpr int kBatchRefill = 4;
 // The scene's program carries flat-union, canonical or typed-SoA tables: the
 // kernels' instances with the table evaluators apply.
 bool has_lowered_tables(const FptRenderConfig &c);

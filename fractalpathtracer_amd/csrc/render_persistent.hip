@@ -114,11 +114,11 @@ hipError_t launch_sdf_batch(const FptRenderConfig *dcfgs, const BatchScene *dsce
     if (event_batch <= 0) event_batch = default_event_batch(uses_path_stash(true, false, false), true);
     if (lowered)
         launch_profiled<true, false, false, false, true>(p, dim3(workgroups), stream, dcfgs, dscenes, nscenes, accum,
-                                                         0u, spp_cap, counter, total_slots, event_batch, kDefaultRefill, prof, none,
+                                                         0u, spp_cap, counter, total_slots, event_batch, kBatchRefill, prof, none,
                                                          tile_order, SampleSplit{});
     else
         launch_profiled<true, false, false, false, false>(p, dim3(workgroups), stream, dcfgs, dscenes, nscenes, accum,
-                                                          0u, spp_cap, counter, total_slots, event_batch, kDefaultRefill, prof, none,
+                                                          0u, spp_cap, counter, total_slots, event_batch, kBatchRefill, prof, none,
                                                           tile_order, SampleSplit{});
     return hipGetLastError();
 }

@@ -798,9 +798,9 @@ int fpt_hip_plan_tune(FptHipPlan *plan,
  *   refill         each wave parks up to 64 further paths in LDS; lanes whose
  *                  path waits for the per-hit work, or that are idle, exchange
  *                  it for a parked marching path once `refill` lanes of the
- *                  wave are not marching (1..64, default 4);
+ *                  wave are not marching (1..64, default 6);
  *   sample_groups  each pixel's samples of a launch are split into at most
- *                  this many runs of at least 8 samples (1..64, default 4),
+ *                  this many runs of at least 8 samples (1..64, default 8),
  *                  separate work slots that different lanes render at once;
  *                  the run that finishes last folds the per-sample colours
  *                  in sample order.  1 = one slot per pixel.  Applies to

@@ -269,6 +269,6 @@ def test_plan_autotune_picks_a_valid_shape_and_keeps_the_image(spp, candidates):
         before = buf.numpy().copy()
         r = plan.autotune(buf.address, st.address, rounds=1)
         assert r.candidates == candidates and r.best_ms > 0 and r.event_batch == 0 and r.workgroups > 0
-        assert r.tile_order in (0, 1) and r.sample_groups in ((4,) if candidates == 4 else (1, 4))
+        assert r.tile_order in (0, 1) and r.sample_groups in ((8,) if candidates == 4 else (1, 8))
         plan.render_into(buf.address)
         assert np.array_equal(buf.numpy(), before)
