@@ -520,6 +520,22 @@ def run_workload(args, api, scenes):
                             "flops_per_launch": flops, "lower_bound": True}
         line["config"]["counts"] = {k: prof[k] for k in ("distance_evals", "march_orbit_iterations",
                                                          "normal_evals", "primary_steps", "shadow_steps")}
+        # The same program's topology compiled by hipRTC into straight-line
+        # distance and surface code (validated at 2^20 points first), timed
+        # at the autotuned launch shape; the interpreter stays the line's value.
+        with api.Plan(cfg) as splan:
+            t0 = time.perf_counter()
+            spec = splan.specialize()
+            spec_s = time.perf_counter() - t0
+            splan.tune(1, tune.workgroups, tune.event_batch)
+            splan.set_tile_order(tune.tile_order)
+            splan.tune_stash(sample_groups=tune.sample_groups)
+            if args.warmup:
+                api.time_renders(splan, accum, stream, args.warmup)
+            sms = api.time_renders(splan, accum, stream, args.steps) / args.steps
+        line["config"]["specialized"] = {"ms_per_step": sms, "value": samples_per_step / (sms / 1e3),
+                                         "specialize_s": spec_s, "cache_status": spec.get("cache_status"),
+                                         "speedup_over_interpreter": ms / sms}
         if not args.no_cpu_baseline:
             base, _, _, _, _ = oracle_band(cfg, args.cpu_rows, 1)
             line["cpu_baseline"] = base
