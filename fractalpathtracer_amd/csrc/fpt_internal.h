@@ -83,7 +83,7 @@ int default_event_batch(bool stash, bool runs);
 // Non-marching lanes of a wave before they take parked marching paths: 6
 // for single-scene plans (the 1080p Mandelbulb in 8 sample runs: 551 ms at
 // 4 runs and refill 4, 549 at 8 runs, 546 at 8 runs and refill 6), 4 for
-// batches of scenes.
+// batches of scenes (64-scene batch: 747 ms at 4, 752 at 6, 760 at 8).
 constexpr int kDefaultRefill = 6;
 constexpr int kBatchRefill = 4;
 // The scene's program carries flat-union, canonical or typed-SoA tables: the
