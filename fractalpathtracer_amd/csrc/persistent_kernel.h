@@ -95,6 +95,13 @@ __device__ __forceinline__ void lane_stat(unsigned long long *ls, int passes, in
 
 // Wave-level fetch of work slots: one atomic per wave, consecutive indices to
 // the requesting lanes in lane order.  Must be reached by every lane.
+// The lowered tables' normal as a call of its own, outside the persistent
+// kernel's flattened body: the 28-primitive union renders 2.7% faster
+// (same box: 81.8 against 84.0 ms) with the same operations.
+__device__ __noinline__ f3 normal_scene_call(f3 p, const FptRenderConfig &cfg, Work &w) {
+    return normal_scene(p, cfg, w);
+}
+
 __device__ __forceinline__ uint32_t fetch_slot(bool want, uint32_t *counter) {
     const unsigned long long m = __ballot(want);
     uint32_t base = 0;
@@ -728,7 +735,7 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
                     if (PROFILE) lane_stat(ls, LS_NORMAL_PASSES, LS_NORMAL_LANES, true);
                     mat = MB ? material_any<SCENE_MB>(p, cfg, w) : user_material(p, cfg, w);
                     if (mat.emission > 0.001f) light = light + mat.rgb * mat.emission;
-                    hit_n = MB ? normal_mandelbulber(p, cfg, w) : (LOWERED ? normal_scene(p, cfg, w) : normal_at(p, cfg, w));
+                    hit_n = MB ? normal_mandelbulber(p, cfg, w) : (LOWERED ? normal_scene_call(p, cfg, w) : normal_at(p, cfg, w));
                     pc.normal_evals++;
                     if (sun_on) {
                         // Shadow ray (SM:5846-5856).
