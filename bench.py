@@ -464,14 +464,21 @@ def run_workload(args, api, scenes):
         # (6), derivative (1), |z| (6) = 73 FLOP with every plane enabled;
         # per distance evaluation the finaliser (4), threshold (9) and step (8)
         flops = prof["march_orbit_iterations"] * 73 + prof["distance_evals"] * 21
-        launches = one_path_launches(cfg)
         achieved = flops / (ms / 1e3) / 1e12
         line["roofline"] = {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                             "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
-                            "kernel": "sdf_accumulate_kernel", "launches_per_step": launches,
-                            "launch_ms": ms / launches, "flops_per_launch": flops / launches}
+                            "kernel": KERNEL, "launches_per_step": 1,
+                            "launch_ms": ms, "flops_per_launch": flops}
         line["config"]["counts"] = {k: prof[k] for k in ("distance_evals", "march_orbit_iterations",
                                                          "normal_evals", "primary_steps", "shadow_steps")}
+        # The same frame on the one-path-per-lane kernel (plan kernel 0).
+        with api.Plan(cfg) as oplan:
+            oplan.tune(kernel=0)
+            if args.warmup:
+                api.time_renders(oplan, accum, stream, args.warmup)
+            oms = api.time_renders(oplan, accum, stream, args.steps) / args.steps
+        line["config"]["one_path_kernel"] = {"ms_per_step": oms, "value": samples_per_step / (oms / 1e3),
+                                             "launches_per_step": one_path_launches(cfg)}
     elif args.workload == "lowered":
         prof, _ = plan.profile(accum)
         t = cfg.sdf_typed_soa

@@ -529,7 +529,12 @@ hipError_t launch_sdf(FptHipPlan *plan, float4 *acc, uint32_t start, uint32_t ch
     fpt::SampleSplit split;
     hipError_t e = plan_split(plan, start, chunk, split);
     if (e != hipSuccess) return e;
-    uint32_t wg = plan->workgroups ? plan->workgroups : default_workgroups();
+    // Imported Mandelbulber scenes run the stash-free instance: as many
+    // workgroups per compute unit as its waves per SIMD.
+    const uint32_t wg = plan->workgroups ? plan->workgroups
+                        : c.sdf_id == FPT_SDF_MANDELBULBER
+                            ? default_workgroups() / 2u * uint32_t(fpt::kMandelbulberPersistentWaves)
+                            : default_workgroups();
     if (plan->spec && !plan->proposer.enabled && !prof && !fpt::single_bulb_program(c)) {
         // Same kernel, built for the program's topology (counting passes use
         // the interpreter: the work counts are the same).
@@ -662,10 +667,6 @@ int fpt_hip_plan_create(const FptRenderConfig *config, FptHipPlan **plan, char *
     config = &hydrated;
     FptHipPlan *p = new FptHipPlan();
     p->host = *config;
-    // Imported Mandelbulber scenes: the one-path-per-lane kernel is the faster
-    // of the two (ifs-20.fract at 1080p, 8 spp: 616 ms against 915 ms for the
-    // persistent kernel, identical images); kernel 1 stays selectable.
-    if (config->sdf_id == FPT_SDF_MANDELBULBER) p->kernel = 0u;
     hipError_t e = hipMalloc(&p->dcfg, sizeof(FptRenderConfig));
     if (e == hipSuccess) e = hipMemcpy(p->dcfg, config, sizeof(FptRenderConfig), hipMemcpyHostToDevice);
     if (e != hipSuccess) {

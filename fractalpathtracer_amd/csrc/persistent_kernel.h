@@ -200,7 +200,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 #endif
 // Minimum resident waves per SIMD for imported Mandelbulber scenes (MB).
 #ifndef FPT_MANDELBULBER_PERSISTENT_WAVES
-#define FPT_MANDELBULBER_PERSISTENT_WAVES 2
+#define FPT_MANDELBULBER_PERSISTENT_WAVES kMandelbulberPersistentWaves
 #endif
 // Path stash of the single-scene exact instances (on/off).
 #ifndef FPT_PATH_STASH

@@ -119,10 +119,10 @@ def test_profile_counters_match_oracle_work():
 
 
 def test_persistent_and_one_path_kernels_agree():
-    """Imported scenes render on the one-path-per-lane kernel by default (the
-    faster one for them) and on the persistent kernel when a plan asks for
-    it: two independent implementations of the same path, whose images must
-    be bit-identical (cut fish-eye too)."""
+    """Imported scenes render on the persistent kernel by default (8 waves per
+    SIMD, the faster one for them) and on the one-path-per-lane kernel when
+    a plan asks for it: two independent implementations of the same path,
+    whose images must be bit-identical (cut fish-eye too)."""
     for proj, fov in ((0.0, None), (3.0, 200.0)):
         c = _ifs20(48, 27, 3)
         c.vset_values[107] = proj
