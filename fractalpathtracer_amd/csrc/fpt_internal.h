@@ -61,6 +61,13 @@ struct BatchScene {
     unsigned long long accum_offset;
 };
 
+// Waves per SIMD of the persistent instance for imported Mandelbulber
+// scenes (no path stash, so no LDS limit): its launch defaults to this many
+// workgroups per compute unit.  ifs-20.fract at 1080p, 8 spp: 910 ms at 2,
+// 674 at 3, 580 at 4, 543 at 5, 530 at 6, 521 at 7, 508 at 8, against
+// 617 ms for the one-path-per-lane kernel (images identical).
+constexpr int kMandelbulberPersistentWaves = 8;
+
 #ifndef __HIPCC_RTC__  // host launchers
 hipError_t launch_zero_counter(uint32_t *counter, hipStream_t stream);
 uint32_t persistent_slots(uint32_t width, uint32_t height);
@@ -80,12 +87,6 @@ bool batch_uses_stash();
 // as many paths wait as march, larger batches change nothing), 48 for one
 // slot per pixel; 24 lanes without the stash.
 int default_event_batch(bool stash, bool runs);
-// Waves per SIMD of the persistent instance for imported Mandelbulber
-// scenes (no path stash, so no LDS limit): its launch defaults to this many
-// workgroups per compute unit.  ifs-20.fract at 1080p, 8 spp: 910 ms at 2,
-// 674 at 3, 580 at 4, 543 at 5, 530 at 6, 521 at 7, 508 at 8, against
-// 617 ms for the one-path-per-lane kernel (images identical).
-constexpr int kMandelbulberPersistentWaves = 8;
 // Non-marching lanes of a wave before they take parked marching paths: 6
 // for single-scene plans (the 1080p Mandelbulb in 8 sample runs: 551 ms at
 // 4 runs and refill 4, 549 at 8 runs, 546 at 8 runs and refill 6), 4 for
