@@ -479,8 +479,8 @@ constexpr uint32_t kMinSampleRun = 8u;
 
 // Sample runs of a launch over samples [start, start + chunk) of the frame:
 // runs of ceil(n / groups) samples, every run non-empty and at least
-// kMinSampleRun long.  Single-scene exact-march plans only (no proposer, no
-// imported Mandelbulber scene), frames whose sides are multiples of 8, and
+// kMinSampleRun long.  Single-scene exact-march plans only (no proposer;
+// imported Mandelbulber scenes included), frames whose sides are multiples of 8, and
 // only while the per-sample colours fit in 8 GB; otherwise one slot per
 // pixel.  Allocates the colour and counter
 // buffers on first use.
@@ -489,7 +489,7 @@ hipError_t plan_split(FptHipPlan *plan, uint32_t start, uint32_t chunk, fpt::Sam
     const FptRenderConfig &c = plan->host;
     const uint32_t spp = clamp_samples(c.samples);
     const uint32_t n = std::min(start + chunk, spp) > start ? std::min(start + chunk, spp) - start : 0u;
-    if (plan->sample_groups <= 1u || n <= 1u || plan->proposer.enabled || c.sdf_id == FPT_SDF_MANDELBULBER)
+    if (plan->sample_groups <= 1u || n <= 1u || plan->proposer.enabled)
         return hipSuccess;
     // Frames whose edge tiles carry padding render one slot per pixel: sample
     // runs are verified only on frames that tile exactly into 8x8.

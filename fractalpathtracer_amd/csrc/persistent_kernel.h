@@ -292,7 +292,7 @@ void sdf_persistent_kernel(const FptRenderConfig *__restrict__ cfgs,
     }
 
     // Sample groups (single scene only): a slot is a run of a pixel's samples.
-    const bool SPLIT = !MULTI && !LEARNED && !MB && split.groups > 1u;
+    const bool SPLIT = !MULTI && !LEARNED && split.groups > 1u;
 
     // Lane state.
     int phase = PH_IDLE;

@@ -22,7 +22,7 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-from fractalpathtracer_amd import api, scenes
+from fractalpathtracer_amd import api, ffi, scenes
 from kifs_sphere import LIMIT, R, sphere_cfg
 from oracle import bind
 
@@ -137,3 +137,24 @@ def test_persistent_and_one_path_kernels_agree():
                 imgs.append(buf.numpy().reshape(27, 48, 4))
         assert np.array_equal(imgs[0], imgs[1]), proj
         assert np.array_equal(imgs[0], api.render(c))
+
+
+def test_sample_runs_on_imported_scenes():
+    """The persistent kernel's Mandelbulber instance in sample runs (an
+    8-aligned frame, 20 and 37 samples split into runs) renders the same
+    image as the one-path-per-lane kernel, chunked launches included."""
+    for spp, chunked in ((20, False), (37, True)):
+        c = _ifs20(40, 32, spp)
+        if chunked:
+            c.sdf_accumulation_mode = ffi.SDF_ACCUMULATION_CHUNKED
+            c.sdf_chunk_samples = 17
+        imgs = []
+        for kernel, runs in ((0, 1), (1, 8), (1, 3)):
+            with api.Plan(c) as plan:
+                plan.tune(kernel=kernel)
+                plan.tune_stash(6, runs)
+                buf = api.DeviceBuffer(40 * 32 * 4)
+                plan.render_into(buf.address)
+                imgs.append(buf.numpy().copy())
+        assert np.array_equal(imgs[0], imgs[1]), spp
+        assert np.array_equal(imgs[0], imgs[2]), spp
