@@ -87,6 +87,10 @@ bool batch_uses_stash();
 // as many paths wait as march, larger batches change nothing), 48 for one
 // slot per pixel; 24 lanes without the stash.
 int default_event_batch(bool stash, bool runs);
+// Event batch of the imported-Mandelbulber instance (no stash, 8 waves per
+// SIMD; ifs-20 at 1080p, 64 spp in sample runs: 3,634 ms at 24, 3,269 at
+// 32, 3,144 at 40, 3,190 at 48, 3,484 at 56).
+constexpr int kMandelbulberEventBatch = 40;
 // Non-marching lanes of a wave before they take parked marching paths: 6
 // for single-scene plans (the 1080p Mandelbulb in 8 sample runs: 551 ms at
 // 4 runs and refill 4, 549 at 8 runs, 546 at 8 runs and refill 6), 4 for

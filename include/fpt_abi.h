@@ -781,8 +781,9 @@ void fpt_hip_plan_destroy(FptHipPlan *plan);
  *                once as many paths wait as march; the others count lanes
  *                (values above 64 then wait for every march to end).  0 =
  *                the instance's default: with parked paths 64 for a launch
- *                in sample runs (below) and 48 otherwise, 24 without
- *                (fpt_hip_batch_tune: 64 and 24).
+ *                in sample runs (below) and 48 otherwise, 24 without, 40
+ *                for imported Mandelbulber scenes (fpt_hip_batch_tune: 64
+ *                and 24).
  */
 int fpt_hip_plan_tune(FptHipPlan *plan,
                       uint32_t kernel,
