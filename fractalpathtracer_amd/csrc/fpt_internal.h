@@ -91,6 +91,10 @@ int default_event_batch(bool stash, bool runs);
 // SIMD; ifs-20 at 1080p, 64 spp in sample runs: 3,634 ms at 24, 3,269 at
 // 32, 3,144 at 40, 3,190 at 48, 3,484 at 56).
 constexpr int kMandelbulberEventBatch = 40;
+// Event batch of the learned-step instance (no stash; 1080p Mandelbulb,
+// 64 spp: 2,386 ms at 8, 2,274 at 16, 2,303 at 20, 2,378 at 24, 2,855 at
+// 32).
+constexpr int kLearnedEventBatch = 16;
 // Non-marching lanes of a wave before they take parked marching paths: 6
 // for single-scene plans (the 1080p Mandelbulb in 8 sample runs: 551 ms at
 // 4 runs and refill 4, 549 at 8 runs, 546 at 8 runs and refill 6), 4 for

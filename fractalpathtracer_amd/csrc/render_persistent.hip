@@ -84,9 +84,10 @@ hipError_t launch_sdf_persistent(const FptRenderConfig *dcfg, const FptRenderCon
         hipLaunchKernelGGL(zero_words_kernel, dim3((npix + 255u) / 256u), dim3(256), 0, stream, split.done, npix);
     }
     if (event_batch <= 0)
-        event_batch = hcfg.sdf_id == FPT_SDF_MANDELBULBER && !prop
+        event_batch = prop ? kLearnedEventBatch
+                      : hcfg.sdf_id == FPT_SDF_MANDELBULBER
                           ? kMandelbulberEventBatch
-                          : default_event_batch(persistent_uses_stash(hcfg, prop != nullptr), split.groups > 1u);
+                          : default_event_batch(persistent_uses_stash(hcfg, false), split.groups > 1u);
     hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, stream, counter);
     const ProposerDevice none{};
     const dim3 grid(workgroups);

@@ -782,8 +782,8 @@ void fpt_hip_plan_destroy(FptHipPlan *plan);
  *                (values above 64 then wait for every march to end).  0 =
  *                the instance's default: with parked paths 64 for a launch
  *                in sample runs (below) and 48 otherwise, 24 without, 40
- *                for imported Mandelbulber scenes (fpt_hip_batch_tune: 64
- *                and 24).
+ *                for imported Mandelbulber scenes, 16 for the learned march
+ *                (fpt_hip_batch_tune: 64 and 24).
  */
 int fpt_hip_plan_tune(FptHipPlan *plan,
                       uint32_t kernel,
